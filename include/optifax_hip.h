@@ -1,0 +1,160 @@
+/*
+ * optifax_hip.h -- C ABI of libofx_hip, the MI355X (gfx950) implementation of
+ * OptiFax's adaptive split-step chi(2) crystal solver and OPO round-trip loop.
+ *
+ * Plain C types only.  Two layers:
+ *
+ *  1. ofx_crystal_pass_host(): one crystal pass over a batch of fields held
+ *     in host memory.  It is the computation behind the reference's MEX
+ *     entry point
+ *        [Et,ApFT,stepMods] = OPOmexBatch(Et,alpha_w,G,w0,bdiffw0,h,NSteps,dt,
+ *                                         beta_op,maxErr,minErr,sel,ApFT,stepMods)
+ *     (toolbox/lib/MEXlib/src/OPO_mex_BATCH.cpp:19-141, which calls
+ *      OPO_TEST_CUDA, OPO_CUDA_BATCH_XTAL.cu:373-525, declared in
+ *      OPO_BATCH_HEADER.hpp:8-30).  The MEX gateway in
+ *     optifax_amd/csrc/mex_gateway.cpp (include/optifax_mex.h) is a thin
+ *     wrapper over it.
+ *
+ *  2. ofx_opo_*: a device-resident batch of OPO instances that runs whole
+ *     round trips (crystal pass + output coupling + cavity detuning + pump)
+ *     inside one persistent kernel.  It replaces the MATLAB loop
+ *     OpticalSim.run (toolbox/lib/classes/OpticalSim.m:207-301) plus its per
+ *     trip solver call (:237-252); there is no C counterpart in the reference.
+ *
+ * Fields are interleaved complex (re, im) arrays of length 2*N, time samples
+ * in fftshifted order (t = 0 at index 0), exactly as OpticalSim passes EtShift
+ * (OpticalSim.m:234).  Spectral tables are in FFT order (OpticalSim.m:223).
+ *
+ * All functions return OFX_OK (0) or a negative OFX_E_* code; the message of
+ * the last failure is available from ofx_last_error().
+ */
+#ifndef OPTIFAX_HIP_H
+#define OPTIFAX_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OFX_OK 0
+#define OFX_E_ARG (-1)          /* invalid argument (shape, size, value)          */
+#define OFX_E_HIP (-2)          /* HIP runtime error                              */
+#define OFX_E_NOMEM (-3)        /* device or host allocation failed               */
+#define OFX_E_NONFINITE (-4)    /* an instance produced NaN/Inf (see statuses)    */
+#define OFX_E_TIMEOUT (-5)      /* a cluster barrier timed out (see statuses)     */
+#define OFX_E_UNSUPPORTED (-6)  /* option not implemented on this device          */
+#define OFX_E_NODEVICE (-7)     /* no HIP device                                  */
+
+/* per-instance status after a run */
+#define OFX_STATUS_RUNNING 0
+#define OFX_STATUS_DONE 1
+#define OFX_STATUS_CONVERGED 2
+#define OFX_STATUS_NONFINITE 3
+#define OFX_STATUS_TIMEOUT 4
+
+#define OFX_F32 0 /* complex64 fields, FP64 phase arguments and norms  */
+#define OFX_F64 1 /* complex128 fields throughout                       */
+
+#define OFX_METHOD_HEUN 0 /* adaptive Heun, NEE_CPU.m:39-94 semantics         */
+#define OFX_METHOD_RK23 1 /* Bogacki-Shampine, ode23/OPO_CUDA_ODE23.cu:375-557 */
+
+typedef struct ofx_context ofx_context;
+typedef struct ofx_opo ofx_opo;
+
+/* Scalars of one crystal pass: arguments 4-8 and 10-12 of OPOmexBatch
+ * (OPO_mex_BATCH.cpp:45-52) plus the derived sizes (:71-72). */
+typedef struct {
+  int n;          /* points N = numel(beta_op), power of two, 64..65536  */
+  int nsteps;     /* NSteps = floor(L/h)                                  */
+  double w0;      /* reference angular frequency [rad/s]                  */
+  double bdiffw0; /* beta0 - beta1*w0 [1/m]                               */
+  double h;       /* minimal step [m]                                     */
+  double dt;      /* time step [s]                                        */
+  double max_err; /* reject threshold, percent                            */
+  double min_err; /* grow threshold, percent                              */
+  double sel;     /* minimal steps per ApFT chunk, > 1                    */
+  int n_cols;     /* ApFT columns (ProgressPlots-1); 0 = do not write     */
+  int method;     /* OFX_METHOD_*                                         */
+} ofx_pass_params;
+
+/* Optional launch geometry.  Zero fields mean "autotuned/default". */
+typedef struct {
+  int cluster;    /* workgroups per instance (power of two)  */
+  int n1;         /* four-step split N = n1*n2 (power of two) */
+  int block;      /* threads per workgroup (64..512)          */
+} ofx_tuning;
+
+int ofx_create(int device, ofx_context** out);
+void ofx_destroy(ofx_context* ctx);
+const char* ofx_last_error(const ofx_context* ctx);
+/* device name, compute units, wall-clock rate (kHz) */
+int ofx_device_info(const ofx_context* ctx, char* name, int name_len, int* cus, int* wallclock_khz);
+
+/*
+ * One crystal pass over `batch` fields in host memory (OPOmexBatch semantics,
+ * Heun step control of NEE_CPU.m:39-94 applied independently per instance).
+ *  Et       [2*n*batch] in/out, complex64 interleaved, fftshifted time
+ *  alpha_w  [n]          T^(1/NSteps) in FFT order (NonlinearCrystal.m:141-144)
+ *  G        [nsteps*g_batch] coupling per minimal step; g_batch 1 (shared) or batch
+ *  beta_op  [n]          phi_rel/L*h in FFT order (OpticalSim.m:223)
+ *  ApFT     [2*n*n_cols*batch] or NULL: spectrum at each chunk end,
+ *           instance b column c at offset 2*n*(b*n_cols + c)
+ *  stepMods [nsteps*batch] or NULL: stepmod accepted at each 1-based k (k-1)
+ *  status   [batch] or NULL: OFX_STATUS_* per instance
+ */
+int ofx_crystal_pass_host(ofx_context* ctx, const ofx_pass_params* p, int batch, float* Et,
+                          const float* alpha_w, const float* G, int g_batch,
+                          const float* beta_op, float* ApFT, float* stepMods, int* status,
+                          const ofx_tuning* tune);
+
+/* Same, FP64 fields (complex128) and FP64 tables; used for exact parity. */
+int ofx_crystal_pass_host_f64(ofx_context* ctx, const ofx_pass_params* p, int batch, double* Et,
+                              const double* alpha_w, const double* G, int g_batch,
+                              const double* beta_op, double* ApFT, double* stepMods,
+                              int* status, const ofx_tuning* tune);
+
+/* ---------------- device-resident OPO batch ---------------- */
+
+/* One OPO instance.  All host arrays are FP64 and are copied (converted to
+ * the batch precision) at ofx_opo_create; FFT order for spectra. */
+typedef struct {
+  ofx_pass_params pass;
+  double delay;          /* cavity detuning per trip [s] (applyGD, OpticalPulse.m:256-261) */
+  const double* beta;    /* [n]   phi_rel/L*h                                  */
+  const double* alpha;   /* [n]   T^(1/NSteps)                                 */
+  const double* G;       /* [nsteps]                                           */
+  const double* field;   /* [2n]  crystal input of the first trip (fftshifted) */
+  const double* pump;    /* [2n]  pump spectrum in the crystal, or NULL        */
+  const double* cav;     /* [2n]  crystal-out -> next crystal-in transfer, or NULL (single passes) */
+  const double* ocm;     /* [2n]  crystal-out -> output-coupled spectrum, or NULL */
+} ofx_opo_spec;
+
+int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_spec* specs,
+                   int trip_cap, const ofx_tuning* tune, ofx_opo** out);
+void ofx_opo_destroy(ofx_opo* run);
+/* enqueue `trips` round trips (0 = one plain crystal pass) on the context
+ * stream; instances stop early once the trip-to-trip relative L2 change of
+ * the output-coupled power spectrum is below conv_tol (<= 0: never) */
+int ofx_opo_launch(ofx_opo* run, int trips, double conv_tol);
+int ofx_opo_sync(ofx_opo* run);
+/* capture `trips` round trips as a hipGraph (once) and replay it `replays` times */
+int ofx_opo_graph_replay(ofx_opo* run, int trips, double conv_tol, int replays);
+/* time of the last ofx_opo_launch / replay span on the stream, milliseconds */
+int ofx_opo_last_ms(ofx_opo* run, double* ms);
+/* read back one instance (any pointer may be NULL).
+ *  field [2n], oc_spec [2n], trip_log [4*trip_cap] (conv, oc energy, attempts, accepted),
+ *  apft [2n*n_cols], stepmods [nsteps], counters [2] (attempts, accepted) */
+int ofx_opo_read(ofx_opo* run, int inst, double* field, double* oc_spec, double* trip_log,
+                 int* status, int* trips_done, long long* counters, double* apft,
+                 double* stepmods);
+/* overwrite the crystal-input field of one instance (warm start / checkpoint restore) */
+int ofx_opo_write_field(ofx_opo* run, int inst, const double* field);
+/* geometry actually used: cluster size, n1, block, grid, LDS bytes */
+int ofx_opo_geometry(const ofx_opo* run, int inst, int* cluster, int* n1, int* block, int* grid,
+                     int* lds_bytes);
+/* hardware XCC id of every launched workgroup of the last launch (diagnostic) */
+int ofx_opo_xcc_ids(ofx_opo* run, int* ids, int cap, int* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OPTIFAX_HIP_H */

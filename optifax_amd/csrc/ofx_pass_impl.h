@@ -1,0 +1,789 @@
+// Body of the persistent cluster kernel that runs adaptive split-step crystal
+// passes (and, in round-trip mode, whole OPO round trips) for a batch of
+// instances.
+//
+// Work decomposition (one instance, N = n1 * n2 points, C workgroups):
+//   time domain   : point j = c + n1*r. Workgroup `rank` owns the W = n1/C
+//                   columns c in [rank*W, rank*W+W) plus `halo` columns on each
+//                   side (halo copies are recomputed redundantly, so the
+//                   nonlinear stages need no communication).
+//   frequency     : bin k = k2 + n2*k1. Workgroup `rank` owns the R = n2/C
+//                   rows k2 in [rank*R, rank*R+R), all k1.
+//   one FFT       : four-step (Bailey 1990): length-n2 transforms of the own
+//                   columns in LDS, twiddle, global transpose (buffer t1),
+//                   length-n1 transforms of the own rows in LDS.  The inverse
+//                   mirrors it through buffer t2.  Each transform is a radix-4
+//                   Stockham autosort FFT (with one radix-2 pass for odd log2).
+//   one accepted step = 2 cluster barriers (t1 and t2 transposes); the error
+//                   max-norm travels with the t1 barrier, so a rejected step
+//                   costs 1 barrier.  Step control is replicated in every
+//                   workgroup from identical inputs, so no broadcast is needed.
+//
+// This header compiles as HIP device code (ofx_kernels.hip) and, with OFX_EMU
+// defined, as plain C++ where each workgroup is one CPU thread; the emulation
+// build exists only to test indexing and control flow on a machine without a
+// GPU.
+#pragma once
+#include "ofx_internal.h"
+#include <math.h>
+
+#ifdef OFX_EMU
+#include <cmath>
+#define OFX_DEV inline
+#define OFX_TID 0
+#define OFX_NT 1
+#define OFX_SYNC() ((void)0)
+namespace ofx {
+bool emu_cluster_barrier(const ClusterDesc* cd, unsigned target);
+}
+#else
+#define OFX_DEV __device__ __forceinline__
+#define OFX_TID ((int)threadIdx.x)
+#define OFX_NT ((int)blockDim.x)
+#define OFX_SYNC() __syncthreads()
+#endif
+
+namespace ofx {
+
+// Hardware register holding the accelerator-complex-die id of the executing
+// wave on gfx940-class parts (HW_REG_XCC_ID, bits [3:0]).
+#define OFX_HWREG_XCC_ID ((20) | (0 << 6) | ((4 - 1) << 11))
+
+template <typename R>
+struct alignas(2 * sizeof(R)) cpx {
+  R x, y;
+};
+
+template <typename R>
+OFX_DEV cpx<R> mkc(R a, R b) {
+  cpx<R> r;
+  r.x = a;
+  r.y = b;
+  return r;
+}
+template <typename R>
+OFX_DEV cpx<R> cadd(cpx<R> a, cpx<R> b) { return mkc<R>(a.x + b.x, a.y + b.y); }
+template <typename R>
+OFX_DEV cpx<R> csub(cpx<R> a, cpx<R> b) { return mkc<R>(a.x - b.x, a.y - b.y); }
+template <typename R>
+OFX_DEV cpx<R> cmul(cpx<R> a, cpx<R> b) {
+  return mkc<R>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+template <typename R>
+OFX_DEV cpx<R> cmulc(cpx<R> a, cpx<R> b) {
+  return mkc<R>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+template <typename R>
+OFX_DEV cpx<R> cscale(cpx<R> a, R s) { return mkc<R>(a.x * s, a.y * s); }
+
+// x - 2*pi*round(x/(2*pi)) with a two-term (Cody-Waite) split of 2*pi; the
+// fma makes n*TWO_PI_HI exact, so the reduction is accurate for |x| < 2^40.
+OFX_DEV double reduce_2pi(double x) {
+  const double TWO_PI_HI = 6.283185307179586232;
+  const double TWO_PI_LO = 2.4492935982947064e-16;
+  const double INV_TWO_PI = 0.15915494309189533577;
+  double q = rint(x * INV_TWO_PI);
+  double r = fma(-q, TWO_PI_HI, x);
+  return fma(-q, TWO_PI_LO, r);
+}
+
+// exp(i*x) for an FP64 argument: reduced in FP64, evaluated in R.
+OFX_DEV cpx<float> expi(double x, float) {
+  float r = (float)reduce_2pi(x);
+#ifdef OFX_EMU
+  return mkc<float>(cosf(r), sinf(r));
+#else
+  float s, c;
+  sincosf(r, &s, &c);
+  return mkc<float>(c, s);
+#endif
+}
+OFX_DEV cpx<double> expi(double x, double) {
+  double r = reduce_2pi(x);
+#ifdef OFX_EMU
+  return mkc<double>(cos(r), sin(r));
+#else
+  double s, c;
+  sincos(r, &s, &c);
+  return mkc<double>(c, s);
+#endif
+}
+
+// Nonlinear polarisation term of the NEE (NEE_CPU.m:98-101):
+//   nl = A^2 e^{i th} + 2 |A|^2 e^{-i th}
+template <typename R>
+OFX_DEV cpx<R> nl_term(cpx<R> A, double th) {
+  cpx<R> e = expi(th, R(0));
+  R a2 = A.x * A.x + A.y * A.y;
+  R sx = A.x * A.x - A.y * A.y;
+  R sy = R(2) * A.x * A.y;
+  return mkc<R>(sx * e.x - sy * e.y + R(2) * a2 * e.x, sx * e.y + sy * e.x - R(2) * a2 * e.y);
+}
+
+// same with the unit phasor e = e^{i th} given
+template <typename R>
+OFX_DEV cpx<R> nl_term_e(cpx<R> A, cpx<R> e) {
+  R a2 = A.x * A.x + A.y * A.y;
+  R sx = A.x * A.x - A.y * A.y;
+  R sy = R(2) * A.x * A.y;
+  return mkc<R>(sx * e.x - sy * e.y + R(2) * a2 * e.x, sx * e.y + sy * e.x - R(2) * a2 * e.y);
+}
+
+// K = -i*g*(nl - i*cc*(up - dn)),  g = h*s*G (NEE_CPU.m:103-105)
+template <typename R>
+OFX_DEV cpx<R> k_term(cpx<R> nl, cpx<R> up, cpx<R> dn, R g, R cc) {
+  R dx = up.x - dn.x, dy = up.y - dn.y;
+  R ix = nl.x + cc * dy;
+  R iy = nl.y - cc * dx;
+  return mkc<R>(g * iy, -g * ix);
+}
+
+// ------------------------------------------------------------------------
+// Batched Stockham FFT in LDS.  nseq sequences of length L = 2^lg, element e
+// of sequence q at buf[q*ss + e].  Ping-pongs between `in` and `out`; returns
+// the buffer that holds the result.  SIGN = -1 forward (unnormalised),
+// +1 inverse (unnormalised).  tw[m] = exp(-2 pi i m / NMAX), tws = NMAX / L.
+// ------------------------------------------------------------------------
+template <typename R, int SIGN>
+OFX_DEV cpx<R>* lds_fft(cpx<R>* in, cpx<R>* out, int lg, int nseq, int ss, const cpx<R>* tw,
+                        int tws) {
+  typedef cpx<R> C;
+  const int L = 1 << lg;
+  const int tid = OFX_TID, nt = OFX_NT;
+  int Ns = 1;
+  if (lg & 1) {
+    const int h = L >> 1;
+    const int nb = nseq << (lg - 1);
+    for (int t = tid; t < nb; t += nt) {
+      const int q = t >> (lg - 1), j = t & (h - 1);
+      const C* ib = in + q * ss;
+      C* ob = out + q * ss;
+      C v0 = ib[j], v1 = ib[j + h];
+      ob[2 * j] = cadd(v0, v1);
+      ob[2 * j + 1] = csub(v0, v1);
+    }
+    OFX_SYNC();
+    C* tmp = in;
+    in = out;
+    out = tmp;
+    Ns = 2;
+  }
+  if (lg >= 2) {
+    const int q4 = L >> 2;
+    const int nb = nseq << (lg - 2);
+    for (; Ns < L; Ns <<= 2) {
+      const int span = (L / (4 * Ns)) * tws;
+      for (int t = tid; t < nb; t += nt) {
+        const int q = t >> (lg - 2), j = t & (q4 - 1);
+        const int k = j & (Ns - 1);
+        const C* ib = in + q * ss;
+        C* ob = out + q * ss;
+        C v0 = ib[j], v1 = ib[j + q4], v2 = ib[j + 2 * q4], v3 = ib[j + 3 * q4];
+        if (Ns > 1) {
+          const int m = k * span;
+          C w1 = tw[m], w2 = tw[2 * m], w3 = tw[3 * m];
+          if (SIGN > 0) {
+            v1 = cmulc(v1, w1);
+            v2 = cmulc(v2, w2);
+            v3 = cmulc(v3, w3);
+          } else {
+            v1 = cmul(v1, w1);
+            v2 = cmul(v2, w2);
+            v3 = cmul(v3, w3);
+          }
+        }
+        C s02 = cadd(v0, v2), d02 = csub(v0, v2), s13 = cadd(v1, v3), d13 = csub(v1, v3);
+        // forward multiplies d13 by -i, inverse by +i
+        C rd = (SIGN < 0) ? mkc<R>(d13.y, -d13.x) : mkc<R>(-d13.y, d13.x);
+        const int base = ((j - k) << 2) + k;
+        ob[base] = cadd(s02, s13);
+        ob[base + Ns] = cadd(d02, rd);
+        ob[base + 2 * Ns] = csub(s02, s13);
+        ob[base + 3 * Ns] = csub(d02, rd);
+      }
+      OFX_SYNC();
+      C* tmp = in;
+      in = out;
+      out = tmp;
+    }
+  }
+  return in;
+}
+
+// ------------------------------------------------------------------------
+// Workgroup reductions (all threads receive the result).
+// ------------------------------------------------------------------------
+OFX_DEV double wg_max(double v, double* red) {
+#ifdef OFX_EMU
+  (void)red;
+  return v;
+#else
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double r = red[0];
+  for (int i = 1; i < nw; i++) r = fmax(r, red[i]);
+  return r;
+#endif
+}
+
+// sums three values; fixed order so every workgroup rounds identically
+OFX_DEV void wg_sum3(double* v, double* red) {
+#ifdef OFX_EMU
+  (void)v;
+  (void)red;
+#else
+  for (int i = 0; i < 3; i++)
+    for (int o = 32; o > 0; o >>= 1) v[i] += __shfl_xor(v[i], o);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  __syncthreads();
+  if (lane == 0) {
+    red[3 * w] = v[0];
+    red[3 * w + 1] = v[1];
+    red[3 * w + 2] = v[2];
+  }
+  __syncthreads();
+  for (int i = 0; i < 3; i++) {
+    double s = red[i];
+    for (int q = 1; q < nw; q++) s += red[3 * q + i];
+    v[i] = s;
+  }
+#endif
+}
+
+// ------------------------------------------------------------------------
+// Cluster exchange and barrier.
+//
+// Workgroups of a cluster exchange data only through the transpose buffers
+// t1/t2 and the per-rank slots, and synchronise on a monotonic counter
+// (target = C * barriers so far).  No agent-scope fence is used: on gfx950 it
+// writes back and invalidates the XCD's whole L2, and those flushes serialise
+// across the cluster (measured 3-13 us per barrier for 8-32 workgroups).
+// Instead the exchanged words are written and read with device-scope relaxed
+// atomics (sc1 accesses, which bypass the non-coherent caches), each wave
+// waits for its stores to complete before the workgroup barrier, and one
+// thread bumps and polls the counter (about 3.3 us per barrier, flat in C, on
+// one XCD or spread over all eight).  A variant that relied on the shared L2
+// of one XCC (plain accesses plus an L1 invalidate) was faster but returned
+// stale exchange data in some geometries, so it is not used.
+// The barrier returns false when the cluster was aborted (a member timed out).
+// ------------------------------------------------------------------------
+template <typename R>
+OFX_DEV void xst(cpx<R>* p, cpx<R> v, int coh) {
+#ifdef OFX_EMU
+  (void)coh;
+  *p = v;
+#else
+  if (!coh) {
+    *p = v;
+    return;
+  }
+  static_assert(sizeof(cpx<R>) % 8 == 0, "complex must be a multiple of 8 bytes");
+  union {
+    cpx<R> c;
+    unsigned long long u[sizeof(cpx<R>) / 8];
+  } x;
+  x.c = v;
+  unsigned long long* q = reinterpret_cast<unsigned long long*>(p);
+  for (int i = 0; i < (int)(sizeof(cpx<R>) / 8); i++)
+    __hip_atomic_store(q + i, x.u[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#endif
+}
+
+template <typename R>
+OFX_DEV cpx<R> xld(const cpx<R>* p, int coh) {
+#ifdef OFX_EMU
+  (void)coh;
+  return *p;
+#else
+  if (!coh) return *p;
+  union {
+    cpx<R> c;
+    unsigned long long u[sizeof(cpx<R>) / 8];
+  } x;
+  unsigned long long* q = const_cast<unsigned long long*>(reinterpret_cast<const unsigned long long*>(p));
+  for (int i = 0; i < (int)(sizeof(cpx<R>) / 8); i++)
+    x.u[i] = __hip_atomic_load(q + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return x.c;
+#endif
+}
+
+OFX_DEV void xstd(double* p, double v, int coh) {
+#ifdef OFX_EMU
+  (void)coh;
+  *p = v;
+#else
+  if (!coh) {
+    *p = v;
+    return;
+  }
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#endif
+}
+
+OFX_DEV double xldd(const double* p, int coh) {
+#ifdef OFX_EMU
+  (void)coh;
+  return *p;
+#else
+  if (!coh) return *p;
+  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#endif
+}
+
+OFX_DEV bool cluster_barrier(const ClusterDesc& cd, unsigned target, unsigned long long timeout,
+                             int* sflag, int coh) {
+#ifdef OFX_EMU
+  (void)timeout;
+  (void)sflag;
+  (void)coh;
+  return emu_cluster_barrier(&cd, target);
+#else
+  // all of this wave's global stores have completed (reached L2, or memory for
+  // device-scope stores); the memory clobber keeps the compiler from moving
+  // any access across it
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    if (cd.C > 1) {
+      __hip_atomic_fetch_add(cd.bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned long long t0 = wall_clock64();
+      while (__hip_atomic_load(cd.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        if (__hip_atomic_load(cd.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+          ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        if (wall_clock64() - t0 > timeout) {
+          __hip_atomic_store(cd.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ok = 0;
+          break;
+        }
+      }
+    }
+    sflag[0] = ok;
+  }
+  __syncthreads();
+  return sflag[0] != 0;
+#endif
+}
+
+// ------------------------------------------------------------------------
+// Per-instance state and geometry held by one workgroup.
+// ------------------------------------------------------------------------
+template <typename R>
+struct Geo {
+  int n, n1, n2, lg1, lg2, lgn;
+  int Cn, lgC, rank, W, lgW, Rr, lgR, HW, LC, sT, sF, c0, k20;
+  int coh;  // exchange accesses: 1 device-scope atomics (always used on the GPU)
+  cpx<R>*B0, *B1, *B2;
+};
+
+template <typename R>
+OFX_DEV int gidx(const Geo<R>& g, int lc, int r) {
+  return ((g.c0 - g.HW + lc) + g.n1 * r) & (g.n - 1);
+}
+
+// relative angular frequency of FFT-order bin k
+OFX_DEV double dw_of(int k, int n, double dt) {
+  const double TWO_PI = 6.283185307179586232;
+  return TWO_PI * (double)(k < n / 2 ? k : k - n) / ((double)n * dt);
+}
+
+// Forward transform of the own time columns held in `src` (layout lc*sT + r,
+// own columns lc in [HW, HW+W)) into t1 (row-major k2*n1 + c, twiddled).
+template <typename R>
+OFX_DEV void fwd_cols_to_t1(const Geo<R>& g, cpx<R>* src, cpx<R>* other, cpx<R>* t1,
+                            const cpx<R>* tw, int nmax) {
+  typedef cpx<R> C;
+  C* res = lds_fft<R, -1>(src + g.HW * g.sT, other + g.HW * g.sT, g.lg2, g.W, g.sT, tw,
+                          nmax >> g.lg2);
+  const int tws = nmax >> g.lgn;
+  const int tot = g.W * g.n2;
+  for (int idx = OFX_TID; idx < tot; idx += OFX_NT) {
+    const int cl = idx & (g.W - 1), k2 = idx >> g.lgW;
+    const int c = g.c0 + cl;
+    C v = res[cl * g.sT + k2];
+    v = cmul(v, tw[(c * k2) * tws]);
+    xst(&t1[k2 * g.n1 + c], v, g.coh);
+  }
+}
+
+// Own frequency rows from t1 into `dst` (layout k2l*sF + c), then length-n1
+// forward transforms; returns the buffer holding bins (k2l*sF + k1).
+template <typename R>
+OFX_DEV cpx<R>* t1_to_rows_fwd(const Geo<R>& g, const cpx<R>* t1, cpx<R>* dst, cpx<R>* other,
+                               const cpx<R>* tw, int nmax) {
+  const int tot = g.Rr * g.n1;
+  for (int idx = OFX_TID; idx < tot; idx += OFX_NT) {
+    const int k2l = idx >> g.lg1, c = idx & (g.n1 - 1);
+    dst[k2l * g.sF + c] = xld(&t1[(g.k20 + k2l) * g.n1 + c], g.coh);
+  }
+  OFX_SYNC();
+  return lds_fft<R, -1>(dst, other, g.lg1, g.Rr, g.sF, tw, nmax >> g.lg1);
+}
+
+// Inverse length-n1 transforms of the own rows, conjugate twiddle, write t2
+// (column-major c*n2 + k2).
+template <typename R>
+OFX_DEV void rows_inv_to_t2(const Geo<R>& g, cpx<R>* src, cpx<R>* other, cpx<R>* t2,
+                            const cpx<R>* tw, int nmax) {
+  typedef cpx<R> C;
+  C* res = lds_fft<R, +1>(src, other, g.lg1, g.Rr, g.sF, tw, nmax >> g.lg1);
+  const int tws = nmax >> g.lgn;
+  const int tot = g.Rr * g.n1;
+  for (int idx = OFX_TID; idx < tot; idx += OFX_NT) {
+    const int k2l = idx & (g.Rr - 1), c = idx >> g.lgR;
+    const int k2 = g.k20 + k2l;
+    C v = res[k2l * g.sF + c];
+    v = cmulc(v, tw[(c * k2) * tws]);
+    xst(&t2[c * g.n2 + k2], v, g.coh);
+  }
+}
+
+// Own+halo columns from t2, inverse length-n2 transforms, scale 1/N and
+// place into A0 (B0) with the row shift that wrapped halo columns carry.
+template <typename R>
+OFX_DEV void t2_to_cols_inv(const Geo<R>& g, const cpx<R>* t2, const cpx<R>* tw, int nmax) {
+  typedef cpx<R> C;
+  const int tot = g.LC * g.n2;
+  for (int idx = OFX_TID; idx < tot; idx += OFX_NT) {
+    const int lc = idx >> g.lg2, k2 = idx & (g.n2 - 1);
+    int gc = g.c0 - g.HW + lc;
+    gc = gc < 0 ? gc + g.n1 : (gc >= g.n1 ? gc - g.n1 : gc);
+    g.B1[lc * g.sT + k2] = xld(&t2[gc * g.n2 + k2], g.coh);
+  }
+  OFX_SYNC();
+  C* res = lds_fft<R, +1>(g.B1, g.B2, g.lg2, g.LC, g.sT, tw, nmax >> g.lg2);
+  const R inv = R(1) / R(g.n);
+  for (int idx = OFX_TID; idx < tot; idx += OFX_NT) {
+    const int lc = idx >> g.lg2, rp = idx & (g.n2 - 1);
+    const int gc = g.c0 - g.HW + lc;
+    const int sh = gc < 0 ? 1 : (gc >= g.n1 ? -1 : 0);
+    const int r = (rp + sh) & (g.n2 - 1);
+    g.B0[lc * g.sT + r] = cscale(res[lc * g.sT + rp], inv);
+  }
+  OFX_SYNC();
+}
+
+// ------------------------------------------------------------------------
+// One instance: single pass (trips == 0) or round trips.
+// ------------------------------------------------------------------------
+template <typename R>
+OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc& d,
+                          int rank, cpx<R>* lds, double* red, int* sflag, unsigned& nbar,
+                          int coh) {
+  typedef cpx<R> C;
+  Geo<R> g;
+  g.n = d.n;
+  g.n1 = d.n1;
+  g.n2 = d.n2;
+  g.lg1 = d.lg1;
+  g.lg2 = d.lg2;
+  g.lgn = d.lg1 + d.lg2;
+  g.Cn = cd.C;
+  g.lgC = 0;
+  while ((1 << g.lgC) < g.Cn) g.lgC++;
+  g.rank = rank;
+  g.W = g.n1 >> g.lgC;
+  g.lgW = g.lg1 - g.lgC;
+  g.Rr = g.n2 >> g.lgC;
+  g.lgR = g.lg2 - g.lgC;
+  g.HW = cd.halo;
+  g.LC = g.W + 2 * g.HW;
+  g.sT = g.n2 + 1;
+  g.sF = g.n1 + 1;
+  g.c0 = rank * g.W;
+  g.k20 = rank * g.Rr;
+  g.coh = coh;
+  {
+    int b1 = g.LC * g.sT, b2 = g.Rr * g.sF;
+    int bs = b1 > b2 ? b1 : b2;
+    g.B0 = lds;
+    g.B1 = lds + bs;
+    g.B2 = lds + 2 * bs;
+  }
+  const int n = g.n, n1 = g.n1, n2 = g.n2;
+  const C* tw = (const C*)a.twiddle;
+  const int nmax = a.nmax;
+  C* field = (C*)d.field;
+  C* t1 = (C*)cd.t1;
+  C* t2 = (C*)cd.t2;
+  const double dt = d.dt, w0 = d.w0;
+  const C* wt = (const C*)d.wt;
+  const R* ab = (const R*)d.absorb;
+  const R cc = (R)(0.5 / dt / w0);
+  const int nsteps = d.nsteps;
+  const double sel = d.sel;
+  const double nChunks = (double)nsteps / sel;
+  const bool cavity = a.trips > 0 && d.cav != nullptr;
+  const int trips = a.trips > 0 ? a.trips : 1;
+  int status = ST_RUNNING;
+  long long attempts = 0, accepted = 0;
+  const int trip_base = d.trips_done[0];
+
+  // load the crystal-input field with halo columns
+  for (int idx = OFX_TID; idx < g.LC * n2; idx += OFX_NT) {
+    const int lc = idx >> g.lg2, r = idx & (n2 - 1);
+    g.B0[lc * g.sT + r] = field[gidx(g, lc, r)];
+  }
+  OFX_SYNC();
+
+  int trip = 0;
+  for (; trip < trips && status == ST_RUNNING; trip++) {
+    long long trip_att = 0, trip_acc = 0;
+    double k = 1.0, stepmod = 4.0;
+    int chunk = 1;
+    // ---------------- adaptive Heun pass (NEE_CPU.m:39-94) ----------------
+    while ((double)chunk <= nChunks) {
+      const double Gk = (double)d.G[(int)k - 1];
+      const double bdwz0 = d.bdiffw0 * (k - 1.0) * d.h;
+      const double bdwz1 = d.bdiffw0 * (k - 1.0 + stepmod) * d.h;
+      const R gk = (R)(d.h * stepmod * Gk);
+      C* A0 = g.B0;
+      C* X = g.B1;
+      C* K = g.B2;
+      int bad = 0;
+      // exp(i*(w0 t_j - bdwz)) = wt[j] * exp(-i*bdwz); bdwz reduced in FP64
+      const C eb0 = expi(-bdwz0, R(0));
+      const C eb1 = expi(-bdwz1, R(0));
+      // stage 1: NL(A0) on all local columns
+      for (int idx = OFX_TID; idx < g.LC * n2; idx += OFX_NT) {
+        const int lc = idx >> g.lg2, r = idx & (n2 - 1);
+        const int j = gidx(g, lc, r);
+        C v = A0[lc * g.sT + r];
+        if (!(isfinite((double)v.x) && isfinite((double)v.y))) bad = 1;
+        X[lc * g.sT + r] = nl_term_e<R>(v, cmul(wt[j], eb0));
+      }
+      OFX_SYNC();
+      // K1 on columns 1..LC-2
+      for (int idx = OFX_TID; idx < (g.LC - 2) * n2; idx += OFX_NT) {
+        const int lc = 1 + (idx >> g.lg2), r = idx & (n2 - 1);
+        const int p = lc * g.sT + r;
+        K[p] = k_term<R>(X[p], X[p + g.sT], X[p - g.sT], gk, cc);
+      }
+      OFX_SYNC();
+      // stage 2: NL(A0 + K1) on columns 1..LC-2
+      for (int idx = OFX_TID; idx < (g.LC - 2) * n2; idx += OFX_NT) {
+        const int lc = 1 + (idx >> g.lg2), r = idx & (n2 - 1);
+        const int p = lc * g.sT + r;
+        const int j = gidx(g, lc, r);
+        X[p] = nl_term_e<R>(cadd(A0[p], K[p]), cmul(wt[j], eb1));
+      }
+      OFX_SYNC();
+      // K2, corrector, error, absorber on own columns; E overwrites K
+      double lmax = -1.0;
+      for (int idx = OFX_TID; idx < g.W * n2; idx += OFX_NT) {
+        const int lc = g.HW + (idx >> g.lg2), r = idx & (n2 - 1);
+        const int p = lc * g.sT + r;
+        C k2 = k_term<R>(X[p], X[p + g.sT], X[p - g.sT], gk, cc);
+        C k1 = K[p];
+        C a0 = A0[p];
+        C a1 = cadd(a0, k1);
+        C e = cadd(cadd(a0, cscale(k1, R(0.5))), cscale(k2, R(0.5)));
+        const double ex = (double)e.x - (double)a1.x, ey = (double)e.y - (double)a1.y;
+        const double err = sqrt(ex * ex + ey * ey) /
+                           (sqrt((double)e.x * (double)e.x + (double)e.y * (double)e.y) + 1.0);
+        if (err > lmax) lmax = err;
+        const R f = ab[gidx(g, lc, r)];
+        if (f != R(1)) e = cscale(e, f);
+        K[p] = e;
+      }
+      OFX_SYNC();
+      lmax = wg_max(lmax, red);
+      double badv = wg_max((double)bad, red);
+      // speculative forward column transforms of E
+      fwd_cols_to_t1<R>(g, K, X, t1, tw, nmax);
+      const int par = (int)(nbar & 1u);
+      if (OFX_TID == 0) {
+        xstd(&cd.slots[(par * g.Cn + rank) * 4 + 0], lmax, g.coh);
+        xstd(&cd.slots[(par * g.Cn + rank) * 4 + 1], badv, g.coh);
+      }
+      nbar++;
+      if (!cluster_barrier(cd, nbar * (unsigned)g.Cn, a.timeout_ticks, sflag, g.coh)) {
+        status = ST_BARRIER_TIMEOUT;
+        break;
+      }
+      double gmax = -1.0, gbad = 0.0;
+      for (int q = 0; q < g.Cn; q++) {
+        const double v = xldd(&cd.slots[(par * g.Cn + q) * 4 + 0], g.coh);
+        const double b = xldd(&cd.slots[(par * g.Cn + q) * 4 + 1], g.coh);
+        gmax = v > gmax ? v : gmax;
+        gbad = b > gbad ? b : gbad;
+      }
+      attempts++;
+      trip_att++;
+      if (gbad > 0.0) {
+        status = ST_NONFINITE;
+        break;
+      }
+      const double pcterr = 100.0 * gmax;
+      if (pcterr > d.max_err && stepmod > 1.0) {
+        stepmod -= 1.0;
+        continue;  // reject: A0 is untouched in B0
+      }
+      // ---------------- accept ----------------
+      accepted++;
+      trip_acc++;
+      if (rank == 0 && d.stepmods) d.stepmods[(int)k - 1] = (float)stepmod;
+      const double s_used = stepmod;
+      double ns = stepmod;
+      if (pcterr < d.min_err) ns += 1.0;
+      if (k < nsteps && (k + ns) > nsteps) ns = (double)nsteps - k;
+      const double knew = k + ns;
+      int cend = chunk;  // chunks [chunk, cend) end with this step
+      while ((double)cend <= nChunks && !(knew < cend * sel && knew <= nsteps)) cend++;
+      // frequency phase: rows, dispersion, snapshots, inverse rows
+      C* F = t1_to_rows_fwd<R>(g, t1, g.B0, g.B1, tw, nmax);
+      C* Fo = (F == g.B0) ? g.B1 : g.B0;
+      for (int idx = OFX_TID; idx < g.Rr * n1; idx += OFX_NT) {
+        const int k2l = idx >> g.lg1, k1 = idx & (n1 - 1);
+        const int kk = (g.k20 + k2l) + n2 * k1;
+        C v = F[k2l * g.sF + k1];
+        const C ph = expi(-d.beta[kk] * s_used, R(0));
+        const R al = (R)d.alpha[kk];
+        const R am = al == R(1) ? R(1) : (al == R(0) ? R(0) : (R)pow(al, (R)s_used));
+        v = cscale(cmul(v, ph), am);
+        F[k2l * g.sF + k1] = v;
+        if (d.apft) {
+          C* ap = (C*)d.apft;
+          for (int cc2 = chunk; cc2 < cend; cc2++)
+            if (cc2 - 1 < d.n_cols) ap[(size_t)(cc2 - 1) * n + kk] = v;
+        }
+      }
+      OFX_SYNC();
+      rows_inv_to_t2<R>(g, F, Fo, t2, tw, nmax);
+      nbar++;
+      if (!cluster_barrier(cd, nbar * (unsigned)g.Cn, a.timeout_ticks, sflag, g.coh)) {
+        status = ST_BARRIER_TIMEOUT;
+        break;
+      }
+      t2_to_cols_inv<R>(g, t2, tw, nmax);
+      k = knew;
+      stepmod = ns;
+      chunk = cend;
+    }
+    if (status != ST_RUNNING) break;
+    if (rank == 0 && d.trip_log && trip_base + trip < d.trip_cap) {
+      double* tl = d.trip_log + 4 * (size_t)(trip_base + trip);
+      tl[2] = (double)trip_att;
+      tl[3] = (double)trip_acc;
+    }
+    if (!cavity) continue;
+    // ---------------- cavity: OC split, detuning, pump (one FFT pair) ----------------
+    fwd_cols_to_t1<R>(g, g.B0, g.B1, t1, tw, nmax);
+    nbar++;
+    if (!cluster_barrier(cd, nbar * (unsigned)g.Cn, a.timeout_ticks, sflag, g.coh)) {
+      status = ST_BARRIER_TIMEOUT;
+      break;
+    }
+    {
+      C* F = t1_to_rows_fwd<R>(g, t1, g.B0, g.B1, tw, nmax);
+      C* Fo = (F == g.B0) ? g.B1 : g.B0;
+      const C* pump = (const C*)d.pump;
+      const C* cav = (const C*)d.cav;
+      const C* ocm = (const C*)d.ocm;
+      C* ocs = (C*)d.oc_spec;
+      const bool have_prev = (trip_base + trip) > 0;
+      double acc[3] = {0.0, 0.0, 0.0};  // sum (S-Sp)^2, sum S^2, sum S
+      for (int idx = OFX_TID; idx < g.Rr * n1; idx += OFX_NT) {
+        const int k2l = idx >> g.lg1, k1 = idx & (n1 - 1);
+        const int kk = (g.k20 + k2l) + n2 * k1;
+        C v = F[k2l * g.sF + k1];
+        C o = cmul(v, ocm[kk]);
+        const double S = (double)o.x * (double)o.x + (double)o.y * (double)o.y;
+        if (have_prev) {
+          C op = ocs[kk];
+          const double Sp = (double)op.x * (double)op.x + (double)op.y * (double)op.y;
+          acc[0] += (S - Sp) * (S - Sp);
+        }
+        acc[1] += S * S;
+        acc[2] += S;
+        ocs[kk] = o;
+        C m = cmul(cav[kk], expi(-d.delay * dw_of(kk, n, dt), R(0)));
+        F[k2l * g.sF + k1] = cadd(cmul(v, m), pump ? pump[kk] : mkc<R>(R(0), R(0)));
+      }
+      OFX_SYNC();
+      wg_sum3(acc, red);
+      rows_inv_to_t2<R>(g, F, Fo, t2, tw, nmax);
+      const int par = (int)(nbar & 1u);
+      if (OFX_TID == 0) {
+        xstd(&cd.slots[(par * g.Cn + rank) * 4 + 0], acc[0], g.coh);
+        xstd(&cd.slots[(par * g.Cn + rank) * 4 + 1], acc[1], g.coh);
+        xstd(&cd.slots[(par * g.Cn + rank) * 4 + 2], acc[2], g.coh);
+      }
+      nbar++;
+      if (!cluster_barrier(cd, nbar * (unsigned)g.Cn, a.timeout_ticks, sflag, g.coh)) {
+        status = ST_BARRIER_TIMEOUT;
+        break;
+      }
+      double tot[3] = {0.0, 0.0, 0.0};
+      for (int q = 0; q < g.Cn; q++)
+        for (int i = 0; i < 3; i++) tot[i] += xldd(&cd.slots[(par * g.Cn + q) * 4 + i], g.coh);
+      t2_to_cols_inv<R>(g, t2, tw, nmax);
+      const double conv = have_prev ? (tot[1] > 0.0 ? sqrt(tot[0] / tot[1]) : 0.0) : INFINITY;
+      if (rank == 0 && d.trip_log && trip_base + trip < d.trip_cap) {
+        double* tl = d.trip_log + 4 * (size_t)(trip_base + trip);
+        tl[0] = conv;
+        tl[1] = tot[2];
+      }
+      if (a.conv_tol > 0.0 && conv < a.conv_tol) {
+        status = ST_CONVERGED;
+        trip++;
+        break;
+      }
+    }
+  }
+  if (status == ST_RUNNING) status = ST_DONE;
+  // store own columns (after a barrier so no member still reads the input halo)
+  if (status != ST_BARRIER_TIMEOUT) {
+    nbar++;
+    if (!cluster_barrier(cd, nbar * (unsigned)g.Cn, a.timeout_ticks, sflag, g.coh))
+      status = ST_BARRIER_TIMEOUT;
+  }
+  if (status != ST_BARRIER_TIMEOUT) {
+    for (int idx = OFX_TID; idx < g.W * n2; idx += OFX_NT) {
+      const int lc = g.HW + (idx >> g.lg2), r = idx & (n2 - 1);
+      field[gidx(g, lc, r)] = g.B0[lc * g.sT + r];
+    }
+  }
+  if (rank == 0 && OFX_TID == 0) {
+    d.status[0] = status;
+    d.trips_done[0] += trip;
+    d.counters[0] += attempts;
+    d.counters[1] += accepted;
+  }
+}
+
+template <typename R>
+OFX_DEV void cluster_main(const LaunchArgs& a, int wg, cpx<R>* lds, double* red, int* sflag) {
+  const int ci = a.wg_cluster[wg];
+  if (ci < 0) return;
+  const ClusterDesc& cd = a.clusters[ci];
+  const int rank = a.wg_rank[wg];
+  unsigned nbar = 0;
+  const int coh = 1;
+#ifndef OFX_EMU
+  // placement diagnostic: hardware XCC id of every member
+  if (threadIdx.x == 0) {
+    const int x = (int)__builtin_amdgcn_s_getreg(OFX_HWREG_XCC_ID);
+    __hip_atomic_store(&cd.xcc[rank], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (a.xcc_ids) a.xcc_ids[wg] = x;
+  }
+#endif
+  for (int ii = 0; ii < cd.n_inst; ii++) {
+    const int id = a.inst_order[cd.first_inst + ii];
+    run_instance<R>(a, cd, a.inst[id], rank, lds, red, sflag, nbar, coh);
+#ifdef OFX_EMU
+    if (*cd.abort_flag) break;
+#else
+    if (__hip_atomic_load(cd.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+#endif
+  }
+}
+
+}  // namespace ofx

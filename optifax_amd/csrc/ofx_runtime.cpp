@@ -1,0 +1,807 @@
+// C ABI of libofx_hip (include/optifax_hip.h) over the device backend.
+// Host-side planning (launch geometry, cluster schedule, uploads) lives here;
+// the kernel body is ofx_pass_impl.h and the HIP specifics ofx_backend_hip.hip.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/optifax_hip.h"
+#include "ofx_backend.h"
+#include "ofx_internal.h"
+
+using namespace ofx;
+
+struct ofx_context {
+  Backend* be = nullptr;
+  DeviceInfo info{};
+  std::string err;
+  std::map<std::pair<int, int>, void*> tw;  // (nmax, precision) -> table
+};
+
+struct InstBufs {
+  void* wt = nullptr;
+  void* absorb = nullptr;
+  void* field = nullptr;
+  void* beta = nullptr;
+  void* alpha = nullptr;
+  void* G = nullptr;
+  void* apft = nullptr;
+  void* stepmods = nullptr;
+  void* pump = nullptr;
+  void* cav = nullptr;
+  void* ocm = nullptr;
+  void* oc_spec = nullptr;
+  void* trip_log = nullptr;
+};
+
+struct ofx_opo {
+  ofx_context* ctx = nullptr;
+  int prec = OFX_F32;
+  size_t csz = 8;
+  int n_inst = 0;
+  int trip_cap = 0;
+  std::vector<ofx_pass_params> pass;
+  std::vector<InstDesc> inst_h;
+  std::vector<InstBufs> bufs;
+  std::vector<ClusterDesc> cl_h;
+  std::vector<int> order_h, wgc_h, wgr_h;
+  std::vector<int> inst_C, inst_n1;
+  InstDesc* inst_d = nullptr;
+  ClusterDesc* cl_d = nullptr;
+  int* order_d = nullptr;
+  int* wgc_d = nullptr;
+  int* wgr_d = nullptr;
+  int* xcc_d = nullptr;
+  int* status_d = nullptr;
+  int* trips_d = nullptr;
+  long long* counters_d = nullptr;
+  void* twiddle = nullptr;
+  int nmax = 0;
+  int grid = 0, block = 256;
+  size_t lds = 0;
+  std::vector<void*> allocs;
+  std::vector<unsigned*> zero_ptrs;
+  std::vector<size_t> zero_words;
+  int graph_trips = -1;
+  double graph_tol = 0.0;
+};
+
+namespace {
+
+int fail(ofx_context* ctx, int code, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
+int fail(ofx_context* ctx, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (ctx) ctx->err = buf;
+  return code;
+}
+
+int ilog2i(long long n) {
+  int l = 0;
+  while ((1LL << l) < n) l++;
+  return ((1LL << l) == n) ? l : -1;
+}
+
+size_t lds_bytes_for(int n1, int n2, int C, int halo, size_t csz) {
+  const int W = n1 / C, Rr = n2 / C, LC = W + 2 * halo;
+  const size_t b1 = (size_t)LC * (size_t)(n2 + 1), b2 = (size_t)Rr * (size_t)(n1 + 1);
+  return 3 * std::max(b1, b2) * csz;
+}
+
+int dev_alloc(ofx_opo* r, void** p, size_t bytes) {
+  int st = be_malloc(r->ctx->be, p, bytes ? bytes : 16);
+  if (st) return fail(r->ctx, OFX_E_NOMEM, "device allocation of %zu bytes failed: %s", bytes,
+                      be_error(r->ctx->be));
+  r->allocs.push_back(*p);
+  return OFX_OK;
+}
+
+int upload(ofx_opo* r, void* dst, const void* src, size_t bytes) {
+  if (be_h2d(r->ctx->be, dst, src, bytes))
+    return fail(r->ctx, OFX_E_HIP, "host-to-device copy failed: %s", be_error(r->ctx->be));
+  return OFX_OK;
+}
+
+// FP64 host array -> precision of the run (staging kept alive in `keep`)
+const void* convert(const double* src, size_t count, int prec, std::vector<std::vector<char>>& keep) {
+  keep.emplace_back(count * (prec == OFX_F64 ? 8 : 4));
+  char* out = keep.back().data();
+  if (prec == OFX_F64) {
+    memcpy(out, src, count * 8);
+  } else {
+    float* f = (float*)out;
+    for (size_t i = 0; i < count; i++) f[i] = (float)src[i];
+  }
+  return out;
+}
+
+void* twiddle_table(ofx_context* ctx, int nmax, int prec) {
+  auto key = std::make_pair(nmax, prec);
+  auto it = ctx->tw.find(key);
+  if (it != ctx->tw.end()) return it->second;
+  std::vector<double> t(2 * (size_t)nmax);
+  for (int m = 0; m < nmax; m++) {
+    // exp(-2 pi i m / nmax); angles are dyadic fractions of a turn
+    const double a = -2.0 * M_PI * (double)m / (double)nmax;
+    t[2 * m] = cos(a);
+    t[2 * m + 1] = sin(a);
+  }
+  void* p = nullptr;
+  const size_t bytes = (size_t)nmax * (prec == OFX_F64 ? 16 : 8);
+  if (be_malloc(ctx->be, &p, bytes)) return nullptr;
+  if (prec == OFX_F64) {
+    be_h2d(ctx->be, p, t.data(), bytes);
+  } else {
+    std::vector<float> f(2 * (size_t)nmax);
+    for (size_t i = 0; i < f.size(); i++) f[i] = (float)t[i];
+    be_h2d(ctx->be, p, f.data(), bytes);
+    be_sync(ctx->be);
+  }
+  be_sync(ctx->be);
+  ctx->tw[key] = p;
+  return p;
+}
+
+// x - 2*pi*round(x/(2*pi)), two-term split of 2*pi with fma (exact q*HI)
+double reduce_2pi_host(double x) {
+  const double TWO_PI_HI = 6.283185307179586232;
+  const double TWO_PI_LO = 2.4492935982947064e-16;
+  double q = rint(x * 0.15915494309189533577);
+  return fma(-q, TWO_PI_LO, fma(-q, TWO_PI_HI, x));
+}
+
+// exp(i w0 t_j) and the absorbing-boundary factor for the fftshifted time
+// axis t_j = fftshift((-N/2:N/2-1)*dt) (NEE_CPU.m:12-14, 53, 60, 114-121)
+void time_tables(int n, double dt, double w0, std::vector<double>& wt, std::vector<double>& ab) {
+  wt.resize(2 * (size_t)n);
+  ab.resize((size_t)n);
+  const double tr = (double)(n / 2 - 1) * dt - (double)(-(n / 2)) * dt;  // range(t)
+  for (int j = 0; j < n; j++) {
+    const double t = (double)(j < n / 2 ? j : j - n) * dt;
+    const double th = reduce_2pi_host(w0 * t);
+    wt[2 * (size_t)j] = cos(th);
+    wt[2 * (size_t)j + 1] = sin(th);
+    const double at2 = fabs(2.0 * t);
+    double f = 1.0;
+    if (tr - at2 < 0.25 * tr) {
+      const double sh = 1.0 / cosh(24.0 * (1.0 - at2 / tr));
+      f = 1.0 - sh * sh;
+    }
+    ab[(size_t)j] = f;
+  }
+}
+
+int check_pass(ofx_context* ctx, const ofx_pass_params* p) {
+  if (!p) return fail(ctx, OFX_E_ARG, "null pass parameters");
+  const int lg = ilog2i(p->n);
+  if (lg < 6 || lg > 16) return fail(ctx, OFX_E_ARG, "N=%d must be a power of two in [64, 65536]", p->n);
+  if (p->nsteps < 1) return fail(ctx, OFX_E_ARG, "NSteps=%d must be >= 1", p->nsteps);
+  if (!(p->sel > 1.0)) return fail(ctx, OFX_E_ARG, "sel=%g must be > 1", p->sel);
+  if (!(p->dt > 0) || !(p->w0 > 0) || !(p->h > 0))
+    return fail(ctx, OFX_E_ARG, "dt, w0 and h must be positive");
+  if (!isfinite(p->bdiffw0) || !isfinite(p->max_err) || !isfinite(p->min_err))
+    return fail(ctx, OFX_E_ARG, "bdiffw0, maxErr, minErr must be finite");
+  if (p->n_cols < 0) return fail(ctx, OFX_E_ARG, "n_cols must be >= 0");
+  if (p->method != OFX_METHOD_HEUN)
+    return fail(ctx, OFX_E_UNSUPPORTED, "method %d is not available on the device path", p->method);
+  return OFX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ofx_create(int device, ofx_context** out) {
+  if (!out) return OFX_E_ARG;
+  *out = nullptr;
+  ofx_context* ctx = new ofx_context();
+  int st = be_create(device, &ctx->be, &ctx->info);
+  if (st) {
+    delete ctx;
+    return st;
+  }
+  *out = ctx;
+  return OFX_OK;
+}
+
+void ofx_destroy(ofx_context* ctx) {
+  if (!ctx) return;
+  for (auto& kv : ctx->tw) be_free(ctx->be, kv.second);
+  be_destroy(ctx->be);
+  delete ctx;
+}
+
+const char* ofx_last_error(const ofx_context* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int ofx_device_info(const ofx_context* ctx, char* name, int name_len, int* cus, int* wallclock_khz) {
+  if (!ctx) return OFX_E_ARG;
+  if (name && name_len > 0) {
+    strncpy(name, ctx->info.name, (size_t)name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  if (cus) *cus = ctx->info.cus;
+  if (wallclock_khz) *wallclock_khz = ctx->info.wallclock_khz;
+  return OFX_OK;
+}
+
+void ofx_opo_destroy(ofx_opo* r) {
+  if (!r) return;
+  be_sync(r->ctx->be);
+  be_forget_graph(r->ctx->be, r);
+  for (void* p : r->allocs) be_free(r->ctx->be, p);
+  delete r;
+}
+
+int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_spec* specs,
+                   int trip_cap, const ofx_tuning* tune, ofx_opo** out) {
+  if (!ctx || !out) return OFX_E_ARG;
+  *out = nullptr;
+  if (precision != OFX_F32 && precision != OFX_F64)
+    return fail(ctx, OFX_E_ARG, "precision must be OFX_F32 or OFX_F64");
+  if (n_inst < 1 || !specs) return fail(ctx, OFX_E_ARG, "need at least one instance");
+  if (trip_cap < 0) return fail(ctx, OFX_E_ARG, "trip_cap must be >= 0");
+  for (int i = 0; i < n_inst; i++) {
+    int st = check_pass(ctx, &specs[i].pass);
+    if (st) return st;
+    const ofx_opo_spec& s = specs[i];
+    if (!s.beta || !s.alpha || !s.G || !s.field)
+      return fail(ctx, OFX_E_ARG, "instance %d: beta, alpha, G and field are required", i);
+    if (s.cav && !s.ocm) return fail(ctx, OFX_E_ARG, "instance %d: cav given without ocm", i);
+    if (!isfinite(s.delay)) return fail(ctx, OFX_E_ARG, "instance %d: delay must be finite", i);
+  }
+  ofx_opo* r = new ofx_opo();
+  r->ctx = ctx;
+  r->prec = precision;
+  r->csz = precision == OFX_F64 ? 16 : 8;
+  r->n_inst = n_inst;
+  r->trip_cap = trip_cap;
+  const size_t rsz = precision == OFX_F64 ? 8 : 4;
+  const int halo = 2;
+  std::vector<std::vector<char>> keep;
+  int st = OFX_OK;
+
+  // ---- launch geometry per instance ----
+  r->inst_C.resize(n_inst);
+  r->inst_n1.resize(n_inst);
+  int nmax = 0;
+  for (int i = 0; i < n_inst; i++) {
+    const int n = specs[i].pass.n, lg = ilog2i(n);
+    nmax = std::max(nmax, n);
+    int n1 = 1 << ((lg + 1) / 2);
+    if (tune && tune->n1) n1 = tune->n1;
+    if (ilog2i(n1) < 2 || n % n1 || n / n1 < 4) {
+      delete r;
+      return fail(ctx, OFX_E_ARG, "n1=%d does not split N=%d into two powers of two >= 4", n1, n);
+    }
+    const int n2 = n / n1;
+    int cfit = 0;
+    for (int C = 1; C <= std::min(n1, n2); C <<= 1) {
+      if ((int)lds_bytes_for(n1, n2, C, halo, r->csz) <= ctx->info.lds_max - 1024) {
+        cfit = C;
+        break;
+      }
+    }
+    if (!cfit) {
+      delete r;
+      return fail(ctx, OFX_E_ARG, "N=%d does not fit the %d-byte LDS limit at any cluster size", n,
+                  ctx->info.lds_max);
+    }
+    int C = cfit;
+    if (tune && tune->cluster) {
+      C = tune->cluster;
+      if (ilog2i(C) < 0 || C > std::min(n1, n2) ||
+          (int)lds_bytes_for(n1, n2, C, halo, r->csz) > ctx->info.lds_max - 1024) {
+        delete r;
+        return fail(ctx, OFX_E_ARG, "cluster size %d is invalid for N=%d (n1=%d)", C, n, n1);
+      }
+    } else {
+      // spread a small batch over more workgroups (at most one XCD per instance)
+      const int slots_est = ctx->info.cus;
+      const int per_xcd = ctx->info.nxcd > 0 ? ctx->info.cus / ctx->info.nxcd : ctx->info.cus;
+      while (C * 2 <= std::min(n1, n2) && (long long)n_inst * C * 2 <= slots_est &&
+             C * 2 <= per_xcd && n / (C * 2) >= 1024)
+        C *= 2;
+    }
+    r->inst_C[i] = C;
+    r->inst_n1[i] = n1;
+  }
+  r->nmax = nmax;
+  r->block = (tune && tune->block) ? tune->block : 256;
+  if (r->block < 64 || r->block > 512 || (r->block & 63)) {
+    delete r;
+    return fail(ctx, OFX_E_ARG, "block=%d must be a multiple of 64 in [64,512]", r->block);
+  }
+  size_t lds = 0;
+  for (int i = 0; i < n_inst; i++) {
+    const int n1 = r->inst_n1[i], n2 = specs[i].pass.n / n1;
+    lds = std::max(lds, lds_bytes_for(n1, n2, r->inst_C[i], halo, r->csz));
+  }
+  r->lds = lds;
+  const int bpc = be_blocks_per_cu(ctx->be, precision, r->block, lds);
+  if (bpc < 1) {
+    delete r;
+    return fail(ctx, OFX_E_ARG, "kernel cannot launch with block=%d and %zu bytes of LDS", r->block, lds);
+  }
+  const int slots = ctx->info.cus * bpc;
+
+  // ---- clusters: group instances by (C, n, n1); LPT over cost ----
+  std::map<std::tuple<int, int, int>, std::vector<int>> groups;
+  for (int i = 0; i < n_inst; i++)
+    groups[std::make_tuple(r->inst_C[i], specs[i].pass.n, r->inst_n1[i])].push_back(i);
+  long long want = 0;
+  for (auto& g : groups) want += (long long)std::get<0>(g.first) * (long long)g.second.size();
+  std::vector<std::vector<int>> cl_members;
+  std::vector<int> cl_C;
+  for (auto& g : groups) {
+    const int C = std::get<0>(g.first);
+    std::vector<int> ids = g.second;
+    long long ncl = (long long)ids.size();
+    if (want > slots) ncl = std::max(1LL, (long long)ids.size() * slots / want);
+    ncl = std::min(ncl, (long long)std::max(1, slots / C));
+    std::sort(ids.begin(), ids.end(), [&](int a, int b) {
+      const double ca = (double)specs[a].pass.nsteps, cb = (double)specs[b].pass.nsteps;
+      return ca != cb ? ca > cb : a < b;
+    });
+    std::vector<std::vector<int>> mem((size_t)ncl);
+    std::vector<double> load((size_t)ncl, 0.0);
+    for (int id : ids) {
+      size_t best = 0;
+      for (size_t c = 1; c < load.size(); c++)
+        if (load[c] < load[best]) best = c;
+      mem[best].push_back(id);
+      load[best] += (double)specs[id].pass.nsteps;
+    }
+    for (auto& m : mem) {
+      if (m.empty()) continue;
+      cl_members.push_back(m);
+      cl_C.push_back(C);
+    }
+  }
+  long long used = 0;
+  for (int C : cl_C) used += C;
+  if (used > slots) {
+    delete r;
+    return fail(ctx, OFX_E_ARG, "schedule needs %lld workgroups, only %d fit at once", used, slots);
+  }
+
+  // ---- XCD-aware placement: workgroup id g runs on XCD g % nxcd ----
+  const int nx = std::max(1, ctx->info.nxcd);
+  const int per_x = slots / nx;
+  std::vector<int> freex(nx, per_x);
+  std::vector<int> order_cl(cl_C.size());
+  for (size_t i = 0; i < order_cl.size(); i++) order_cl[i] = (int)i;
+  std::stable_sort(order_cl.begin(), order_cl.end(), [&](int a, int b) { return cl_C[a] > cl_C[b]; });
+  std::vector<std::vector<int>> cl_wgs(cl_C.size());
+  int maxid = -1;
+  for (int ci : order_cl) {
+    const int C = cl_C[ci];
+    int x = (int)(std::max_element(freex.begin(), freex.end()) - freex.begin());
+    if (freex[x] >= C) {
+      for (int q = 0; q < C; q++) {
+        const int m = per_x - freex[x];
+        cl_wgs[ci].push_back(x + nx * m);
+        freex[x]--;
+      }
+    } else {
+      for (int q = 0; q < C; q++) {
+        x = (int)(std::max_element(freex.begin(), freex.end()) - freex.begin());
+        const int m = per_x - freex[x];
+        cl_wgs[ci].push_back(x + nx * m);
+        freex[x]--;
+      }
+    }
+    for (int id : cl_wgs[ci]) maxid = std::max(maxid, id);
+  }
+  r->grid = maxid + 1;
+  r->wgc_h.assign(r->grid, -1);
+  r->wgr_h.assign(r->grid, 0);
+
+  // ---- device buffers ----
+  st = dev_alloc(r, (void**)&r->status_d, sizeof(int) * n_inst);
+  if (!st) st = dev_alloc(r, (void**)&r->trips_d, sizeof(int) * n_inst);
+  if (!st) st = dev_alloc(r, (void**)&r->counters_d, sizeof(long long) * 2 * n_inst);
+  if (!st) st = dev_alloc(r, (void**)&r->xcc_d, sizeof(int) * r->grid);
+  if (st) { ofx_opo_destroy(r); return st; }
+  {
+    std::vector<char> z(std::max(sizeof(long long) * 2 * n_inst, sizeof(int) * (size_t)r->grid), 0);
+    upload(r, r->status_d, z.data(), sizeof(int) * n_inst);
+    upload(r, r->trips_d, z.data(), sizeof(int) * n_inst);
+    upload(r, r->counters_d, z.data(), sizeof(long long) * 2 * n_inst);
+    upload(r, r->xcc_d, z.data(), sizeof(int) * r->grid);
+    be_sync(ctx->be);
+  }
+  r->twiddle = twiddle_table(ctx, nmax, precision);
+  if (!r->twiddle) { ofx_opo_destroy(r); return fail(ctx, OFX_E_NOMEM, "twiddle table allocation failed"); }
+
+  r->pass.resize(n_inst);
+  r->inst_h.resize(n_inst);
+  r->bufs.resize(n_inst);
+  for (int i = 0; i < n_inst && !st; i++) {
+    const ofx_opo_spec& s = specs[i];
+    const ofx_pass_params& p = s.pass;
+    r->pass[i] = p;
+    InstBufs& b = r->bufs[i];
+    const size_t n = (size_t)p.n;
+    st = dev_alloc(r, &b.field, n * r->csz);
+    if (!st) st = dev_alloc(r, &b.wt, n * r->csz);
+    if (!st) st = dev_alloc(r, &b.absorb, n * rsz);
+    if (!st) st = dev_alloc(r, &b.beta, n * 8);
+    if (!st) st = dev_alloc(r, &b.alpha, n * 8);
+    if (!st) st = dev_alloc(r, &b.G, (size_t)p.nsteps * 8);
+    if (!st) st = dev_alloc(r, &b.stepmods, (size_t)p.nsteps * 4);
+    if (!st && p.n_cols > 0) st = dev_alloc(r, &b.apft, n * (size_t)p.n_cols * r->csz);
+    if (!st && s.cav) {
+      st = dev_alloc(r, &b.cav, n * r->csz);
+      if (!st) st = dev_alloc(r, &b.ocm, n * r->csz);
+      if (!st) st = dev_alloc(r, &b.oc_spec, n * r->csz);
+      if (!st && s.pump) st = dev_alloc(r, &b.pump, n * r->csz);
+    }
+    if (!st && trip_cap > 0) st = dev_alloc(r, &b.trip_log, (size_t)trip_cap * 4 * 8);
+    if (st) break;
+    upload(r, b.field, convert(s.field, 2 * n, precision, keep), n * r->csz);
+    {
+      std::vector<double> wt, ab;
+      time_tables(p.n, p.dt, p.w0, wt, ab);
+      upload(r, b.wt, convert(wt.data(), wt.size(), precision, keep), n * r->csz);
+      upload(r, b.absorb, convert(ab.data(), ab.size(), precision, keep), n * rsz);
+    }
+    upload(r, b.beta, s.beta, n * 8);
+    upload(r, b.alpha, s.alpha, n * 8);
+    upload(r, b.G, s.G, (size_t)p.nsteps * 8);
+    {
+      keep.emplace_back((size_t)p.nsteps * 4, 0);
+      upload(r, b.stepmods, keep.back().data(), (size_t)p.nsteps * 4);
+    }
+    if (b.apft) {
+      keep.emplace_back(n * (size_t)p.n_cols * r->csz, 0);
+      upload(r, b.apft, keep.back().data(), n * (size_t)p.n_cols * r->csz);
+    }
+    if (b.cav) {
+      upload(r, b.cav, convert(s.cav, 2 * n, precision, keep), n * r->csz);
+      upload(r, b.ocm, convert(s.ocm, 2 * n, precision, keep), n * r->csz);
+      keep.emplace_back(n * r->csz, 0);
+      upload(r, b.oc_spec, keep.back().data(), n * r->csz);
+      if (b.pump) upload(r, b.pump, convert(s.pump, 2 * n, precision, keep), n * r->csz);
+    }
+    if (b.trip_log) {
+      keep.emplace_back((size_t)trip_cap * 4 * 8, 0);
+      upload(r, b.trip_log, keep.back().data(), (size_t)trip_cap * 4 * 8);
+    }
+    InstDesc& d = r->inst_h[i];
+    memset(&d, 0, sizeof d);
+    d.n = p.n;
+    d.n1 = r->inst_n1[i];
+    d.n2 = p.n / d.n1;
+    d.lg1 = ilog2i(d.n1);
+    d.lg2 = ilog2i(d.n2);
+    d.nsteps = p.nsteps;
+    d.n_cols = p.n_cols;
+    d.w0 = p.w0;
+    d.bdiffw0 = p.bdiffw0;
+    d.h = p.h;
+    d.dt = p.dt;
+    d.max_err = p.max_err;
+    d.min_err = p.min_err;
+    d.sel = p.sel;
+    d.delay = s.delay;
+    d.beta = (const double*)b.beta;
+    d.alpha = (const double*)b.alpha;
+    d.G = (const double*)b.G;
+    d.field = b.field;
+    d.wt = b.wt;
+    d.absorb = b.absorb;
+    d.apft = b.apft;
+    d.stepmods = (float*)b.stepmods;
+    d.pump = b.pump;
+    d.cav = b.cav;
+    d.ocm = b.ocm;
+    d.oc_spec = b.oc_spec;
+    d.trip_log = (double*)b.trip_log;
+    d.trip_cap = trip_cap;
+    d.status = r->status_d + i;
+    d.trips_done = r->trips_d + i;
+    d.counters = r->counters_d + 2 * i;
+  }
+  if (st) { be_sync(ctx->be); ofx_opo_destroy(r); return st; }
+
+  // clusters
+  r->cl_h.resize(cl_C.size());
+  for (size_t c = 0; c < cl_C.size() && !st; c++) {
+    ClusterDesc& cd = r->cl_h[c];
+    memset(&cd, 0, sizeof cd);
+    cd.C = cl_C[c];
+    cd.first_inst = (int)r->order_h.size();
+    cd.n_inst = (int)cl_members[c].size();
+    cd.halo = halo;
+    int nc = 0;
+    for (int id : cl_members[c]) {
+      r->order_h.push_back(id);
+      nc = std::max(nc, specs[id].pass.n);
+    }
+    st = dev_alloc(r, &cd.t1, (size_t)nc * r->csz);
+    if (!st) st = dev_alloc(r, &cd.t2, (size_t)nc * r->csz);
+    if (!st) st = dev_alloc(r, (void**)&cd.slots, sizeof(double) * 8 * (size_t)cd.C);
+    if (!st) st = dev_alloc(r, (void**)&cd.bar, 64);
+    if (!st) st = dev_alloc(r, (void**)&cd.abort_flag, 64);
+    if (!st) st = dev_alloc(r, (void**)&cd.xcc, sizeof(int) * (size_t)cd.C);
+    if (st) break;
+    r->zero_ptrs.push_back(cd.bar);
+    r->zero_words.push_back(1);
+    r->zero_ptrs.push_back((unsigned*)cd.abort_flag);
+    r->zero_words.push_back(1);
+    for (int q = 0; q < cd.C; q++) {
+      r->wgc_h[cl_wgs[c][q]] = (int)c;
+      r->wgr_h[cl_wgs[c][q]] = q;
+    }
+  }
+  if (!st) st = dev_alloc(r, (void**)&r->inst_d, sizeof(InstDesc) * n_inst);
+  if (!st) st = dev_alloc(r, (void**)&r->cl_d, sizeof(ClusterDesc) * r->cl_h.size());
+  if (!st) st = dev_alloc(r, (void**)&r->order_d, sizeof(int) * r->order_h.size());
+  if (!st) st = dev_alloc(r, (void**)&r->wgc_d, sizeof(int) * r->grid);
+  if (!st) st = dev_alloc(r, (void**)&r->wgr_d, sizeof(int) * r->grid);
+  if (st) { be_sync(ctx->be); ofx_opo_destroy(r); return st; }
+  upload(r, r->inst_d, r->inst_h.data(), sizeof(InstDesc) * n_inst);
+  upload(r, r->cl_d, r->cl_h.data(), sizeof(ClusterDesc) * r->cl_h.size());
+  upload(r, r->order_d, r->order_h.data(), sizeof(int) * r->order_h.size());
+  upload(r, r->wgc_d, r->wgc_h.data(), sizeof(int) * r->grid);
+  upload(r, r->wgr_d, r->wgr_h.data(), sizeof(int) * r->grid);
+  if (be_sync(ctx->be)) {
+    st = fail(ctx, OFX_E_HIP, "upload failed: %s", be_error(ctx->be));
+    ofx_opo_destroy(r);
+    return st;
+  }
+  *out = r;
+  return OFX_OK;
+}
+
+static LaunchArgs make_args(ofx_opo* r, int trips, double tol) {
+  LaunchArgs a;
+  memset(&a, 0, sizeof a);
+  a.inst = r->inst_d;
+  a.clusters = r->cl_d;
+  a.inst_order = r->order_d;
+  a.wg_cluster = r->wgc_d;
+  a.wg_rank = r->wgr_d;
+  a.twiddle = r->twiddle;
+  a.nmax = r->nmax;
+  a.trips = trips;
+  a.conv_tol = tol;
+  // barrier timeout: 10 s of wall clock
+  a.timeout_ticks = (unsigned long long)r->ctx->info.wallclock_khz * 1000ULL * 10ULL;
+  a.xcc_ids = r->xcc_d;
+  return a;
+}
+
+int ofx_opo_launch(ofx_opo* r, int trips, double conv_tol) {
+  if (!r) return OFX_E_ARG;
+  if (trips < 0) return fail(r->ctx, OFX_E_ARG, "trips must be >= 0");
+  LaunchArgs a = make_args(r, trips, conv_tol);
+  int st = be_launch(r->ctx->be, a, r->prec, r->grid, r->block, r->lds, r->zero_ptrs.data(),
+                     r->zero_words.data(), (int)r->zero_ptrs.size(), nullptr, 0);
+  if (st) return fail(r->ctx, OFX_E_HIP, "launch failed: %s", be_error(r->ctx->be));
+  return OFX_OK;
+}
+
+int ofx_opo_graph_replay(ofx_opo* r, int trips, double conv_tol, int replays) {
+  if (!r || replays < 1 || trips < 0) return OFX_E_ARG;
+  if (r->graph_trips != trips || r->graph_tol != conv_tol) {
+    be_forget_graph(r->ctx->be, r);
+    r->graph_trips = trips;
+    r->graph_tol = conv_tol;
+  }
+  LaunchArgs a = make_args(r, trips, conv_tol);
+  int st = be_launch(r->ctx->be, a, r->prec, r->grid, r->block, r->lds, r->zero_ptrs.data(),
+                     r->zero_words.data(), (int)r->zero_ptrs.size(), r, replays);
+  if (st) return fail(r->ctx, OFX_E_HIP, "graph replay failed: %s", be_error(r->ctx->be));
+  return OFX_OK;
+}
+
+int ofx_opo_sync(ofx_opo* r) {
+  if (!r) return OFX_E_ARG;
+  if (be_sync(r->ctx->be)) return fail(r->ctx, OFX_E_HIP, "synchronize failed: %s", be_error(r->ctx->be));
+  std::vector<int> stv(r->n_inst);
+  if (be_d2h(r->ctx->be, stv.data(), r->status_d, sizeof(int) * r->n_inst) || be_sync(r->ctx->be))
+    return fail(r->ctx, OFX_E_HIP, "status readback failed: %s", be_error(r->ctx->be));
+  for (int i = 0; i < r->n_inst; i++) {
+    if (stv[i] == ST_BARRIER_TIMEOUT)
+      return fail(r->ctx, OFX_E_TIMEOUT, "instance %d: cluster barrier timed out", i);
+  }
+  for (int i = 0; i < r->n_inst; i++) {
+    if (stv[i] == ST_NONFINITE)
+      return fail(r->ctx, OFX_E_NONFINITE, "instance %d: field became non-finite", i);
+  }
+  return OFX_OK;
+}
+
+int ofx_opo_last_ms(ofx_opo* r, double* ms) {
+  if (!r || !ms) return OFX_E_ARG;
+  if (be_last_ms(r->ctx->be, ms)) return fail(r->ctx, OFX_E_HIP, "no timing available");
+  return OFX_OK;
+}
+
+static int read_conv(ofx_opo* r, void* src, double* dst, size_t count) {
+  std::vector<char> tmp(count * (r->prec == OFX_F64 ? 8 : 4));
+  if (be_d2h(r->ctx->be, tmp.data(), src, tmp.size()) || be_sync(r->ctx->be))
+    return fail(r->ctx, OFX_E_HIP, "readback failed: %s", be_error(r->ctx->be));
+  if (r->prec == OFX_F64) {
+    memcpy(dst, tmp.data(), count * 8);
+  } else {
+    const float* f = (const float*)tmp.data();
+    for (size_t i = 0; i < count; i++) dst[i] = f[i];
+  }
+  return OFX_OK;
+}
+
+int ofx_opo_read(ofx_opo* r, int inst, double* field, double* oc_spec, double* trip_log, int* status,
+                 int* trips_done, long long* counters, double* apft, double* stepmods) {
+  if (!r || inst < 0 || inst >= r->n_inst) return OFX_E_ARG;
+  if (be_sync(r->ctx->be)) return fail(r->ctx, OFX_E_HIP, "synchronize failed: %s", be_error(r->ctx->be));
+  const InstBufs& b = r->bufs[inst];
+  const ofx_pass_params& p = r->pass[inst];
+  const size_t n = (size_t)p.n;
+  int st = OFX_OK;
+  if (field) st = read_conv(r, b.field, field, 2 * n);
+  if (!st && oc_spec) {
+    if (!b.oc_spec) return fail(r->ctx, OFX_E_ARG, "instance %d has no cavity", inst);
+    st = read_conv(r, b.oc_spec, oc_spec, 2 * n);
+  }
+  if (!st && trip_log && b.trip_log) {
+    if (be_d2h(r->ctx->be, trip_log, b.trip_log, (size_t)r->trip_cap * 32) || be_sync(r->ctx->be))
+      st = fail(r->ctx, OFX_E_HIP, "readback failed");
+  }
+  if (!st && status) {
+    if (be_d2h(r->ctx->be, status, r->status_d + inst, sizeof(int)) || be_sync(r->ctx->be))
+      st = fail(r->ctx, OFX_E_HIP, "readback failed");
+  }
+  if (!st && trips_done) {
+    if (be_d2h(r->ctx->be, trips_done, r->trips_d + inst, sizeof(int)) || be_sync(r->ctx->be))
+      st = fail(r->ctx, OFX_E_HIP, "readback failed");
+  }
+  if (!st && counters) {
+    if (be_d2h(r->ctx->be, counters, r->counters_d + 2 * inst, 2 * sizeof(long long)) ||
+        be_sync(r->ctx->be))
+      st = fail(r->ctx, OFX_E_HIP, "readback failed");
+  }
+  if (!st && apft && b.apft) st = read_conv(r, b.apft, apft, 2 * n * (size_t)p.n_cols);
+  if (!st && stepmods) {
+    std::vector<float> sm((size_t)p.nsteps);
+    if (be_d2h(r->ctx->be, sm.data(), b.stepmods, sm.size() * 4) || be_sync(r->ctx->be))
+      st = fail(r->ctx, OFX_E_HIP, "readback failed");
+    for (size_t j = 0; !st && j < sm.size(); j++) stepmods[j] = sm[j];
+  }
+  return st;
+}
+
+int ofx_opo_write_field(ofx_opo* r, int inst, const double* field) {
+  if (!r || !field || inst < 0 || inst >= r->n_inst) return OFX_E_ARG;
+  std::vector<std::vector<char>> keep;
+  const size_t n = (size_t)r->pass[inst].n;
+  int st = upload(r, r->bufs[inst].field, convert(field, 2 * n, r->prec, keep), n * r->csz);
+  if (!st && be_sync(r->ctx->be)) st = fail(r->ctx, OFX_E_HIP, "upload failed");
+  return st;
+}
+
+int ofx_opo_geometry(const ofx_opo* r, int inst, int* cluster, int* n1, int* block, int* grid,
+                     int* lds_bytes) {
+  if (!r || inst < 0 || inst >= r->n_inst) return OFX_E_ARG;
+  if (cluster) *cluster = r->inst_C[inst];
+  if (n1) *n1 = r->inst_n1[inst];
+  if (block) *block = r->block;
+  if (grid) *grid = r->grid;
+  if (lds_bytes) *lds_bytes = (int)r->lds;
+  return OFX_OK;
+}
+
+int ofx_opo_xcc_ids(ofx_opo* r, int* ids, int cap, int* count) {
+  if (!r || !ids || cap < 0) return OFX_E_ARG;
+  const int n = std::min(cap, r->grid);
+  if (be_d2h(r->ctx->be, ids, r->xcc_d, sizeof(int) * (size_t)n) || be_sync(r->ctx->be))
+    return fail(r->ctx, OFX_E_HIP, "readback failed");
+  if (count) *count = r->grid;
+  return OFX_OK;
+}
+
+}  // extern "C"
+
+// ---------------- host single pass (OPOmexBatch computation) ----------------
+template <typename T>
+static int pass_host_impl(ofx_context* ctx, int prec, const ofx_pass_params* p, int batch, T* Et,
+                          const T* alpha_w, const T* G, int g_batch, const T* beta_op, T* ApFT,
+                          T* stepMods, int* status, const ofx_tuning* tune) {
+  if (!ctx) return OFX_E_ARG;
+  int st = check_pass(ctx, p);
+  if (st) return st;
+  if (batch < 1 || !Et || !alpha_w || !G || !beta_op)
+    return fail(ctx, OFX_E_ARG, "Et, alpha_w, G, beta_op are required and batch >= 1");
+  if (g_batch != 1 && g_batch != batch)
+    return fail(ctx, OFX_E_ARG, "G must hold NSteps or NSteps*batch values (g_batch=%d)", g_batch);
+  const size_t n = (size_t)p->n;
+  std::vector<double> beta(n), alpha(n), field(2 * n * batch), Gd((size_t)p->nsteps * g_batch);
+  for (size_t j = 0; j < n; j++) {
+    beta[j] = (double)beta_op[j];
+    alpha[j] = (double)alpha_w[j];
+  }
+  for (size_t j = 0; j < field.size(); j++) field[j] = (double)Et[j];
+  for (size_t j = 0; j < Gd.size(); j++) Gd[j] = (double)G[j];
+  std::vector<ofx_opo_spec> specs((size_t)batch);
+  for (int b = 0; b < batch; b++) {
+    ofx_opo_spec& s = specs[b];
+    memset(&s, 0, sizeof s);
+    s.pass = *p;
+    if (!ApFT) s.pass.n_cols = 0;
+    s.beta = beta.data();
+    s.alpha = alpha.data();
+    s.G = Gd.data() + (g_batch == 1 ? 0 : (size_t)b * p->nsteps);
+    s.field = field.data() + 2 * n * b;
+  }
+  ofx_opo* r = nullptr;
+  st = ofx_opo_create(ctx, prec, batch, specs.data(), 0, tune, &r);
+  if (st) return st;
+  const size_t cs = r->csz;
+  std::vector<std::vector<char>> keep;
+  // the MEX contract returns the input ApFT/stepMods with written entries replaced
+  for (int b = 0; b < batch && !st; b++) {
+    if (ApFT && p->n_cols > 0) {
+      std::vector<double> tmp(2 * n * p->n_cols);
+      for (size_t j = 0; j < tmp.size(); j++) tmp[j] = (double)ApFT[2 * n * p->n_cols * b + j];
+      st = upload(r, r->bufs[b].apft, convert(tmp.data(), tmp.size(), prec, keep), n * p->n_cols * cs);
+    }
+    if (!st && stepMods) {
+      keep.emplace_back((size_t)p->nsteps * 4);
+      float* f = (float*)keep.back().data();
+      for (int j = 0; j < p->nsteps; j++) f[j] = (float)stepMods[(size_t)b * p->nsteps + j];
+      st = upload(r, r->bufs[b].stepmods, f, (size_t)p->nsteps * 4);
+    }
+  }
+  if (!st) st = ofx_opo_launch(r, 0, 0.0);
+  int sync_st = st ? st : ofx_opo_sync(r);
+  if (sync_st == OFX_E_HIP || sync_st == OFX_E_TIMEOUT) {
+    ofx_opo_destroy(r);
+    return sync_st;
+  }
+  std::vector<double> f(2 * n), ap(2 * n * std::max(1, p->n_cols)), sm((size_t)p->nsteps);
+  for (int b = 0; b < batch && (st == OFX_OK); b++) {
+    int s_b = 0;
+    st = ofx_opo_read(r, b, f.data(), nullptr, nullptr, &s_b, nullptr, nullptr,
+                      (ApFT && p->n_cols > 0) ? ap.data() : nullptr, stepMods ? sm.data() : nullptr);
+    if (st) break;
+    for (size_t j = 0; j < 2 * n; j++) Et[2 * n * b + j] = (T)f[j];
+    if (ApFT && p->n_cols > 0)
+      for (size_t j = 0; j < 2 * n * p->n_cols; j++) ApFT[2 * n * p->n_cols * b + j] = (T)ap[j];
+    if (stepMods)
+      for (int j = 0; j < p->nsteps; j++) stepMods[(size_t)b * p->nsteps + j] = (T)sm[j];
+    if (status) status[b] = s_b;
+  }
+  ofx_opo_destroy(r);
+  if (st) return st;
+  return sync_st;
+}
+
+extern "C" {
+
+int ofx_crystal_pass_host(ofx_context* ctx, const ofx_pass_params* p, int batch, float* Et,
+                          const float* alpha_w, const float* G, int g_batch, const float* beta_op,
+                          float* ApFT, float* stepMods, int* status, const ofx_tuning* tune) {
+  return pass_host_impl<float>(ctx, OFX_F32, p, batch, Et, alpha_w, G, g_batch, beta_op, ApFT,
+                               stepMods, status, tune);
+}
+
+int ofx_crystal_pass_host_f64(ofx_context* ctx, const ofx_pass_params* p, int batch, double* Et,
+                              const double* alpha_w, const double* G, int g_batch,
+                              const double* beta_op, double* ApFT, double* stepMods, int* status,
+                              const ofx_tuning* tune) {
+  return pass_host_impl<double>(ctx, OFX_F64, p, batch, Et, alpha_w, G, g_batch, beta_op, ApFT,
+                                stepMods, status, tune);
+}
+
+}  // extern "C"

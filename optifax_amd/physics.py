@@ -1,0 +1,296 @@
+"""Host-side producers of the crystal-solver and cavity inputs (FP64, numpy).
+
+These mirror the MATLAB objects that build the arguments of the reference's
+solver call (toolbox/lib/classes/OpticalSim.m:213-252): the simulation grid
+(SimWindow.m), the MgO:PPLN dispersion (n_mgoppln_gayer.m, phi_calc.m,
+phi2GD.m), the QPM poling (QPMcrystal.m, NonlinearCrystal.m:76-110), the pump
+(Laser.m:66-77, sechPulse.m), and the cavity's spectral operators
+(OpticalPulse.m:137-197,256-261).  New here: seeded one-photon-per-mode noise.
+
+All arrays are numpy float64/complex128; the device library converts them.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+C0 = 299792458.0          # toolbox/static/CONST/c.m:2
+EPS0 = 8.854e-12          # toolbox/static/CONST/eps0.m:2
+HBAR = 1.054571817e-34    # CODATA 2018 reduced Planck constant
+
+
+class SimWindow:
+    """Time-constrained simulation window (SimWindow.m:36-163).
+
+    Index order of every spectral array here is the SimWindow order (centred:
+    index N/2 is the reference frequency); `fft_order(x)` converts to the
+    solver's FFT order (OpticalSim.m:223 fftshift)."""
+
+    def __init__(self, lambda_ref: float, n_points: int, temporal_range: float,
+                 spectral_limits_nm=(300.0, 6000.0)):
+        if n_points < 4 or n_points & (n_points - 1):
+            raise ValueError("n_points must be a power of two")
+        self.reference_wave = float(lambda_ref)
+        self.n = int(n_points)
+        self.temporal_range = float(temporal_range)
+        self.spectral_limits = tuple(float(x) for x in spectral_limits_nm)
+        m = np.arange(-self.n // 2, self.n // 2, dtype=np.float64)
+        self.times = m / self.n * self.temporal_range          # SimWindow.m:69
+        self.dt = self.times[1] - self.times[0]                # SimWindow.m:126
+        self.rel_frequencies = m / (self.n * self.dt)          # fftax.m
+        f0 = C0 / self.reference_wave
+        self.omegas = 2 * np.pi * (self.rel_frequencies + 1 * f0)  # SimWindow.m:102-106
+        self.reference_omega = 2 * np.pi * C0 / self.reference_wave  # SimWindow.m:61-63
+        self.rel_omegas = self.omegas - self.reference_omega   # SimWindow.m:108-110
+        self.wavelengths = 2 * np.pi * C0 / self.omegas        # SimWindow.m:112-114
+        self.reference_index = self.n // 2                     # SimWindow.m:161-163 (0-based)
+        self.d_omega = self.omegas[1] - self.omegas[0]         # SimWindow.m:133-135
+        self.d_nu = self.rel_frequencies[1] - self.rel_frequencies[0]
+
+    @staticmethod
+    def fft_order(x):
+        return np.fft.ifftshift(x)
+
+    @staticmethod
+    def window_order(x):
+        return np.fft.fftshift(x)
+
+    def in_band(self):
+        lam_nm = self.wavelengths * 1e9
+        lo, hi = self.spectral_limits
+        return (lam_nm >= lo) & (lam_nm <= hi) & (self.omegas > 0)
+
+    # SpectralField / k2t (OpticalPulse.m:283-316): the 2N padded transform
+    # decimated by 2 equals the N-point transform.
+    @staticmethod
+    def t2k(E):
+        return np.fft.fftshift(np.fft.fft(np.fft.fftshift(E)))
+
+    @staticmethod
+    def k2t(Ek):
+        return np.fft.ifftshift(np.fft.ifft(np.fft.ifftshift(Ek)))
+
+
+def gayer_n(lam_um, T=20.0, col=1):
+    """MgO:LN index, n_mgoppln_gayer.m:11-37; real part of a complex sqrt."""
+    a1 = [5.756, 5.653, 5.078, 5.35583][col - 1]
+    a2 = [0.0983, 0.1185, 0.0964, 0.100473][col - 1]
+    a3 = [0.2020, 0.2091, 0.2065, 0.20692][col - 1]
+    a4 = [189.32, 89.61, 61.16, 100][col - 1]
+    a5 = [12.52, 10.85, 10.55, 11.34927][col - 1]
+    a6 = [1.32e-2, 1.97e-2, 1.59e-2, 1.5334e-2][col - 1]
+    b1 = [2.860e-6, 7.941e-7, 4.677e-6, 4.629e-7][col - 1]
+    b2 = [4.700e-8, 3.134e-8, 7.822e-8, 3.862e-8][col - 1]
+    b3 = [6.113e-8, -4.641e-9, -2.653e-8, -0.89e-8][col - 1]
+    b4 = [1.516e-4, -2.188e-6, 1.096e-4, 2.657e-5][col - 1]
+    rl = np.asarray(lam_um, dtype=np.float64)
+    fn = (T - 24.5) * (T + 570.82)
+    x = ((a1 + b1 * fn) + (a2 + b2 * fn) / (rl * rl - (a3 + b3 * fn) * (a3 + b3 * fn))
+         + (a4 + b4 * fn) / (rl * rl - a5 * a5) - a6 * rl * rl)
+    return np.sqrt(x.astype(np.complex128)).real
+
+
+def phi_calc(L, T, simwin: SimWindow):
+    """phi_calc.m:5-45 for PPLN: (phi_rel, phi_abs).
+
+    phi_rel is defined on the 0.2-20 um knot band (NaN elsewhere): at the
+    knots the reference's spline (phi_calc.m:40) returns the cumtrapz values
+    themselves, and outside the 300-6000 nm spectral limits the crystal
+    transmission is zero, so the extrapolated values never reach the field."""
+    w_abs, w_rel = simwin.omegas, simwin.rel_omegas
+    lam = 2 * np.pi * C0 / w_abs
+    n_mat = gayer_n(lam * 1e6, T, 1)
+    beta_abs = w_abs * n_mat / C0
+    phi_abs = L * beta_abs
+    gd_abs = np.concatenate([[0.0], np.diff(phi_abs) / np.diff(w_abs)])
+    ref = simwin.reference_index
+    gd_rel = gd_abs - gd_abs[ref]
+    del_phi = gd_rel * np.concatenate([[0.0], np.diff(w_rel)])
+    band = (lam > 2e-7) & (lam < 2e-5)
+    idx = np.nonzero(band)[0]
+    if idx.size == 0 or not band[ref] or np.any(np.diff(idx) != 1):
+        raise ValueError("phi_calc: knot band must be contiguous and contain the reference")
+    y = del_phi[idx]
+    acc = np.concatenate([[0.0], np.cumsum(0.5 * (y[1:] + y[:-1]))])
+    phi = np.full(simwin.n, np.nan)
+    phi[idx] = acc
+    return phi - phi[ref], phi_abs
+
+
+def phi2gd(phi, dw):
+    """phi2GD.m:1-17 (GD only)."""
+    n = phi.size
+    phi_r = phi - phi[n // 2 - 1]
+    gd = np.diff(phi_r) / dw
+    return np.concatenate([[gd[0]], gd])
+
+
+def qpm_poling(nsteps, L, period, duty_off=0.0):
+    """QPMcrystal.m:1-57 with zero domain-width uncertainty: +-1 per step."""
+    n_dom = int(math.floor(2.0 * L / period))
+    poled = period * (1.0 - 2.0 * duty_off) / 2.0
+    widths = np.full(n_dom, poled)
+    for i in range(1, n_dom - 1, 2):          # domain_widths(2:2:nDs-1), QPMcrystal.m:30
+        widths[i] = period - 0.5 * (widths[i - 1] + widths[i + 1])
+    walls = np.concatenate([[0.0], np.cumsum(widths)])
+    if walls[-1] < L:
+        walls = np.concatenate([walls, [L]])
+    else:
+        walls[-1] = L
+    n1 = nsteps - 1
+    z = (np.arange(nsteps, dtype=np.float64) * L) / n1   # linspace(0, L, nsteps)
+    z[-1] = L
+    bins = np.searchsorted(walls, z, side="right")        # discretize: edges(k) <= z < edges(k+1)
+    bins = np.minimum(bins, walls.size - 1)               # last bin includes the right edge
+    q = np.mod(bins, 2)
+    return -(2.0 * q - 1.0)
+
+
+@dataclass
+class NonlinearCrystal:
+    """MgO:PPLN crystal (NonlinearCrystal.m, Dielectric.m) with an ideal grating."""
+    period: float                  # grating period [m]
+    length: float                  # [m]
+    chi2: float = 2 * 27e-12       # NonlinearCrystal.m:53-54 default for PPLN
+    temperature: float = 20.0      # Optic.m:56 default
+    duty_off: float = 0.0
+
+    def nsteps(self, step):
+        return int(math.floor(self.length / step))      # NonlinearCrystal.m:79
+
+    def refractive_index(self, simwin: SimWindow):
+        """Dielectric.simulate (Dielectric.m:39-46): real part, clamped at 1."""
+        nr = gayer_n(simwin.wavelengths * 1e6, self.temperature, 1)
+        return np.where(nr < 1, 1.0, nr)
+
+    def solver_inputs(self, simwin: SimWindow, step=1e-7):
+        """Arguments of the solver call (OpticalSim.m:213-226), FFT order."""
+        ns = self.nsteps(step)
+        P = qpm_poling(ns, self.length, self.period, self.duty_off)
+        nr = self.refractive_index(simwin)
+        ref = simwin.reference_index
+        n0 = nr[ref]
+        w0 = simwin.reference_omega
+        G33 = P * self.chi2 * w0 / n0 / 4 / C0
+        beta0_abs = 2 * np.pi * nr / simwin.wavelengths
+        phi_rel, phi_abs = phi_calc(self.length, self.temperature, simwin)
+        gd = phi2gd(phi_abs, simwin.d_omega)
+        beta1_w0 = gd[ref] / self.length
+        bdiffw0 = beta0_abs[ref] - beta1_w0 * w0
+        T = simwin.in_band().astype(np.float64)          # transmission.m:28 (lossless band)
+        alpha = T ** (1.0 / ns)
+        hB = phi_rel / self.length * step
+        hB = np.where(np.isfinite(hB) & (T > 0), hB, 0.0)
+        return dict(G33=G33, beta_op=simwin.fft_order(hB), alpha_w=simwin.fft_order(alpha),
+                    bdiffw0=float(bdiffw0), w0=float(w0), h=float(step), nSteps=ns,
+                    dt=float(simwin.dt), n_xtal=nr, n0=float(n0))
+
+
+def sech_pump(simwin: SimWindow, fwhm, energy, area, n_medium=1.0):
+    """sech^2-intensity pump with the given energy (Laser.m:66-77).
+
+    E(t) = E0 sech(t/tau), tau = fwhm / (2 ln(1+sqrt 2)) (sechPulse.m:12);
+    peak intensity from energy / (sum |E|^2 dt) / area (Laser.m:106-110)."""
+    tau = fwhm / (2.0 * math.log(1.0 + math.sqrt(2.0)))
+    shape = 1.0 / np.cosh(simwin.times / tau)
+    I0 = energy / (np.sum(shape ** 2) * simwin.dt) / area
+    E0 = math.sqrt(I0 * 2.0 / n_medium / EPS0 / C0)
+    return (E0 * shape).astype(np.complex128)
+
+
+def quantum_noise(simwin: SimWindow, area, seed, n_medium=1.0):
+    """One photon per mode with uniform random phase (new; no reference
+    counterpart).  Energy per spectral bin follows I2pow.m:15-18:
+    U_k = |Ek|^2 (c n eps0 / 2) A dt / N = hbar w_k."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    phase = rng.uniform(0.0, 2 * np.pi, simwin.n)
+    w = simwin.omegas
+    band = simwin.in_band()
+    amp = np.zeros(simwin.n)
+    amp[band] = np.sqrt(2 * HBAR * w[band] * simwin.n / (C0 * n_medium * EPS0 * area * simwin.dt))
+    Ek = amp * np.exp(1j * phase)
+    return SimWindow.k2t(Ek)
+
+
+def energy_of(simwin: SimWindow, E, area, n_medium=1.0):
+    """Pulse energy from the spectral field (OpticalPulse.m:404-418, I2pow.m)."""
+    Ek = SimWindow.t2k(E)
+    return float(np.sum(np.abs(Ek) ** 2 * (C0 * n_medium * EPS0 / 2) * area * simwin.dt / simwin.n))
+
+
+@dataclass
+class OutputCoupler:
+    """Cavity output coupler in the "R" regime (Optic.m; OpticalPulse.m:163-188):
+    the intracavity field keeps sqrt(R) e^{-i phi}, the output gets sqrt(T) e^{-i phi}."""
+    transmission: float = 0.05
+    phase: float = 0.0
+
+    def masks(self, simwin: SimWindow):
+        T = np.full(simwin.n, self.transmission)
+        R = 1.0 - T
+        ph = np.full(simwin.n, self.phase)
+        return R, T, ph
+
+
+def cavity_operators(simwin: SimWindow, n_xtal, oc: OutputCoupler, E_pump, E_cav0, n_air=None):
+    """Crystal-frame spectral operators of one round trip, FFT order.
+
+    The reference trip (OpticalSim.m:230-288) is
+      E <- E + E_pump ; refract(xtal) ; solve ; refract(air) ;
+      output copy sqrt(T) ; intracavity sqrt(R) ; applyGD(delay).
+    All steps after the solver and before the next solve are spectral
+    multiplies or a sum, so with X the crystal output spectrum
+      next input  = r_in (sqrt(R) e^{-i phi} e^{-i delay dw} r_out X + P_air)
+                  = cav * e^{-i delay dw} * X + pump
+      output      = sqrt(T) e^{-i phi} r_out X = ocm * X
+    with r_in = sqrt(n_air/n_xtal) = 1/r_out (OpticalPulse.m:190-197).
+    The delay factor is applied on the device (FP64 phase).  Also returns the
+    crystal input of the first trip (fftshifted time)."""
+    if n_air is None:
+        n_air = np.ones(simwin.n)
+    R, T, ph = oc.masks(simwin)
+    r_in = np.sqrt(n_air / n_xtal)
+    r_out = np.sqrt(n_xtal / n_air)
+    cav = np.sqrt(R) * np.exp(-1j * ph) * r_out * r_in
+    ocm = np.sqrt(T) * np.exp(-1j * ph) * r_out
+    P = r_in * SimWindow.t2k(E_pump)
+    X0k = r_in * (SimWindow.t2k(E_cav0) + SimWindow.t2k(E_pump))
+    X0 = SimWindow.k2t(X0k)                          # natural time order
+    f = SimWindow.fft_order
+    return dict(cav=f(cav), ocm=f(ocm), pump=f(P), field0=np.fft.fftshift(X0))
+
+
+@dataclass
+class OPOConfig:
+    """One synchronously pumped ring OPO instance (OpticalSim + Cavity + Laser)."""
+    n: int = 2 ** 15
+    window: float = 12e-12
+    lambda_ref: float = 1040e-9
+    period: float = 30.5e-6
+    length: float = 5e-3
+    chi2: float = 2 * 27e-12
+    pump_fwhm: float = 200e-15
+    pump_energy: float = 2e-9       # 2 W at 1 GHz
+    area: float = math.pi * (50e-6) ** 2
+    oc_transmission: float = 0.05
+    delay: float = 0.3e-12
+    step: float = 1e-7
+    error_bounds: tuple = (2.5e-3, 5e-2)   # OpticalSim.m:69 [min, max] percent
+    progress_plots: int = 5                 # OpticalSim.m:23
+    noise_seed: int = 20261202
+    temperature: float = 20.0
+
+    def build(self):
+        sw = SimWindow(self.lambda_ref, self.n, self.window)
+        xt = NonlinearCrystal(self.period, self.length, self.chi2, self.temperature)
+        si = xt.solver_inputs(sw, self.step)
+        pump = sech_pump(sw, self.pump_fwhm, self.pump_energy, self.area)
+        noise = quantum_noise(sw, self.area, self.noise_seed)
+        ops = cavity_operators(sw, si["n_xtal"], OutputCoupler(self.oc_transmission), pump, noise)
+        sel = math.ceil(si["nSteps"] / (self.progress_plots - 1))   # OpticalSim.m:214
+        si.update(maxErr=self.error_bounds[1], minErr=self.error_bounds[0], sel=float(sel),
+                  n_cols=self.progress_plots - 1)
+        return dict(simwin=sw, crystal=xt, solver=si, pump=pump, noise=noise, ops=ops,
+                    delay=self.delay)

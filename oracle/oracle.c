@@ -1,0 +1,680 @@
+/*
+ * FP64 CPU oracle for the OptiFax chi(2) split-step crystal pass and the
+ * synchronously pumped OPO round-trip loop.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the shipped library links, loads or
+ * calls this file.  It is used by tests/ as the checker, by
+ * __graft_entry__.smoke() as the checker and by bench.py as the timed CPU
+ * baseline ("port" of the reference's MATLAB CPU solver, plotting removed).
+ *
+ * Every function restates the reference algorithm in plain C99 double
+ * precision and cites the reference file:line it follows.  Deliberate
+ * differences from the reference are listed next to the code they affect.
+ *
+ * Parity status of each function (what pins it without a MATLAB run) is in the
+ * header comment of each function; functions without an independent check
+ * say "parity unpinned".
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef double complex cplx;
+
+#define ORC_OK 0
+#define ORC_EARG 1
+#define ORC_ENOMEM 2
+#define ORC_ENONFINITE 3
+
+/* ------------------------------------------------------------------------- */
+/* FFT: iterative radix-2 decimation-in-time, twiddles from a per-size table.  */
+/* Forward is the unnormalised DFT X[k] = sum_j x[j] exp(-2 pi i jk/N) (MATLAB  */
+/* fft); inverse is (1/N) sum_k X[k] exp(+2 pi i jk/N) (MATLAB ifft).          */
+/* Pinned by: brute-force O(N^2) DFT in long double (tests/test_oracle_fft.py)  */
+/* and numpy.fft on random inputs up to 2^16.                                  */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    int n;
+    int log2n;
+    cplx *tw;     /* tw[j] = exp(-2 pi i j / n), j < n/2 */
+    int *rev;     /* bit reversal permutation */
+} fft_plan;
+
+static int ilog2(int n) {
+    int l = 0;
+    while ((1 << l) < n) l++;
+    return ((1 << l) == n) ? l : -1;
+}
+
+static int plan_init(fft_plan *p, int n) {
+    p->n = n;
+    p->log2n = ilog2(n);
+    if (p->log2n < 0) return ORC_EARG;
+    p->tw = (cplx *)malloc(sizeof(cplx) * (size_t)(n / 2 > 0 ? n / 2 : 1));
+    p->rev = (int *)malloc(sizeof(int) * (size_t)n);
+    if (!p->tw || !p->rev) return ORC_ENOMEM;
+    for (int j = 0; j < n / 2; j++) {
+        /* angle reduced exactly: 2*pi*j/n with j/n a dyadic rational */
+        double a = -2.0 * M_PI * (double)j / (double)n;
+        p->tw[j] = cos(a) + I * sin(a);
+    }
+    for (int j = 0; j < n; j++) {
+        int r = 0;
+        for (int b = 0; b < p->log2n; b++)
+            if (j & (1 << b)) r |= 1 << (p->log2n - 1 - b);
+        p->rev[j] = r;
+    }
+    return ORC_OK;
+}
+
+static void plan_free(fft_plan *p) {
+    free(p->tw);
+    free(p->rev);
+    p->tw = NULL;
+    p->rev = NULL;
+}
+
+/* in-place transform; sign = -1 forward (unnormalised), +1 inverse (scaled 1/N) */
+static void fft_exec(const fft_plan *p, cplx *x, int sign) {
+    const int n = p->n;
+    for (int j = 0; j < n; j++) {
+        int r = p->rev[j];
+        if (r > j) {
+            cplx t = x[j];
+            x[j] = x[r];
+            x[r] = t;
+        }
+    }
+    for (int len = 2; len <= n; len <<= 1) {
+        int half = len >> 1;
+        int step = n / len;
+        for (int s = 0; s < n; s += len) {
+            for (int j = 0; j < half; j++) {
+                cplx w = p->tw[j * step];
+                if (sign > 0) w = conj(w);
+                cplx u = x[s + j];
+                cplx v = x[s + j + half] * w;
+                x[s + j] = u + v;
+                x[s + j + half] = u - v;
+            }
+        }
+    }
+    if (sign > 0) {
+        double inv = 1.0 / (double)n;
+        for (int j = 0; j < n; j++) x[j] *= inv;
+    }
+}
+
+/* exported for the FFT pin tests: x is interleaved complex, in place */
+int orc_fft(double *x, int n, int inverse) {
+    fft_plan p;
+    int st = plan_init(&p, n);
+    if (st) { plan_free(&p); return st; }
+    fft_exec(&p, (cplx *)x, inverse ? +1 : -1);
+    plan_free(&p);
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Shared pieces of the NEE right-hand side (NEE_CPU.m:98-121).               */
+/* ------------------------------------------------------------------------- */
+
+/* fftshift(t) for t = (-N/2:N/2-1)*dt (NEE_CPU.m:12-14 used at :53,:60) */
+static void centred_time(int n, double dt, double *ts) {
+    for (int j = 0; j < n; j++) {
+        int m = (j < n / 2) ? j : j - n;
+        ts[j] = (double)m * dt;
+    }
+}
+
+/* nlfn (NEE_CPU.m:98-101): nl = A^2 e^{i th} + 2 conj(e^{i th} |A|^2), th = w0 t - bdwz */
+static void nlfn(int n, const cplx *A, const double *ts, double w0, double bdwz, cplx *nl) {
+    for (int j = 0; j < n; j++) {
+        double th = w0 * ts[j] - bdwz;
+        cplx e = cos(th) + I * sin(th);
+        double a = cabs(A[j]);
+        nl[j] = A[j] * A[j] * e + 2.0 * conj(e * (a * a));
+    }
+}
+
+/* kfn (NEE_CPU.m:103-105) with ind1 = j+1, ind2 = j-1, periodic (NEE_CPU.m:22-23):
+ * K = -h i Gk (nl - i (0.5/dt/w0) (nl[j+1] - nl[j-1])) */
+static void kfn(int n, const cplx *nl, double Gk, double w0, double hs, double dt, cplx *K) {
+    const double c = 0.5 / dt / w0;
+    const cplx pre = -hs * I * Gk;
+    for (int j = 0; j < n; j++) {
+        cplx up = nl[(j + 1) % n];
+        cplx dn = nl[(j - 1 + n) % n];
+        K[j] = pre * (nl[j] - I * c * (up - dn));
+    }
+}
+
+/* abcfn (NEE_CPU.m:114-121): absorbing boundary on the outer eighth of each side */
+static void abcfn(int n, cplx *E, const double *ts) {
+    const double a = 24.0, b = 0.25;
+    double tmin = ts[0], tmax = ts[0];
+    for (int j = 1; j < n; j++) {
+        if (ts[j] < tmin) tmin = ts[j];
+        if (ts[j] > tmax) tmax = ts[j];
+    }
+    const double tr = tmax - tmin;  /* range(t) */
+    for (int j = 0; j < n; j++) {
+        double at2 = fabs(2.0 * ts[j]);
+        if (tr - at2 < b * tr) {
+            double s = 1.0 / cosh(a * (1.0 - at2 / tr));
+            E[j] *= (1.0 - s * s);
+        }
+    }
+}
+
+/* dfn (NEE_CPU.m:107-112): Ek = fft(E) .* exp(-i barg s) .* T^s ; E = ifft(Ek) */
+static void dfn(const fft_plan *p, cplx *E, const double *Tstep, const double *barg, double s,
+                cplx *spec_out) {
+    const int n = p->n;
+    fft_exec(p, E, -1);
+    for (int j = 0; j < n; j++) {
+        double ph = -barg[j] * s;
+        E[j] *= (cos(ph) + I * sin(ph)) * pow(Tstep[j], s);
+    }
+    if (spec_out) memcpy(spec_out, E, sizeof(cplx) * (size_t)n);
+    fft_exec(p, E, +1);
+}
+
+static int any_nan(int n, const cplx *E) {
+    for (int j = 0; j < n; j++)
+        if (isnan(creal(E[j])) || isnan(cimag(E[j]))) return 1;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Adaptive Heun split-step pass: NEE_CPU.m:1-121 restated exactly, FP64.     */
+/*                                                                           */
+/* Et     : N complex (interleaved), fftshifted time, in/out                  */
+/* xT_h   : N, alpha_w = fftshift(T^(1/NSteps)) (NonlinearCrystal.m:141-144)   */
+/* G33    : nSteps poling-signed coupling (OpticalSim.m:217)                  */
+/* hBarg  : N, fftshift(phi_rel/L*h) (OpticalSim.m:223)                        */
+/* ApFT   : N x nCols complex, column chunk-1 = fft(Et) at chunk end (:93)    */
+/* stepMods: nSteps or NULL; stepMods[k-1] = stepmod of the step accepted at  */
+/*          1-based k.  (NEE_CPU.m never writes it; the CUDA driver records    */
+/*          the same quantity, OPO_CUDA_BATCH_XTAL.cu:277-280.)                 */
+/* counters: [0] attempted steps, [1] accepted steps, [2] final k (1-based)    */
+/*                                                                           */
+/* Kept quirks (they define the reference's FP64 semantics):                  */
+/*  - k advances by the UPDATED stepmod (NEE_CPU.m:77-85);                     */
+/*  - both Heun stages use G33(k) of the step start (:54,:61);                 */
+/*  - nChunks = nSteps/sel iterates floor(nChunks) chunks (:39-41);            */
+/*  - max-norm error against |Et|+1 with strict '>' (:65-68,:77);              */
+/*  - NaN in Et leaves the chunk loop early (:44-46).                          */
+/* Pinned by: linear closed form (Chi2 = 0), Heun 2nd-order convergence with   */
+/* beta = 0, energy conservation with alpha = 1, low-gain SHG sinc^2 law.      */
+/* ------------------------------------------------------------------------- */
+int orc_nee_heun(int n, double *Et_io, const double *xT_h, const double *G33, double w0,
+                 double bdiffw0, double h, int nSteps, double dt, const double *hBarg,
+                 double maxErr, double minErr, double sel, double *ApFT_io, int nCols,
+                 double *stepMods, long long *counters) {
+    if (n < 4 || (n & (n - 1)) || nSteps < 1 || sel <= 0) return ORC_EARG;
+    cplx *Et = (cplx *)Et_io;
+    cplx *ApFT = (cplx *)ApFT_io;
+    fft_plan p;
+    int st = plan_init(&p, n);
+    if (st) { plan_free(&p); return st; }
+    double *ts = (double *)malloc(sizeof(double) * (size_t)n);
+    cplx *A0 = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    cplx *A1 = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    cplx *NL = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    cplx *K1 = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    cplx *K2 = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    if (!ts || !A0 || !A1 || !NL || !K1 || !K2) { st = ORC_ENOMEM; goto done; }
+    centred_time(n, dt, ts);
+
+    double k = 1.0;
+    double stepmod = 4.0;
+    const double nChunks = (double)nSteps / sel;
+    long long attempts = 0, accepted = 0;
+    int nonfinite = 0;
+
+    for (int chunk = 1; (double)chunk <= nChunks; chunk++) {
+        while (k < chunk * sel && k <= nSteps) {
+            if (any_nan(n, Et)) { nonfinite = 1; break; }
+            const double Gk = G33[(int)k - 1];
+            double bdwz = bdiffw0 * (k - 1.0) * h;
+            memcpy(A0, Et, sizeof(cplx) * (size_t)n);
+            nlfn(n, A0, ts, w0, bdwz, NL);
+            kfn(n, NL, Gk, w0, h * stepmod, dt, K1);
+            for (int j = 0; j < n; j++) A1[j] = A0[j] + K1[j];
+            bdwz = bdiffw0 * (k - 1.0 + stepmod) * h;
+            nlfn(n, A1, ts, w0, bdwz, NL);
+            kfn(n, NL, Gk, w0, h * stepmod, dt, K2);
+            /* MATLAB max() ignores NaN entries and is NaN only if all are NaN */
+            double pcterr = NAN;
+            for (int j = 0; j < n; j++) {
+                Et[j] = A0[j] + 0.5 * K1[j] + 0.5 * K2[j];
+                double e = cabs(Et[j] - A1[j]) / (cabs(Et[j]) + 1.0);
+                if (!isnan(e) && (isnan(pcterr) || e > pcterr)) pcterr = e;
+            }
+            pcterr *= 100.0;
+            attempts++;
+            if (pcterr > maxErr && stepmod > 1) {
+                stepmod = stepmod - 1;
+                memcpy(Et, A0, sizeof(cplx) * (size_t)n);
+            } else {
+                abcfn(n, Et, ts);
+                dfn(&p, Et, xT_h, hBarg, stepmod, NULL);
+                accepted++;
+                if (stepMods) stepMods[(int)k - 1] = stepmod;
+                if (pcterr < minErr) stepmod = stepmod + 1;
+                if (k < nSteps && (k + stepmod) > nSteps) stepmod = (double)nSteps - k;
+                k = k + stepmod;
+            }
+        }
+        if (ApFT && chunk <= nCols) {
+            cplx *col = ApFT + (size_t)(chunk - 1) * (size_t)n;
+            memcpy(col, Et, sizeof(cplx) * (size_t)n);
+            fft_exec(&p, col, -1);
+        }
+    }
+    if (counters) {
+        counters[0] = attempts;
+        counters[1] = accepted;
+        counters[2] = (long long)k;
+    }
+    st = nonfinite ? ORC_ENONFINITE : ORC_OK;
+done:
+    free(ts); free(A0); free(A1); free(NL); free(K1); free(K2);
+    plan_free(&p);
+    return st;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Adaptive Bogacki-Shampine RK23 split-step pass, restating the control flow */
+/* of ode23/OPO_CUDA_ODE23.cu:375-557 in FP64 for one instance.               */
+/*                                                                           */
+/* Kept (reference RK23 semantics):                                          */
+/*  - FSAL: after an accepted step K1 <- K4 computed from the pre-dispersion  */
+/*    field with the old step size (:512-514), A1 = A0 + K1/2 (:516);          */
+/*  - rejection: k -= s; s -= 1; K1 recomputed at z = (k+s)h (:477-489), the   */
+/*    loop increment then gives a net k-1 start (:432 pattern);                */
+/*  - error 200*|E-A1|/(|E|+1) over points with |E|>0, '>=' tests (:332-346);  */
+/*  - chunk-end truncation and step doubling (:519-538);                        */
+/*  - Nchunks = Nsteps/sel, integer division (:411).                           */
+/* Deliberate differences (same as the Heun path): centred time axis          */
+/* (NEE_CPU.m:53) instead of gx*dt; the NEE_CPU absorber (NEE_CPU.m:114-121); */
+/* exact rational tableau instead of float-rounded constants (:425-427).      */
+/* Pinned by: 3rd-order global convergence with beta = 0 (a wrong tableau     */
+/* entry or stage abscissa drops the order), linear closed form.              */
+/* ------------------------------------------------------------------------- */
+int orc_rk23(int n, double *Et_io, const double *xT_h, const double *G, double w0,
+             double bdiffw0, double h, int Nsteps, double dt, const double *hBarg,
+             double maxpe, double minpe, int sel, double *ApFT_io, int nCols,
+             double *stepMods, long long *counters) {
+    if (n < 4 || (n & (n - 1)) || Nsteps < 1 || sel < 1) return ORC_EARG;
+    cplx *A0 = (cplx *)Et_io;
+    cplx *ApFT = (cplx *)ApFT_io;
+    fft_plan p;
+    int st = plan_init(&p, n);
+    if (st) { plan_free(&p); return st; }
+    double *ts = (double *)malloc(sizeof(double) * (size_t)n);
+    cplx *A1 = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    cplx *E = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    cplx *NL = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    cplx *Kb[4];
+    for (int i = 0; i < 4; i++) Kb[i] = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    cplx *spec = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    if (!ts || !A1 || !E || !NL || !Kb[0] || !Kb[1] || !Kb[2] || !Kb[3] || !spec) {
+        st = ORC_ENOMEM;
+        goto done;
+    }
+    centred_time(n, dt, ts);
+    {
+        cplx *K1 = Kb[0], *K2 = Kb[1], *K3 = Kb[2], *K4 = Kb[3];
+        const double a1 = 1.0 / 2.0, a2 = 3.0 / 4.0, a3 = 4.0 / 9.0, a4 = 1.0 / 3.0, a5 = 2.0 / 9.0;
+        const double b1 = 1.0 / 8.0, b2 = 1.0 / 3.0, b3 = 1.0 / 4.0, b4 = 7.0 / 24.0;
+        double k = 0.0, stepprev = 4.0, stepmod, stepend, hmod;
+        double bdwz;
+        long long attempts = 0, accepted = 0;
+        const int Nchunks = Nsteps / sel;
+        int nonfinite = 0;
+
+        stepmod = stepprev;
+        hmod = stepmod * h;
+        bdwz = bdiffw0 * k * h;
+        nlfn(n, A0, ts, w0, bdwz, NL);
+        kfn(n, NL, G[(int)k], w0, hmod, dt, K1);
+        for (int j = 0; j < n; j++) A1[j] = A0[j] + a1 * K1[j];
+
+        for (int chunk = 0; chunk < Nchunks && !nonfinite; chunk++) {
+            stepmod = stepprev;
+            hmod = stepmod * h;
+            const double cend = (double)(chunk + 1) * sel;
+            for (; k < cend; k = k + stepmod) {
+                if ((int)k >= Nsteps) { st = ORC_EARG; goto done; }
+                const double Gk = G[(int)k];
+                bdwz = bdiffw0 * (k + a1 * stepmod) * h;
+                nlfn(n, A1, ts, w0, bdwz, NL);
+                kfn(n, NL, Gk, w0, hmod, dt, K2);
+                for (int j = 0; j < n; j++) A1[j] = A0[j] + a2 * K2[j];
+                bdwz = bdiffw0 * (k + a2 * stepmod) * h;
+                nlfn(n, A1, ts, w0, bdwz, NL);
+                kfn(n, NL, Gk, w0, hmod, dt, K3);
+                for (int j = 0; j < n; j++) E[j] = A0[j] + (a3 * K3[j] + a4 * K2[j]) + a5 * K1[j];
+                bdwz = bdiffw0 * (k + stepmod) * h;
+                nlfn(n, E, ts, w0, bdwz, NL);
+                kfn(n, NL, Gk, w0, hmod, dt, K4);
+                for (int j = 0; j < n; j++)
+                    A1[j] = A0[j] + (b1 * K4[j] + b2 * K3[j]) + b3 * K2[j] + b4 * K1[j];
+                double pmax = -1.0;
+                for (int j = 0; j < n; j++) {
+                    double aE = cabs(E[j]);
+                    if (aE > 0) {
+                        double pct = 4.0 * 100.0 * (0.5 * cabs(A1[j] - E[j]) / (aE + 1.0));
+                        if (pct > pmax) pmax = pct;
+                    }
+                    if (isnan(creal(E[j])) || isnan(cimag(E[j]))) nonfinite = 1;
+                }
+                attempts++;
+                if (nonfinite) break;
+                const int flag0 = pmax >= maxpe;
+                const int flag1 = !flag0 && pmax >= minpe;
+                if (flag0 && stepmod > 1 && (k - stepmod) > 0) {
+                    k = k - stepmod;
+                    stepmod = stepmod - 1;
+                    hmod = stepmod * h;
+                    stepprev = stepmod;
+                    bdwz = bdiffw0 * (k + stepmod) * h;
+                    nlfn(n, A0, ts, w0, bdwz, NL);
+                    kfn(n, NL, G[(int)k], w0, hmod, dt, K1);
+                    for (int j = 0; j < n; j++) A1[j] = A0[j] + a1 * K1[j];
+                } else {
+                    abcfn(n, E, ts);
+                    memcpy(A0, E, sizeof(cplx) * (size_t)n);
+                    dfn(&p, A0, xT_h, hBarg, stepmod, spec);
+                    accepted++;
+                    if (ApFT && chunk < nCols)
+                        memcpy(ApFT + (size_t)chunk * (size_t)n, spec, sizeof(cplx) * (size_t)n);
+                    if (stepMods && (int)k < Nsteps) stepMods[(int)k] = stepmod;
+                    cplx *tmp = K1; K1 = K4; K4 = tmp;
+                    for (int j = 0; j < n; j++) A1[j] = A0[j] + a1 * K1[j];
+                    if (k + stepmod >= cend && k < cend - 1) {
+                        stepend = (cend - k) - 1;
+                        hmod = stepend * h;
+                        stepprev = stepmod;
+                        stepmod = stepend;
+                    } else {
+                        if (!flag1 && !flag0 && (k + stepmod + stepmod) < cend - 1) {
+                            stepmod = stepmod * 2;
+                            hmod = stepmod * h;
+                        }
+                        stepprev = stepmod;
+                    }
+                }
+            }
+        }
+        if (counters) {
+            counters[0] = attempts;
+            counters[1] = accepted;
+            counters[2] = (long long)k;
+        }
+        if (st == ORC_OK && nonfinite) st = ORC_ENONFINITE;
+    }
+done:
+    free(ts); free(A1); free(E); free(NL);
+    for (int i = 0; i < 4; i++) free(Kb[i]);
+    free(spec);
+    plan_free(&p);
+    return st;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Solver-input producers (OpticalSim.m:213-226 and the objects behind it).   */
+/* ------------------------------------------------------------------------- */
+
+/* Gayer 5% MgO:CLN extraordinary index (n_mgoppln_gayer.m:11-37, column 1;   */
+/* also columns 2-4).  MATLAB's sqrt of a negative argument is complex and    */
+/* callers take the real part (phi_calc.m:22, Dielectric.m:43), so a negative */
+/* argument gives 0 here.  Pinned by: the published value n_e(1.04um,20C).    */
+double orc_gayer_n(double lam_um, double T, int col) {
+    static const double A1[4] = {5.756, 5.653, 5.078, 5.35583};
+    static const double A2[4] = {0.0983, 0.1185, 0.0964, 0.100473};
+    static const double A3[4] = {0.2020, 0.2091, 0.2065, 0.20692};
+    static const double A4[4] = {189.32, 89.61, 61.16, 100};
+    static const double A5[4] = {12.52, 10.85, 10.55, 11.34927};
+    static const double A6[4] = {1.32e-2, 1.97e-2, 1.59e-2, 1.5334e-2};
+    static const double B1[4] = {2.860e-6, 7.941e-7, 4.677e-6, 4.629e-7};
+    static const double B2[4] = {4.700e-8, 3.134e-8, 7.822e-8, 3.862e-8};
+    static const double B3[4] = {6.113e-8, -4.641e-9, -2.653e-8, -0.89e-8};
+    static const double B4[4] = {1.516e-4, -2.188e-6, 1.096e-4, 2.657e-5};
+    int c = col - 1;
+    if (c < 0 || c > 3) return NAN;
+    double fn = (T - 24.5) * (T + 570.82);
+    double l2 = lam_um * lam_um;
+    double a3 = A3[c] + B3[c] * fn;
+    double x = (A1[c] + B1[c] * fn) + (A2[c] + B2[c] * fn) / (l2 - a3 * a3) +
+               (A4[c] + B4[c] * fn) / (l2 - A5[c] * A5[c]) - A6[c] * l2;
+    return x > 0 ? sqrt(x) : 0.0;
+}
+
+/* phi_calc.m:5-45 for one PPLN length, and phi2GD.m:1-17.
+ * w_abs, w_rel: N angular frequencies (SimWindow order).  ref: index of w0.
+ * Outputs (any may be NULL):
+ *   phi_rel[N]: relative phase; NaN outside 0.2-20 um (see below)
+ *   phi_abs[N]: L*w*n/c
+ *   gd[N]     : phi2GD(phi_abs, dw) (GD(1) duplicated at index 0)
+ * Difference: phi_calc.m:40 spline-interpolates the cumtrapz result back onto
+ * the full grid.  At the knots (every grid point inside 0.2-20 um) the spline
+ * returns the knot values, so those are used directly; outside that band the
+ * spline extrapolates and the result is NaN here.  The solver never uses those
+ * points because T = 0 outside the 300-6000 nm SpectralLimits
+ * (transmission.m:28), which lie inside the knot band.
+ * Pinned by: numpy diff/cumulative trapezoid on the same arrays. */
+int orc_phi_calc(int n, double L, double T, const double *w_abs, const double *w_rel, int ref,
+                 double *phi_rel, double *phi_abs_out, double *gd_out) {
+    const double c0 = 299792458.0;
+    double *lam = (double *)malloc(sizeof(double) * (size_t)n);
+    double *pa = (double *)malloc(sizeof(double) * (size_t)n);
+    double *gdrel = (double *)malloc(sizeof(double) * (size_t)n);
+    double *dphi = (double *)malloc(sizeof(double) * (size_t)n);
+    if (!lam || !pa || !gdrel || !dphi) { free(lam); free(pa); free(gdrel); free(dphi); return ORC_ENOMEM; }
+    for (int j = 0; j < n; j++) {
+        lam[j] = 2 * M_PI * c0 / w_abs[j];
+        double nm = orc_gayer_n(lam[j] * 1e6, T, 1);
+        double beta = w_abs[j] * nm / c0;
+        pa[j] = L * beta;
+    }
+    /* GD_abs = [0 diff(phi)/diff(w)] ; GD_rel = GD_abs - GD_abs(ref) */
+    gdrel[0] = 0.0;
+    for (int j = 1; j < n; j++) gdrel[j] = (pa[j] - pa[j - 1]) / (w_abs[j] - w_abs[j - 1]);
+    double g0 = gdrel[ref];
+    for (int j = 0; j < n; j++) gdrel[j] -= g0;
+    dphi[0] = 0.0;
+    for (int j = 1; j < n; j++) dphi[j] = gdrel[j] * (w_rel[j] - w_rel[j - 1]);
+    if (phi_rel) {
+        /* cumtrapz over the contiguous mask 2e-7 < lam < 2e-5 (unit spacing) */
+        double acc = 0.0;
+        int first = 1;
+        double prev = 0.0;
+        for (int j = 0; j < n; j++) {
+            int in = (lam[j] > 2e-7) && (lam[j] < 2e-5);
+            if (!in) { phi_rel[j] = NAN; continue; }
+            if (first) { acc = 0.0; first = 0; }
+            else acc += 0.5 * (prev + dphi[j]);
+            prev = dphi[j];
+            phi_rel[j] = acc;
+        }
+        double r0 = phi_rel[ref];
+        for (int j = 0; j < n; j++) phi_rel[j] -= r0;
+    }
+    if (phi_abs_out) memcpy(phi_abs_out, pa, sizeof(double) * (size_t)n);
+    if (gd_out) {
+        /* phi2GD.m: phi_r = phi - phi(n/2) (1-based); GD = diff(phi_r)/dw; GD = [GD(1) GD] */
+        double dw = w_abs[1] - w_abs[0];
+        double off = pa[n / 2 - 1];
+        for (int j = 1; j < n; j++) gd_out[j] = ((pa[j] - off) - (pa[j - 1] - off)) / dw;
+        gd_out[0] = gd_out[1];
+    }
+    free(lam); free(pa); free(gdrel); free(dphi);
+    return ORC_OK;
+}
+
+/* QPMcrystal.m:1-57 with uncertainty = 0 and duty offset dutyOff.
+ * P[i] = +-1 for i < nSteps at z = linspace(0,L,nSteps).
+ * Pinned by: exact square-wave (domain count and the sign of the first domain). */
+int orc_qpm_poling(int nSteps, double L, double period, double dutyOff, double *P) {
+    if (nSteps < 2 || period <= 0) return ORC_EARG;
+    int nD = (int)floor(2.0 * L / period);
+    double poled = period * (1.0 - 2.0 * dutyOff) / 2.0;
+    /* domain widths: normrnd(poled, 0) = poled; even domains period - movmean(odd, 2) */
+    int nw = nD + 1;
+    double *w = (double *)calloc((size_t)nw + 1, sizeof(double));
+    double *walls = (double *)calloc((size_t)nw + 2, sizeof(double));
+    if (!w || !walls) { free(w); free(walls); return ORC_ENOMEM; }
+    for (int i = 0; i < nD; i++) w[i] = poled;
+    for (int i = 1; i + 1 < nD; i += 2) {
+        /* domain_widths(2:2:nDs-1) = grating - movmean(widths(1:2:nDs),2,'discard') */
+        w[i] = period - 0.5 * (w[i - 1] + w[i + 1]);
+    }
+    int nwalls = nD + 1;
+    walls[0] = 0.0;
+    for (int i = 0; i < nD; i++) walls[i + 1] = walls[i] + w[i];
+    if (walls[nD] < L) {
+        walls[nD + 1] = L;
+        nwalls = nD + 2;
+    } else {
+        walls[nD] = L;
+    }
+    int bin = 0;
+    for (int i = 0; i < nSteps; i++) {
+        double z = (i == nSteps - 1) ? L : ((double)i * L) / (double)(nSteps - 1);
+        while (bin + 1 < nwalls - 1 && z >= walls[bin + 1]) bin++;
+        int q = (bin + 1) % 2;           /* discretize gives 1-based bin index */
+        P[i] = -(2.0 * q - 1.0);
+    }
+    free(w);
+    free(walls);
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Spectral operations of OpticalPulse (OpticalPulse.m:283-316).               */
+/* SpectralField = fftshift(fft(fftshift(E))) ; k2t = ifftshift(ifft(ifftshift)). */
+/* The reference uses a 2N zero-padded transform decimated by 2, which is the  */
+/* N-point transform exactly (even-indexed bins of the padded DFT).            */
+/* ------------------------------------------------------------------------- */
+static void shift_half(int n, cplx *x, cplx *tmp) {
+    for (int j = 0; j < n; j++) tmp[j] = x[(j + n / 2) % n];
+    memcpy(x, tmp, sizeof(cplx) * (size_t)n);
+}
+
+static void t2k(const fft_plan *p, cplx *x, cplx *tmp) {
+    shift_half(p->n, x, tmp);
+    fft_exec(p, x, -1);
+    shift_half(p->n, x, tmp);
+}
+
+static void k2t(const fft_plan *p, cplx *x, cplx *tmp) {
+    shift_half(p->n, x, tmp);
+    fft_exec(p, x, +1);
+    shift_half(p->n, x, tmp);
+}
+
+/* ------------------------------------------------------------------------- */
+/* OPO round trips: OpticalSim.run (OpticalSim.m:207-301) for one instance,   */
+/* restated with the literal operation order:                                 */
+/*   pump (OpticalSim.m:361-364): E += E_pump (OpticalPulse.m:518-527, equal  */
+/*        media and areas so TemporalRootPower addition is field addition);   */
+/*        refract into crystal: Ek *= sqrt(n_air/n_xtal) (OpticalPulse.m:190)  */
+/*   solver pass on fftshift(E) (OpticalSim.m:234-254)                         */
+/*   refract to air (OpticalSim.m:267): Ek *= sqrt(n_xtal/n_air)               */
+/*   detect (OpticalSim.m:346-359): output copy Ek*sqrt(Toc)*e^{-i phi_oc},    */
+/*        intracavity Ek*sqrt(Roc)*e^{-i phi_oc} (OpticalPulse.m:137-188)      */
+/*   applyGD(delay) (OpticalSim.m:284, OpticalPulse.m:256-261):               */
+/*        Ek *= exp(-i delay (w - w0))                                         */
+/* All spectral arrays here are in SimWindow (centred) order.                  */
+/* Fields are natural-order time (Pulse.TemporalField).                        */
+/* NEW (no reference counterpart): per-trip convergence metric               */
+/*   conv[t] = ||S_t - S_{t-1}||_2 / ||S_t||_2, S = |SpectralField(E_oc)|^2.  */
+/* method: 0 Heun (orc_nee_heun), 1 RK23 (orc_rk23).                           */
+/* Outputs: E_cav (in/out), E_oc_last (field of the last trip's output),       */
+/*   oc_energy[trips] = sum |Ek_oc|^2 (proportional to pulse energy),         */
+/*   conv[trips], steps[2*trips] (attempts, accepted).                         */
+/* ------------------------------------------------------------------------- */
+int orc_opo_run(int n, int method, int trips, double *Ecav_io, const double *Epump_in,
+                const double *n_air, const double *n_xtal, const double *Roc, const double *Toc,
+                const double *phi_oc, double delay, const double *w_rel,
+                /* solver arguments */
+                const double *xT_h, const double *G33, double w0, double bdiffw0, double h,
+                int nSteps, double dt, const double *hBarg, double maxErr, double minErr,
+                double sel, int nCols,
+                /* outputs */
+                double *Eoc_last, double *oc_energy, double *conv, long long *steps,
+                double conv_tol, int *trips_done) {
+    cplx *E = (cplx *)Ecav_io;
+    const cplx *Ep = (const cplx *)Epump_in;
+    fft_plan p;
+    int st = plan_init(&p, n);
+    if (st) { plan_free(&p); return st; }
+    cplx *tmp = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    cplx *Es = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    cplx *Eoc = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    double *S = (double *)malloc(sizeof(double) * (size_t)n);
+    double *Sprev = (double *)malloc(sizeof(double) * (size_t)n);
+    cplx *ApFT = (cplx *)malloc(sizeof(cplx) * (size_t)n * (size_t)(nCols > 0 ? nCols : 1));
+    if (!tmp || !Es || !Eoc || !S || !Sprev || !ApFT) { st = ORC_ENOMEM; goto done; }
+    int have_prev = 0;
+    int t;
+    for (t = 0; t < trips; t++) {
+        /* pump + refract into the crystal */
+        for (int j = 0; j < n; j++) E[j] += Ep[j];
+        t2k(&p, E, tmp);
+        for (int j = 0; j < n; j++) E[j] *= sqrt(n_air[j] / n_xtal[j]);
+        k2t(&p, E, tmp);
+        /* solver on the fftshifted field */
+        for (int j = 0; j < n; j++) Es[j] = E[(j + n / 2) % n];
+        long long cnt[3] = {0, 0, 0};
+        if (method == 0)
+            st = orc_nee_heun(n, (double *)Es, xT_h, G33, w0, bdiffw0, h, nSteps, dt, hBarg,
+                              maxErr, minErr, sel, (double *)ApFT, nCols, NULL, cnt);
+        else
+            st = orc_rk23(n, (double *)Es, xT_h, G33, w0, bdiffw0, h, nSteps, dt, hBarg, maxErr,
+                          minErr, (int)sel, (double *)ApFT, nCols, NULL, cnt);
+        if (steps) { steps[2 * t] = cnt[0]; steps[2 * t + 1] = cnt[1]; }
+        if (st) break;
+        for (int j = 0; j < n; j++) E[j] = Es[(j + n / 2) % n];
+        /* refract to air */
+        t2k(&p, E, tmp);
+        for (int j = 0; j < n; j++) E[j] *= sqrt(n_xtal[j] / n_air[j]);
+        k2t(&p, E, tmp);
+        /* detect: output copy and intracavity reflection */
+        memcpy(Eoc, E, sizeof(cplx) * (size_t)n);
+        t2k(&p, Eoc, tmp);
+        for (int j = 0; j < n; j++) Eoc[j] *= sqrt(Toc[j]) * cexp(-I * phi_oc[j]);
+        double en = 0.0, d2 = 0.0, s2 = 0.0;
+        for (int j = 0; j < n; j++) {
+            S[j] = creal(Eoc[j]) * creal(Eoc[j]) + cimag(Eoc[j]) * cimag(Eoc[j]);
+            en += S[j];
+            s2 += S[j] * S[j];
+            if (have_prev) d2 += (S[j] - Sprev[j]) * (S[j] - Sprev[j]);
+        }
+        if (oc_energy) oc_energy[t] = en;
+        double r = have_prev ? (s2 > 0 ? sqrt(d2 / s2) : 0.0) : INFINITY;
+        if (conv) conv[t] = r;
+        memcpy(Sprev, S, sizeof(double) * (size_t)n);
+        have_prev = 1;
+        k2t(&p, Eoc, tmp);
+        if (Eoc_last) memcpy(Eoc_last, Eoc, sizeof(cplx) * (size_t)n);
+        t2k(&p, E, tmp);
+        for (int j = 0; j < n; j++) E[j] *= sqrt(Roc[j]) * cexp(-I * phi_oc[j]);
+        k2t(&p, E, tmp);
+        /* cavity detuning */
+        t2k(&p, E, tmp);
+        for (int j = 0; j < n; j++) E[j] *= cexp(-I * (delay * w_rel[j]));
+        k2t(&p, E, tmp);
+        if (conv_tol > 0 && r < conv_tol) { t++; break; }
+    }
+    if (trips_done) *trips_done = t;
+done:
+    free(tmp); free(Es); free(Eoc); free(S); free(Sprev); free(ApFT);
+    plan_free(&p);
+    return st;
+}

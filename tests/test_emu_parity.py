@@ -1,0 +1,123 @@
+"""The persistent pass kernel's body (optifax_amd/csrc/ofx_pass_impl.h),
+compiled for the CPU (tests/emu/libofx_emu.so, one thread per workgroup), run
+through the unchanged runtime layer and compared with the FP64 oracle.  This
+checks the kernel's indexing, four-step FFT, halo handling, step control and
+cavity stage on a machine without a GPU; the GPU tests repeat the same checks
+on the device."""
+import numpy as np
+import pytest
+
+import oracle
+from optifax_amd import OfxError
+from optifax_amd.solver import OPOBatch, OPOmexBatch
+
+from cases import crystal_case, opo_case, oracle_next_input, rel_l2
+
+
+def _run(ctx, E, args, n, precision, C, apft_cols=4):
+    ns = args[5]
+    return OPOmexBatch(E, *args, np.zeros((n, apft_cols), complex), np.zeros(ns), ctx=ctx,
+                       precision=precision, tune={"cluster": C})
+
+
+@pytest.mark.parametrize("n,C,window", [(256, 1, 2e-12), (256, 2, 2e-12), (256, 4, 2e-12),
+                                        (1024, 2, 4e-12), (1024, 8, 4e-12)])
+def test_pass_fp64_matches_oracle(emu_ctx, n, C, window):
+    sw, si, E, args = crystal_case(n=n, window=window)
+    Eo, apo, smo, cnt = oracle.nee_heun(E, *args, n_cols=4)
+    Eg, apg, smg = _run(emu_ctx, E, args, n, "double", C)
+    assert np.array_equal(smg, smo)                 # identical adaptive step sequence
+    assert rel_l2(Eg, Eo) < 1e-10
+    assert rel_l2(apg, apo) < 1e-10
+
+
+def test_pass_fp32_tracks_oracle(emu_ctx):
+    sw, si, E, args = crystal_case(n=512, window=3e-12)
+    Eo, apo, smo, _ = oracle.nee_heun(E, *args, n_cols=4)
+    Eg, apg, smg = _run(emu_ctx, E, args, 512, "single", 2)
+    assert np.array_equal(smg, smo)
+    # FP32 fields over ~750 high-gain steps (the error budget is in DESIGN.md)
+    assert rel_l2(Eg, Eo) < 2e-4
+
+
+def test_round_trips_match_literal_opticalsim(emu_ctx):
+    cfg, b = opo_case()
+    si, ops = b["solver"], b["ops"]
+    sw = b["simwin"]
+    trips = 3
+    o = oracle.opo_run(trips, b["noise"], b["pump"], np.ones(sw.n), si["n_xtal"],
+                       np.full(sw.n, 1 - cfg.oc_transmission), np.full(sw.n, cfg.oc_transmission),
+                       np.zeros(sw.n), cfg.delay, sw.rel_omegas, si)
+    for C in (1, 2):
+        run = OPOBatch([dict(solver=si, field0=ops["field0"], pump=ops["pump"], cav=ops["cav"],
+                             ocm=ops["ocm"], delay=cfg.delay)], precision="double", trip_cap=trips,
+                       ctx=emu_ctx, tune={"cluster": C})
+        run.launch(trips)
+        run.sync()
+        r = run.read(0)
+        assert r["trips_done"] == trips
+        assert np.array_equal(r["trip_log"][:, 2:].astype(int), o["steps"])
+        assert rel_l2(r["field"], oracle_next_input(b, o["E_cav"])) < 1e-10
+        from optifax_amd.physics import SimWindow
+        assert rel_l2(r["oc_spec"], SimWindow.fft_order(SimWindow.t2k(o["E_oc"]))) < 1e-10
+        assert np.allclose(r["trip_log"][1:, 0], o["conv"][1:], rtol=1e-9)
+        assert np.allclose(r["trip_log"][:, 1], o["oc_energy"], rtol=1e-10)
+        run.close()
+
+
+def test_batch_invariance_and_per_instance_G(emu_ctx):
+    sw, si, E, args = crystal_case(n=256, window=2e-12)
+    n, ns = 256, args[5]
+    B = 3
+    Eb = np.stack([E] * B, axis=1)
+    Eg, apg, smg = OPOmexBatch(Eb, *args, np.zeros((n, 4, B), complex), np.zeros(ns), ctx=emu_ctx,
+                               precision="double", tune={"cluster": 2})
+    E1, _, _ = _run(emu_ctx, E, args, n, "double", 2)
+    for b in range(B):
+        assert np.array_equal(Eg[:, b], E1)
+    # per-instance poling (G is NSteps x B, OPO_CUDA_BATCH_XTAL.cu:222): flip one sign pattern
+    G = np.stack([args[1], -args[1], args[1]], axis=1)
+    a2 = list(args)
+    a2[1] = G
+    Eg2, _, _ = OPOmexBatch(Eb, *a2, np.zeros((n, 4, B), complex), np.zeros(ns), ctx=emu_ctx,
+                            precision="double")
+    a3 = list(args)
+    a3[1] = -args[1]
+    Eo, _, _, _ = oracle.nee_heun(E, *a3, n_cols=4)
+    assert rel_l2(Eg2[:, 1], Eo) < 1e-10
+    assert np.array_equal(Eg2[:, 0], Eg2[:, 2])
+
+
+def test_nonfinite_input_reports_status(emu_ctx):
+    sw, si, E, args = crystal_case(n=256, window=2e-12)
+    E2 = np.stack([E, E], axis=1)
+    E2[5, 1] = np.nan
+    Eg, _, _, st = OPOmexBatch(E2, *args, np.zeros((256, 4, 2), complex), np.zeros(args[5]),
+                               ctx=emu_ctx, precision="double", return_status=True)
+    assert list(st) == [1, 3]             # DONE, NONFINITE
+    assert np.all(np.isfinite(Eg[:, 0]))
+
+
+def test_forced_rejection_terminates(emu_ctx):
+    # maxErr = 0: every attempt above stepmod 1 is rejected; the stepmod > 1
+    # guard (NEE_CPU.m:68) makes the pass finish with unit steps.
+    sw, si, E, args = crystal_case(n=256, window=2e-12, L=0.02e-3)
+    a = list(args)
+    a[8], a[9] = 0.0, -1.0
+    Eo, _, smo, cnt = oracle.nee_heun(E, *a, n_cols=4)
+    Eg, _, smg = _run(emu_ctx, E, a, 256, "double", 2)
+    assert np.array_equal(smg, smo) and set(np.unique(smo[smo > 0])) == {1.0}
+    assert rel_l2(Eg, Eo) < 1e-10
+
+
+def test_argument_errors(emu_ctx):
+    sw, si, E, args = crystal_case(n=256, window=2e-12)
+    with pytest.raises(ValueError):
+        OPOmexBatch(E, args[0], args[1][:-1], *args[2:], None, None, ctx=emu_ctx)
+    a = list(args)
+    a[10] = 1.0                      # sel must exceed 1
+    with pytest.raises(OfxError):
+        OPOmexBatch(E, *a, None, None, ctx=emu_ctx)
+    with pytest.raises(OfxError):
+        OPOmexBatch(E[:100], args[0][:100], args[1], *args[2:7], args[7][:100], *args[8:], None,
+                    None, ctx=emu_ctx)      # N not a power of two

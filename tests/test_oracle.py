@@ -1,0 +1,208 @@
+"""Pins of the FP64 CPU oracle (oracle/oracle.c) against checks that do not
+retype its formulas: brute-force DFT, numpy/scipy library routines, published
+material data, closed forms of special cases, convergence orders and the
+low-gain SHG law.  The reference ships no tests or golden vectors, so these
+are what fix the oracle."""
+import math
+
+import numpy as np
+import pytest
+from scipy.integrate import cumulative_trapezoid
+
+import oracle
+from optifax_amd import physics as ph
+
+from cases import rel_l2
+
+
+# ---------------------------------------------------------------- FFT
+@pytest.mark.parametrize("n", [4, 8, 16, 64, 256])
+def test_fft_matches_bruteforce_dft(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    j = np.arange(n, dtype=np.longdouble)
+    ang = -2 * np.pi * np.outer(j, j) / n
+    X = (np.cos(ang) * x.real - np.sin(ang) * x.imag).sum(1) + 1j * (
+        np.sin(ang) * x.real + np.cos(ang) * x.imag).sum(1)
+    y = oracle.fft(x)
+    assert np.abs(y - X.astype(np.complex128)).max() < 1e-12 * np.abs(X).max()
+    assert np.abs(oracle.fft(y, inverse=True) - x).max() < 1e-13 * np.abs(x).max()
+
+
+@pytest.mark.parametrize("n", [2 ** 13, 2 ** 15, 2 ** 16])
+def test_fft_matches_numpy_large(n):
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    assert rel_l2(oracle.fft(x), np.fft.fft(x)) < 1e-14
+    assert rel_l2(oracle.fft(x, inverse=True), np.fft.ifft(x)) < 1e-14
+
+
+# ---------------------------------------------------------------- materials
+def test_gayer_index_published_values():
+    # 5% MgO:congruent LN extraordinary index near room temperature
+    # (Gayer et al., Appl. Phys. B 91, 343 (2008); Zelmon et al., JOSA B 14,
+    # 3319 (1997) for 5% MgO:LN): n_e ~ 2.147 at 1064 nm and ~ 2.129 at
+    # 1550 nm.  The ordinary column is ~0.08 higher.
+    ne = oracle.gayer_n([1.064, 1.55], 24.5, 1)
+    assert abs(ne[0] - 2.147) < 3e-3
+    assert abs(ne[1] - 2.129) < 3e-3
+    no = oracle.gayer_n([1.064], 24.5, 2)
+    assert no[0] - ne[0] > 0.06
+    # positive thermo-optic coefficient of n_e; at 24.5 C the temperature
+    # terms of the formula vanish, so the two checks above pin the dispersion
+    # coefficients and this one the sign of the thermal terms
+    d = (oracle.gayer_n([1.064], 60.0, 1) - oracle.gayer_n([1.064], 20.0, 1))[0] / 40.0
+    assert d > 0
+    # the product's independent restatement agrees
+    lam = np.linspace(0.4, 4.5, 50)
+    assert np.allclose(ph.gayer_n(lam, 20.0), oracle.gayer_n(lam, 20.0), rtol=1e-14, atol=0)
+
+
+def test_phi_calc_reduces_to_diff_and_cumtrapz():
+    sw = ph.SimWindow(1040e-9, 2048, 6e-12)
+    L = 2e-3
+    pr, pa, gd = oracle.phi_calc(L, 20.0, sw.omegas, sw.rel_omegas, sw.reference_index)
+    lam = 2 * np.pi * ph.C0 / sw.omegas
+    n = ph.gayer_n(lam * 1e6, 20.0)
+    phi_abs = L * sw.omegas * n / ph.C0
+    assert np.allclose(pa, phi_abs, rtol=1e-15)
+    gd_abs = np.concatenate([[0], np.diff(phi_abs) / np.diff(sw.omegas)])
+    gd_rel = gd_abs - gd_abs[sw.reference_index]
+    dphi = gd_rel * np.concatenate([[0], np.diff(sw.rel_omegas)])
+    band = (lam > 2e-7) & (lam < 2e-5)
+    ref = cumulative_trapezoid(dphi[band], initial=0.0)
+    ref = ref - ref[np.nonzero(band)[0].tolist().index(sw.reference_index)]
+    assert np.allclose(pr[band], ref, rtol=1e-12, atol=1e-9)
+    assert np.all(np.isnan(pr[~band]))
+    # normal dispersion at 1040 nm: curvature of phi_rel is positive, flat GD at w0
+    i = sw.reference_index
+    curv = (pr[i + 1] - 2 * pr[i] + pr[i - 1]) / sw.d_omega ** 2 / L
+    assert 1e-25 < curv < 1e-24          # GVD of MgO:LN at 1 um ~ 3.5e-25 s^2/m
+    assert abs(pr[i]) == 0.0
+    # product restatement agrees
+    p_rel, p_abs = ph.phi_calc(L, 20.0, sw)
+    assert np.allclose(p_rel[band], pr[band], rtol=1e-13, atol=1e-10)
+
+
+def test_qpm_poling_square_wave():
+    L, period, ns = 1e-3, 30e-6, 10000
+    P = oracle.qpm_poling(ns, L, period)
+    assert P[0] == -1.0 and set(np.unique(P)) == {-1.0, 1.0}
+    z = np.arange(ns) * L / (ns - 1)
+    expect = np.where(np.floor(2 * z / period) % 2 == 0, -1.0, 1.0)
+    far = np.abs(2 * z / period - np.round(2 * z / period)) > 1e-6
+    assert np.array_equal(P[far], expect[far])
+    flips = np.count_nonzero(np.diff(P))
+    assert flips == int(math.floor(2 * L / period))
+    assert np.array_equal(ph.qpm_poling(ns, L, period), P)
+
+
+# ---------------------------------------------------------------- linear closed form
+def _narrow_pulse_case(n=1024, window=4e-12, L=0.3e-3, fwhm=50e-15, energy=1e-12, chi2=0.0):
+    sw = ph.SimWindow(1040e-9, n, window)
+    xt = ph.NonlinearCrystal(30e-6, L, chi2=chi2)
+    si = xt.solver_inputs(sw)
+    E = np.fft.fftshift(ph.sech_pump(sw, fwhm, energy, 20e-12))
+    return sw, si, E
+
+
+def test_linear_propagation_closed_form():
+    sw, si, E = _narrow_pulse_case(chi2=0.0)
+    ns = si["nSteps"]
+    sel = float(ns)             # one chunk
+    Eo, ap, sm, cnt = oracle.nee_heun(E, si["alpha_w"], si["G33"], si["w0"], si["bdiffw0"],
+                                      si["h"], ns, si["dt"], si["beta_op"], 5e-2, 2.5e-3, sel,
+                                      n_cols=1)
+    # with Chi2 = 0 every step is accepted and grows by one: 4, 5, 6, ...
+    steps = sm[sm > 0]
+    assert steps[0] == 4 and np.all(np.diff(steps)[:-1] == 1)
+    S = steps.sum()
+    # NEE_CPU.m:77-85 advances k by the UPDATED stepmod, so the k coordinate
+    # runs ahead of the field by (next step - first step); the last update is
+    # the clip nSteps - k (:81-83) and that step is never taken.
+    k_last = np.nonzero(sm)[0][-1] + 1
+    assert cnt[2] == ns
+    assert (cnt[2] - 1) - (S - steps[0]) == ns - k_last
+    Ek = np.fft.fft(E) * np.exp(-1j * si["beta_op"] * S) * si["alpha_w"] ** S
+    assert rel_l2(Eo, np.fft.ifft(Ek)) < 1e-12
+    assert rel_l2(ap[:, 0], Ek) < 1e-12
+
+
+# ---------------------------------------------------------------- convergence orders
+def _fixed_step_runs(method, ks):
+    """beta = 0, alpha = 1, constant G: the split step is exact, so the
+    integrator's own order shows.  Step fixed at 4 minimal steps."""
+    sw, si, E = _narrow_pulse_case(n=512, window=4e-12, L=0.2e-3, fwhm=60e-15, energy=5e-11,
+                                   chi2=np.pi * 14e-12)
+    beta = np.zeros(sw.n)
+    alpha = np.ones(sw.n)
+    out = []
+    L = 0.2e-3
+    for k in ks:
+        ns = 4 * k + 1
+        # Heun (NEE_CPU.m:42) stops at k = nSteps: k steps of 4h.  The RK23 loop
+        # (OPO_CUDA_ODE23.cu:432) also takes the step starting at k = 4k, with
+        # no chunk-end truncation since k = cend - 1 there: k+1 steps of 4h.
+        h = L / (ns - 1) if method == "heun" else L / (4 * (k + 1))
+        G = np.full(ns, si["G33"][0] * 0 + abs(si["G33"][0]))
+        if method == "heun":
+            Eo, _, _, cnt = oracle.nee_heun(E, alpha, G, si["w0"], si["bdiffw0"], h, ns, si["dt"],
+                                            beta, 1e30, -1.0, float(ns), n_cols=1)
+        else:
+            Eo, _, _, cnt = oracle.rk23(E, alpha, G, si["w0"], si["bdiffw0"], h, ns, si["dt"],
+                                        beta, 1e30, -1.0, ns, n_cols=1)
+        out.append(Eo)
+    return out
+
+
+def test_heun_second_order():
+    E1, E2, E3 = _fixed_step_runs("heun", [200, 400, 800])
+    r = np.linalg.norm(E1 - E2) / np.linalg.norm(E2 - E3)
+    assert 3.6 < r < 4.3, r          # 2nd order: 4
+
+
+def test_rk23_third_order():
+    E1, E2, E3 = _fixed_step_runs("rk23", [200, 400, 800])
+    r = np.linalg.norm(E1 - E2) / np.linalg.norm(E2 - E3)
+    assert 7.5 < r < 8.8, r          # 3rd order: 8 (a wrong tableau entry gives 2 or 4)
+
+
+# ---------------------------------------------------------------- low-gain SHG
+def _shg_energy(L, period, n=8192, window=10e-12):
+    sw = ph.SimWindow(1040e-9, n, window)
+    xt = ph.NonlinearCrystal(period, L, chi2=np.pi * 14e-12)
+    si = xt.solver_inputs(sw)
+    Ep = ph.sech_pump(sw, 1e-12, 1e-12, 20e-12)
+    E = np.fft.fftshift(Ep)
+    ns = si["nSteps"]
+    Eo, _, _, _ = oracle.nee_heun(E, si["alpha_w"], si["G33"], si["w0"], si["bdiffw0"], si["h"],
+                                  ns, si["dt"], si["beta_op"], 1e30, -1.0, float(ns), n_cols=1)
+    Ek = np.fft.fft(Eo)
+    f = np.fft.fftfreq(n, sw.dt)
+    sh = np.abs(f - ph.C0 / 1040e-9) < 40e12           # around 2 w0 (f_rel = +f0)
+    E_sh = np.sum(np.abs(Ek[sh]) ** 2)
+    G = abs(si["G33"][0])
+    A = np.fft.fftshift(Ep)
+    # SH driving term -i G (1 - i/w0 d/dt)[A^2 e^{i w0 t}]; the scheme's central
+    # difference (NEE_CPU.m:22-23,104) turns d/dt e^{i w0 t} into
+    # i sin(w0 dt)/dt e^{i w0 t}, so the operator gives 1 + sinc(w0 dt) (-> 2
+    # as dt -> 0); first-order QPM averages G(z) to (2/pi) G.
+    x = si["w0"] * sw.dt
+    op = 1.0 + np.sin(x) / x
+    pred = (op * (2 / np.pi) * G * L) ** 2 * np.sum(np.abs(A) ** 4) * n   # Parseval
+    return E_sh, pred
+
+
+def test_low_gain_shg_qpm_law():
+    lam = 1040e-9
+    n1 = ph.gayer_n(lam * 1e6, 20.0)
+    n2 = ph.gayer_n(lam / 2 * 1e6, 20.0)
+    period = lam / (2 * (n2 - n1))                      # first-order QPM period
+    e1, p1 = _shg_energy(0.1e-3, period)
+    e2, p2 = _shg_energy(0.2e-3, period)
+    # E_sh = -i (1 + sinc(w0 dt)) (2/pi) G L A^2 on phase matching: magnitude and L^2 law
+    assert 0.9 < e1 / p1 < 1.05, e1 / p1
+    assert 3.6 < e2 / e1 < 4.2, e2 / e1
+    # far off the QPM period the SH stays orders of magnitude weaker
+    e3, _ = _shg_energy(0.2e-3, 30e-6)
+    assert e3 < 1e-2 * e2
