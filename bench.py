@@ -1,0 +1,149 @@
+"""Benchmark: OPO round trips per second on one MI355X.
+
+Workload (one "step" = one round trip): synchronously pumped ring OPO,
+N = 2^15 points over a 12 ps window at 1040 nm, 5 mm MgO:PPLN with a
+30.5 um grating, 200 fs sech^2 pump of 2 W at 1 GHz, 5 % output coupler,
+0.3 ps cavity detuning, started from seeded one-photon-per-mode noise.
+Adaptive Heun split-step with the reference's default error bounds
+([2.5e-3, 5e-2] %, h = 0.1 um).  Synthetic inputs, resident in HBM before
+the timed region; FP32 complex fields with FP64 phase arguments and norms.
+
+Prints one JSON line.  `roofline` is for the dominant kernel (the persistent
+pass kernel) measured with HIP events on its own stream; `cpu_baseline` is
+the FP64 C restatement of the reference's CPU solver (oracle/, plotting and
+pause removed) timed on a bounded sample on one host core.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "OPO round trips/s; time-to-steady-state warm-start vs plain @1e-4 err"
+
+
+def algorithmic_bytes(n, accepted, rejected, trips):
+    """Per instance: 24 N B per accepted adaptive step (16 B field r/w + 8 B
+    beta/alpha), 16 N B per rejected step, 40 N B per round trip (cavity)."""
+    return 24.0 * n * accepted + 16.0 * n * rejected + 40.0 * n * trips
+
+
+def workload():
+    from optifax_amd import physics as ph
+    cfg = ph.OPOConfig()
+    return cfg, cfg.build()
+
+
+def cpu_baseline(b, attempts_per_trip):
+    """Oracle (FP64, one core) on the first 1 mm of the 5 mm pass; the time per
+    adaptive step is scaled by the steps of one full round trip."""
+    import oracle
+    si = b["solver"]
+    sample_steps = si["nSteps"] // 5
+    sel = float(sample_steps)
+    t0 = time.perf_counter()
+    _, _, _, cnt = oracle.nee_heun(b["ops"]["field0"], si["alpha_w"], si["G33"][:sample_steps],
+                                   si["w0"], si["bdiffw0"], si["h"], sample_steps, si["dt"],
+                                   si["beta_op"], si["maxErr"], si["minErr"], sel, n_cols=0,
+                                   want_stepmods=False)
+    dt = time.perf_counter() - t0
+    per_step = dt / max(int(cnt[0]), 1)
+    trip_s = per_step * attempts_per_trip
+    return {
+        "value": 1.0 / trip_s,
+        "unit": "round trips/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"first {sample_steps} of {si['nSteps']} minimal steps of the pass "
+                   f"({int(cnt[0])} adaptive steps, {dt:.1f} s), scaled by the "
+                   f"{attempts_per_trip} adaptive steps of one round trip"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20, help="timed round trips")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed round trips")
+    ap.add_argument("--precision", default="single", choices=["single", "double"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(instances shard across GPUs by design, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from optifax_amd.solver import Context, OPOBatch
+    cfg, b = workload()
+    si, ops = b["solver"], b["ops"]
+    n = si["beta_op"].size
+    ctx = Context(0)
+    trip_cap = args.warmup + args.steps + 1
+    run = OPOBatch([dict(solver=si, field0=ops["field0"], pump=ops["pump"], cav=ops["cav"],
+                         ocm=ops["ocm"], delay=b["delay"])], precision=args.precision,
+                   trip_cap=trip_cap, ctx=ctx)
+    geo = run.geometry(0)
+    # warm-up trips (also builds the one-trip graph)
+    if args.warmup > 0:
+        run.graph_replay(1, args.warmup)
+        run.sync()
+    t0 = time.perf_counter()
+    run.graph_replay(1, args.steps)
+    run.sync()
+    wall = time.perf_counter() - t0
+    ev_ms = run.last_ms()                      # HIP events around the replays
+    r = run.read(0)
+    log = r["trip_log"][args.warmup:args.warmup + args.steps]
+    att = int(log[:, 2].sum())
+    acc = int(log[:, 3].sum())
+    rej = att - acc
+    bytes_total = algorithmic_bytes(n, acc, rej, args.steps)
+    per_launch_ms = ev_ms / args.steps
+    achieved = bytes_total / args.steps / (per_launch_ms * 1e-3) / 1e9
+    value = args.steps / wall
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "round trips/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision == "single" else "f64",
+        "data": "synthetic (sech^2 pump + seeded one-photon-per-mode noise)",
+        "config": {
+            "workload": "synchronously pumped ring OPO, N=2^15, 12 ps window, 1040 nm; 5 mm "
+                        "MgO:PPLN 30.5 um; 2 W @ 1 GHz 200 fs sech^2; 5% OC; 0.3 ps detuning",
+            "instances": 1,
+            "adaptive_steps_per_trip": att / args.steps,
+            "rejected_per_trip": rej / args.steps,
+            "cluster_workgroups": geo["cluster"],
+            "block": geo["block"],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(b, att / args.steps)
+    run.close()
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -45,27 +45,33 @@ def build_hip(verbose=False) -> str:
     return out
 
 
-def build_mex(verbose=False) -> str:
-    """OPOmexBatch gateway over the in-repo mx-API shim (no MATLAB here)."""
+def build_mex(verbose=False):
+    """MEX gateways (OPOmexBatch, OPOmexCavity) over the in-repo mx-API shim.
+
+    Without MATLAB the gateways link libofx_mx.so (optifax_amd/csrc/mx_shim.cpp),
+    which implements the documented mx*/mex* functions they use; with MATLAB
+    the same sources build with `mex -R2018a`."""
     os.makedirs(LIBDIR, exist_ok=True)
-    out = os.path.join(LIBDIR, "libOPOmexBatch.so")
-    tmp = out + ".tmp"
-    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall",
-           os.path.join(CSRC, "mex_gateway.cpp"), os.path.join(CSRC, "mx_shim.cpp"),
-           "-I", os.path.join(ROOT, "include"), "-L", LIBDIR, "-lofx_hip",
-           "-Wl,-rpath,$ORIGIN", "-o", tmp]
-    log = _run(cmd)
-    if verbose and log.strip():
-        print(log)
-    os.replace(tmp, out)
+    inc = ["-I", os.path.join(ROOT, "include")]
+    flags = ["-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-DOFX_MX_SHIM"]
+    out = []
+    shim = os.path.join(LIBDIR, "libofx_mx.so")
+    _run(["g++", *flags, os.path.join(CSRC, "mx_shim.cpp"), *inc, "-o", shim + ".tmp"])
+    os.replace(shim + ".tmp", shim)
+    out.append(shim)
+    for name, src in (("libOPOmexBatch.so", "mex_gateway.cpp"), ("libOPOmexCavity.so", "mex_cavity.cpp")):
+        dst = os.path.join(LIBDIR, name)
+        log = _run(["g++", *flags, os.path.join(CSRC, src), *inc, "-L", LIBDIR, "-lofx_mx",
+                    "-lofx_hip", "-Wl,-rpath,$ORIGIN", "-o", dst + ".tmp"])
+        if verbose and log.strip():
+            print(log)
+        os.replace(dst + ".tmp", dst)
+        out.append(dst)
     return out
 
 
 def build_all(verbose=False):
-    paths = [build_hip(verbose)]
-    if os.path.exists(os.path.join(CSRC, "mex_gateway.cpp")):
-        paths.append(build_mex(verbose))
-    return paths
+    return [build_hip(verbose), *build_mex(verbose)]
 
 
 if __name__ == "__main__":

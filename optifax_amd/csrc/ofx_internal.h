@@ -68,7 +68,7 @@ struct ClusterDesc {
   int halo;        // halo columns each side
   void* t1;        // N complex transpose buffer (forward), N = largest of the cluster
   void* t2;        // N complex transpose buffer (inverse)
-  double* slots;   // 2 * C * 4 doubles: per-rank partial results, double-buffered
+  double* slots;   // [2][C][8][4] doubles: per-wave partial results, double-buffered
   unsigned* bar;   // barrier counter (zeroed before every launch)
   int* abort_flag; // set when a barrier times out
   int* xcc;        // C ints: hardware XCC id of each member (placement diagnostic)

@@ -212,47 +212,21 @@ OFX_DEV cpx<R>* lds_fft(cpx<R>* in, cpx<R>* out, int lg, int nseq, int ss, const
 }
 
 // ------------------------------------------------------------------------
-// Workgroup reductions (all threads receive the result).
+// Cluster-wide reductions through the slots.  Every wave reduces its lanes
+// and publishes one record of three doubles (no workgroup reduction before
+// the barrier); after the barrier the C * waves records are read in
+// parallel, one per thread, and reduced.  op 0 = max, op 1 = sum.  The
+// reduction tree is the same in every workgroup, so all members obtain
+// bit-identical results (the step-control decisions depend on it).
+// Slot layout: [parity][rank][wave < 8][4] doubles.
 // ------------------------------------------------------------------------
-OFX_DEV double wg_max(double v, double* red) {
-#ifdef OFX_EMU
-  (void)red;
-  return v;
-#else
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-  __syncthreads();
-  if (lane == 0) red[w] = v;
-  __syncthreads();
-  double r = red[0];
-  for (int i = 1; i < nw; i++) r = fmax(r, red[i]);
-  return r;
-#endif
-}
+#define OFX_SLOT_WAVES 8
+OFX_DEV double red_op(double a, double b, int op) { return op ? a + b : (a > b ? a : b); }
 
-// sums three values; fixed order so every workgroup rounds identically
-OFX_DEV void wg_sum3(double* v, double* red) {
-#ifdef OFX_EMU
-  (void)v;
-  (void)red;
-#else
-  for (int i = 0; i < 3; i++)
-    for (int o = 32; o > 0; o >>= 1) v[i] += __shfl_xor(v[i], o);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-  __syncthreads();
-  if (lane == 0) {
-    red[3 * w] = v[0];
-    red[3 * w + 1] = v[1];
-    red[3 * w + 2] = v[2];
-  }
-  __syncthreads();
-  for (int i = 0; i < 3; i++) {
-    double s = red[i];
-    for (int q = 1; q < nw; q++) s += red[3 * q + i];
-    v[i] = s;
-  }
-#endif
-}
+OFX_DEV void publish_slots(const ClusterDesc& cd, int par, int rank, int op, double a, double b,
+                           double c, int coh);
+OFX_DEV void gather_slots(const ClusterDesc& cd, int par, int op, double* out, double* red,
+                          int coh);
 
 // ------------------------------------------------------------------------
 // Cluster exchange and barrier.
@@ -331,6 +305,56 @@ OFX_DEV double xldd(const double* p, int coh) {
 #else
   if (!coh) return *p;
   return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#endif
+}
+
+OFX_DEV void publish_slots(const ClusterDesc& cd, int par, int rank, int op, double a, double b,
+                           double c, int coh) {
+  double v[3] = {a, b, c};
+#ifdef OFX_EMU
+  const int w = 0;
+#else
+  for (int i = 0; i < 3; i++)
+    for (int o = 32; o > 0; o >>= 1) v[i] = red_op(v[i], __shfl_xor(v[i], o), op);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) != 0) return;
+#endif
+  double* sl = cd.slots + ((size_t)(par * cd.C + rank) * OFX_SLOT_WAVES + w) * 4;
+  xstd(sl + 0, v[0], coh);
+  xstd(sl + 1, v[1], coh);
+  xstd(sl + 2, v[2], coh);
+}
+
+OFX_DEV void gather_slots(const ClusterDesc& cd, int par, int op, double* out, double* red,
+                          int coh) {
+#ifdef OFX_EMU
+  (void)red;
+  for (int i = 0; i < 3; i++) out[i] = op ? 0.0 : -1.0;
+  for (int q = 0; q < cd.C; q++) {
+    const double* sl = cd.slots + ((size_t)(par * cd.C + q) * OFX_SLOT_WAVES) * 4;
+    for (int i = 0; i < 3; i++) out[i] = red_op(out[i], xldd(sl + i, coh), op);
+  }
+#else
+  const int nw = (int)(blockDim.x >> 6);
+  const int tot = cd.C * nw;
+  double v[3];
+  for (int i = 0; i < 3; i++) v[i] = op ? 0.0 : -1.0;
+  for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+    const int q = t / nw, w = t - q * nw;
+    const double* sl = cd.slots + ((size_t)(par * cd.C + q) * OFX_SLOT_WAVES + w) * 4;
+    for (int i = 0; i < 3; i++) v[i] = red_op(v[i], xldd(sl + i, coh), op);
+  }
+  for (int i = 0; i < 3; i++)
+    for (int o = 32; o > 0; o >>= 1) v[i] = red_op(v[i], __shfl_xor(v[i], o), op);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0)
+    for (int i = 0; i < 3; i++) red[3 * (threadIdx.x >> 6) + i] = v[i];
+  __syncthreads();
+  for (int i = 0; i < 3; i++) {
+    double r = red[i];
+    for (int w = 1; w < nw; w++) r = red_op(r, red[3 * w + i], op);
+    out[i] = r;
+  }
 #endif
 }
 
@@ -532,6 +556,26 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
     g.B0[lc * g.sT + r] = field[gidx(g, lc, r)];
   }
   OFX_SYNC();
+  // Per-point tables of this workgroup's fixed thread->point mapping, held in
+  // registers for the whole instance: dispersion (beta, alpha of the own
+  // frequency bins) and the absorber factor of the own time points.  Points
+  // beyond MP per thread are read from global memory.
+  constexpr int MP = sizeof(R) == 4 ? 8 : 4;
+  double breg[MP];
+  R areg[MP], abreg[MP];
+#pragma unroll
+  for (int q = 0; q < MP; q++) {
+    const int idx = OFX_TID + q * OFX_NT;
+    breg[q] = 0.0;
+    areg[q] = R(0);
+    abreg[q] = R(1);
+    if (idx < g.Rr * n1) {
+      const int kk = (g.k20 + (idx >> g.lg1)) + n2 * (idx & (n1 - 1));
+      breg[q] = d.beta[kk];
+      areg[q] = (R)d.alpha[kk];
+    }
+    if (idx < g.W * n2) abreg[q] = ab[gidx(g, g.HW + (idx >> g.lg2), idx & (n2 - 1))];
+  }
 
   int trip = 0;
   for (; trip < trips && status == ST_RUNNING; trip++) {
@@ -577,7 +621,7 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
       OFX_SYNC();
       // K2, corrector, error, absorber on own columns; E overwrites K
       double lmax = -1.0;
-      for (int idx = OFX_TID; idx < g.W * n2; idx += OFX_NT) {
+      auto corrector = [&](int idx, R f) {
         const int lc = g.HW + (idx >> g.lg2), r = idx & (n2 - 1);
         const int p = lc * g.sT + r;
         C k2 = k_term<R>(X[p], X[p + g.sT], X[p - g.sT], gk, cc);
@@ -589,32 +633,29 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
         const double err = sqrt(ex * ex + ey * ey) /
                            (sqrt((double)e.x * (double)e.x + (double)e.y * (double)e.y) + 1.0);
         if (err > lmax) lmax = err;
-        const R f = ab[gidx(g, lc, r)];
         if (f != R(1)) e = cscale(e, f);
         K[p] = e;
+      };
+#pragma unroll
+      for (int q = 0; q < MP; q++) {
+        const int idx = OFX_TID + q * OFX_NT;
+        if (idx < g.W * n2) corrector(idx, abreg[q]);
       }
+      for (int idx = OFX_TID + MP * OFX_NT; idx < g.W * n2; idx += OFX_NT)
+        corrector(idx, ab[gidx(g, g.HW + (idx >> g.lg2), idx & (n2 - 1))]);
       OFX_SYNC();
-      lmax = wg_max(lmax, red);
-      double badv = wg_max((double)bad, red);
+      const int par = (int)(nbar & 1u);
+      publish_slots(cd, par, rank, 0, lmax, (double)bad, 0.0, g.coh);
       // speculative forward column transforms of E
       fwd_cols_to_t1<R>(g, K, X, t1, tw, nmax);
-      const int par = (int)(nbar & 1u);
-      if (OFX_TID == 0) {
-        xstd(&cd.slots[(par * g.Cn + rank) * 4 + 0], lmax, g.coh);
-        xstd(&cd.slots[(par * g.Cn + rank) * 4 + 1], badv, g.coh);
-      }
       nbar++;
       if (!cluster_barrier(cd, nbar * (unsigned)g.Cn, a.timeout_ticks, sflag, g.coh)) {
         status = ST_BARRIER_TIMEOUT;
         break;
       }
-      double gmax = -1.0, gbad = 0.0;
-      for (int q = 0; q < g.Cn; q++) {
-        const double v = xldd(&cd.slots[(par * g.Cn + q) * 4 + 0], g.coh);
-        const double b = xldd(&cd.slots[(par * g.Cn + q) * 4 + 1], g.coh);
-        gmax = v > gmax ? v : gmax;
-        gbad = b > gbad ? b : gbad;
-      }
+      double gr[3];
+      gather_slots(cd, par, 0, gr, red, g.coh);
+      const double gmax = gr[0], gbad = gr[1];
       attempts++;
       trip_att++;
       if (gbad > 0.0) {
@@ -640,12 +681,11 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
       // frequency phase: rows, dispersion, snapshots, inverse rows
       C* F = t1_to_rows_fwd<R>(g, t1, g.B0, g.B1, tw, nmax);
       C* Fo = (F == g.B0) ? g.B1 : g.B0;
-      for (int idx = OFX_TID; idx < g.Rr * n1; idx += OFX_NT) {
+      auto disperse = [&](int idx, double beta, R al) {
         const int k2l = idx >> g.lg1, k1 = idx & (n1 - 1);
         const int kk = (g.k20 + k2l) + n2 * k1;
         C v = F[k2l * g.sF + k1];
-        const C ph = expi(-d.beta[kk] * s_used, R(0));
-        const R al = (R)d.alpha[kk];
+        const C ph = expi(-beta * s_used, R(0));
         const R am = al == R(1) ? R(1) : (al == R(0) ? R(0) : (R)pow(al, (R)s_used));
         v = cscale(cmul(v, ph), am);
         F[k2l * g.sF + k1] = v;
@@ -654,6 +694,15 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
           for (int cc2 = chunk; cc2 < cend; cc2++)
             if (cc2 - 1 < d.n_cols) ap[(size_t)(cc2 - 1) * n + kk] = v;
         }
+      };
+#pragma unroll
+      for (int q = 0; q < MP; q++) {
+        const int idx = OFX_TID + q * OFX_NT;
+        if (idx < g.Rr * n1) disperse(idx, breg[q], areg[q]);
+      }
+      for (int idx = OFX_TID + MP * OFX_NT; idx < g.Rr * n1; idx += OFX_NT) {
+        const int kk = (g.k20 + (idx >> g.lg1)) + n2 * (idx & (n1 - 1));
+        disperse(idx, d.beta[kk], (R)d.alpha[kk]);
       }
       OFX_SYNC();
       rows_inv_to_t2<R>(g, F, Fo, t2, tw, nmax);
@@ -708,22 +757,16 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
         F[k2l * g.sF + k1] = cadd(cmul(v, m), pump ? pump[kk] : mkc<R>(R(0), R(0)));
       }
       OFX_SYNC();
-      wg_sum3(acc, red);
-      rows_inv_to_t2<R>(g, F, Fo, t2, tw, nmax);
       const int par = (int)(nbar & 1u);
-      if (OFX_TID == 0) {
-        xstd(&cd.slots[(par * g.Cn + rank) * 4 + 0], acc[0], g.coh);
-        xstd(&cd.slots[(par * g.Cn + rank) * 4 + 1], acc[1], g.coh);
-        xstd(&cd.slots[(par * g.Cn + rank) * 4 + 2], acc[2], g.coh);
-      }
+      publish_slots(cd, par, rank, 1, acc[0], acc[1], acc[2], g.coh);
+      rows_inv_to_t2<R>(g, F, Fo, t2, tw, nmax);
       nbar++;
       if (!cluster_barrier(cd, nbar * (unsigned)g.Cn, a.timeout_ticks, sflag, g.coh)) {
         status = ST_BARRIER_TIMEOUT;
         break;
       }
-      double tot[3] = {0.0, 0.0, 0.0};
-      for (int q = 0; q < g.Cn; q++)
-        for (int i = 0; i < 3; i++) tot[i] += xldd(&cd.slots[(par * g.Cn + q) * 4 + i], g.coh);
+      double tot[3];
+      gather_slots(cd, par, 1, tot, red, g.coh);
       t2_to_cols_inv<R>(g, t2, tw, nmax);
       const double conv = have_prev ? (tot[1] > 0.0 ? sqrt(tot[0] / tot[1]) : 0.0) : INFINITY;
       if (rank == 0 && d.trip_log && trip_base + trip < d.trip_cap) {

@@ -1,0 +1,130 @@
+"""Call the MEX gateways (OPOmexBatch, OPOmexCavity) without MATLAB.
+
+The gateways are built against optifax_amd/csrc/mx_shim.cpp, which implements
+the documented MATLAB C Matrix API subset they use.  This module converts
+numpy arrays to mxArrays (column-major, interleaved complex), calls
+mexFunction through ofx_mex_call, and converts the outputs back, so the exact
+compiled gateway MATLAB would load is what runs.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(_HERE, "lib")
+
+mxDOUBLE, mxSINGLE, mxUINT32 = 6, 7, 13
+_shim = None
+_gateways = {}
+
+
+class MexError(RuntimeError):
+    def __init__(self, ident, msg):
+        super().__init__(f"{ident}: {msg}")
+        self.identifier = ident
+
+
+def _load_shim():
+    global _shim
+    if _shim is None:
+        path = os.path.join(LIBDIR, "libofx_mx.so")
+        if not os.path.exists(path):
+            raise OSError(f"{path} is not built; run __graft_entry__.build() first")
+        L = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+        vp, sz = ctypes.c_void_p, ctypes.c_size_t
+        L.mxCreateNumericArray.restype = vp
+        L.mxCreateNumericArray.argtypes = [sz, ctypes.POINTER(sz), ctypes.c_int, ctypes.c_int]
+        L.mxDestroyArray.argtypes = [vp]
+        L.mxDestroyArray.restype = None
+        L.mxGetData.restype = vp
+        L.mxGetData.argtypes = [vp]
+        L.mxGetNumberOfDimensions.restype = sz
+        L.mxGetNumberOfDimensions.argtypes = [vp]
+        L.mxGetDimensions.restype = ctypes.POINTER(sz)
+        L.mxGetDimensions.argtypes = [vp]
+        L.mxGetClassID.restype = ctypes.c_int
+        L.mxGetClassID.argtypes = [vp]
+        L.mxIsComplex.restype = ctypes.c_int
+        L.mxIsComplex.argtypes = [vp]
+        L.ofx_mex_call.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp), ctypes.c_int,
+                                   ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_int,
+                                   ctypes.c_char_p, ctypes.c_int]
+        L.ofx_mex_clear.restype = None
+        _shim = L
+    return _shim
+
+
+def _gateway(name):
+    if name not in _gateways:
+        _load_shim()
+        L = ctypes.CDLL(os.path.join(LIBDIR, f"lib{name}.so"))
+        _gateways[name] = L
+    return _gateways[name]
+
+
+def to_mx(x):
+    L = _load_shim()
+    a = np.asarray(x)
+    if a.dtype == np.uint32:
+        cls, cpx, base = mxUINT32, 0, np.uint32
+    elif a.dtype in (np.float32, np.complex64):
+        cls, cpx, base = mxSINGLE, int(np.iscomplexobj(a)), np.float32
+    else:
+        cls, cpx, base = mxDOUBLE, int(np.iscomplexobj(a)), np.float64
+    shape = a.shape if a.ndim >= 2 else ((1, 1) if a.ndim == 0 else (a.shape[0], 1))
+    dims = (ctypes.c_size_t * len(shape))(*shape)
+    h = L.mxCreateNumericArray(len(shape), dims, cls, cpx)
+    flat = np.asarray(a.reshape(shape), order="F").ravel(order="F")
+    if cpx:
+        buf = np.empty(2 * flat.size, base)
+        buf[0::2], buf[1::2] = flat.real, flat.imag
+    else:
+        buf = flat.astype(base)
+    ctypes.memmove(L.mxGetData(h), buf.ctypes.data, buf.nbytes)
+    return h
+
+
+def from_mx(h):
+    L = _load_shim()
+    nd = L.mxGetNumberOfDimensions(h)
+    dims = L.mxGetDimensions(h)
+    shape = tuple(dims[i] for i in range(nd))
+    cls = L.mxGetClassID(h)
+    cpx = L.mxIsComplex(h)
+    base = {mxSINGLE: np.float32, mxDOUBLE: np.float64, mxUINT32: np.uint32}[cls]
+    n = int(np.prod(shape))
+    raw = np.ctypeslib.as_array((ctypes.c_char * (n * np.dtype(base).itemsize * (2 if cpx else 1)))
+                                .from_address(L.mxGetData(h))).view(base).copy()
+    arr = raw[0::2] + 1j * raw[1::2] if cpx else raw
+    return arr.reshape(shape, order="F")
+
+
+def call(name, *inputs, nargout=3):
+    """[out1, ..., outN] = name(inputs...) through the compiled gateway."""
+    L = _load_shim()
+    G = _gateway(name)
+    prhs = (ctypes.c_void_p * max(len(inputs), 1))(*[to_mx(x) for x in inputs])
+    plhs = (ctypes.c_void_p * max(nargout, 1))()
+    ident = ctypes.create_string_buffer(256)
+    msg = ctypes.create_string_buffer(1024)
+    fn = ctypes.cast(G.mexFunction, ctypes.c_void_p)
+    try:
+        st = L.ofx_mex_call(fn, nargout, plhs, len(inputs), prhs, ident, 256, msg, 1024)
+        if st:
+            raise MexError(ident.value.decode(), msg.value.decode())
+        return [from_mx(plhs[i]) for i in range(nargout)]
+    finally:
+        for i in range(len(inputs)):
+            L.mxDestroyArray(prhs[i])
+        for i in range(nargout):
+            if plhs[i]:
+                L.mxDestroyArray(plhs[i])
+
+
+def clear():
+    """MATLAB `clear mex`: run the gateways' exit hooks (releases the device)."""
+    if _shim is not None:
+        _shim.ofx_mex_clear()
