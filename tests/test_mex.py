@@ -52,10 +52,12 @@ def test_opomexbatch_matches_c_api_and_oracle(gpu_ctx):
     a = _mex_args(E, args)
     Et, Ap, sm = mexshim.call("OPOmexBatch", *a)
     assert Et.shape == (1024, 1) and Ap.shape == (1024, 4) and sm.shape == (1, args[5])
-    E2, Ap2, sm2 = OPOmexBatch(a[0][:, 0], *args, a[12], a[13], ctx=gpu_ctx)
+    # the gateway reads w0, h, maxErr and minErr as single (OPO_mex_BATCH.cpp:45-51)
+    same = [float(np.float32(x)) if i in (2, 4, 8, 9) else x for i, x in enumerate(args)]
+    E2, Ap2, sm2 = OPOmexBatch(a[0][:, 0], *same, a[12], a[13], ctx=gpu_ctx)
     assert np.array_equal(Et[:, 0], E2.astype(np.complex64))
     assert np.array_equal(sm[0], sm2.reshape(-1))
-    Eo, apo, smo, _ = oracle.nee_heun(E, *args, n_cols=4)
+    Eo, apo, smo, _ = oracle.nee_heun(E, *same, n_cols=4)
     assert np.array_equal(sm[0], smo)
     assert rel_l2(Et[:, 0], Eo) < 2e-4
     mexshim.clear()
@@ -73,9 +75,10 @@ def test_opomexcavity_matches_device_batch(gpu_ctx):
          np.float64(si["sel"]), ops["pump"], ops["cav"], ops["ocm"], np.float64(cfg.delay),
          np.float64(trips), np.float64(0.0)]
     Et, oc, log, status = mexshim.call("OPOmexCavity", *a, nargout=4)
-    s32 = dict(si, beta_op=si["beta_op"].astype(np.float32).astype(np.float64),
-               alpha_w=si["alpha_w"].astype(np.float32).astype(np.float64),
-               G33=si["G33"].astype(np.float32).astype(np.float64))
+    f32 = lambda x: np.asarray(x, np.float32).astype(np.float64)
+    s32 = dict(si, beta_op=f32(si["beta_op"]), alpha_w=f32(si["alpha_w"]), G33=f32(si["G33"]),
+               w0=float(f32(si["w0"])), h=float(f32(si["h"])), maxErr=float(f32(si["maxErr"])),
+               minErr=float(f32(si["minErr"])))
     run = OPOBatch([dict(solver=s32, field0=a[0][:, 0], pump=ops["pump"], cav=ops["cav"],
                          ocm=ops["ocm"], delay=cfg.delay)], trip_cap=trips, ctx=gpu_ctx)
     run.launch(trips)
