@@ -451,6 +451,47 @@ OFX_DEV cpx<R>* t1_to_rows_fwd(const Geo<R>& g, const cpx<R>* t1, cpx<R>* dst, c
   return lds_fft<R, -1>(dst, other, g.lg1, g.Rr, g.sF, tw, nmax >> g.lg1);
 }
 
+// After the first barrier of an attempt: read the cluster's error records
+// and this workgroup's t1 rows (into B1; A0 in B0 stays intact for a
+// rejection) with the loads of both in flight together, then reduce the
+// records (max).  out = {max error quotient, non-finite flag, -}.
+template <typename R>
+OFX_DEV void gather_err_and_rows(const ClusterDesc& cd, int par, const Geo<R>& g,
+                                 const cpx<R>* t1, double* out, double* red) {
+  const int totR = g.Rr * g.n1;
+#ifdef OFX_EMU
+  for (int idx = 0; idx < totR; idx++) {
+    const int k2l = idx >> g.lg1, c = idx & (g.n1 - 1);
+    g.B1[k2l * g.sF + c] = xld(&t1[(g.k20 + k2l) * g.n1 + c], g.coh);
+  }
+  gather_slots(cd, par, 0, out, red, g.coh);
+#else
+  const int nw = (int)(blockDim.x >> 6);
+  const int tot = cd.C * nw;
+  double v[3] = {-1.0, -1.0, -1.0};
+  for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+    const int q = t / nw, w = t - q * nw;
+    const double* sl = cd.slots + ((size_t)(par * cd.C + q) * OFX_SLOT_WAVES + w) * 4;
+    for (int i = 0; i < 3; i++) v[i] = red_op(v[i], xldd(sl + i, g.coh), 0);
+  }
+  for (int idx = threadIdx.x; idx < totR; idx += blockDim.x) {
+    const int k2l = idx >> g.lg1, c = idx & (g.n1 - 1);
+    g.B1[k2l * g.sF + c] = xld(&t1[(g.k20 + k2l) * g.n1 + c], g.coh);
+  }
+  for (int i = 0; i < 3; i++)
+    for (int o = 32; o > 0; o >>= 1) v[i] = red_op(v[i], __shfl_xor(v[i], o), 0);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0)
+    for (int i = 0; i < 3; i++) red[3 * (threadIdx.x >> 6) + i] = v[i];
+  __syncthreads();
+  for (int i = 0; i < 3; i++) {
+    double r = red[i];
+    for (int w = 1; w < nw; w++) r = red_op(r, red[3 * w + i], 0);
+    out[i] = r;
+  }
+#endif
+}
+
 // Inverse length-n1 transforms of the own rows, conjugate twiddle, write t2
 // (column-major c*n2 + k2).
 template <typename R>
@@ -556,25 +597,22 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
     g.B0[lc * g.sT + r] = field[gidx(g, lc, r)];
   }
   OFX_SYNC();
-  // Per-point tables of this workgroup's fixed thread->point mapping, held in
-  // registers for the whole instance: dispersion (beta, alpha of the own
-  // frequency bins) and the absorber factor of the own time points.  Points
-  // beyond MP per thread are read from global memory.
+  // Dispersion tables (beta, alpha) of this workgroup's own frequency bins,
+  // held in registers for the whole instance (the thread->bin mapping of the
+  // frequency phase is fixed); bins beyond MP per thread are read from memory.
   constexpr int MP = sizeof(R) == 4 ? 8 : 4;
   double breg[MP];
-  R areg[MP], abreg[MP];
+  R areg[MP];
 #pragma unroll
   for (int q = 0; q < MP; q++) {
     const int idx = OFX_TID + q * OFX_NT;
     breg[q] = 0.0;
     areg[q] = R(0);
-    abreg[q] = R(1);
     if (idx < g.Rr * n1) {
       const int kk = (g.k20 + (idx >> g.lg1)) + n2 * (idx & (n1 - 1));
       breg[q] = d.beta[kk];
       areg[q] = (R)d.alpha[kk];
     }
-    if (idx < g.W * n2) abreg[q] = ab[gidx(g, g.HW + (idx >> g.lg2), idx & (n2 - 1))];
   }
 
   int trip = 0;
@@ -595,66 +633,58 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
       // exp(i*(w0 t_j - bdwz)) = wt[j] * exp(-i*bdwz); bdwz reduced in FP64
       const C eb0 = expi(-bdwz0, R(0));
       const C eb1 = expi(-bdwz1, R(0));
-      // stage 1: NL(A0) on all local columns
-      for (int idx = OFX_TID; idx < g.LC * n2; idx += OFX_NT) {
-        const int lc = idx >> g.lg2, r = idx & (n2 - 1);
-        const int j = gidx(g, lc, r);
-        C v = A0[lc * g.sT + r];
-        if (!(isfinite((double)v.x) && isfinite((double)v.y))) bad = 1;
-        X[lc * g.sT + r] = nl_term_e<R>(v, cmul(wt[j], eb0));
-      }
-      OFX_SYNC();
-      // K1 on columns 1..LC-2
+      // Stage 1: K1 on columns 1..LC-2.  Each point evaluates the nonlinear
+      // term at itself and its two neighbours j+-1 (the central difference of
+      // NEE_CPU.m:104) instead of staging the term through LDS: three cheap
+      // evaluations instead of a workgroup barrier and an LDS round trip.
       for (int idx = OFX_TID; idx < (g.LC - 2) * n2; idx += OFX_NT) {
         const int lc = 1 + (idx >> g.lg2), r = idx & (n2 - 1);
         const int p = lc * g.sT + r;
-        K[p] = k_term<R>(X[p], X[p + g.sT], X[p - g.sT], gk, cc);
+        const C v = A0[p];
+        if (lc >= g.HW && lc < g.HW + g.W && !(isfinite((double)v.x) && isfinite((double)v.y)))
+          bad = 1;
+        const C nc = nl_term_e<R>(v, cmul(wt[gidx(g, lc, r)], eb0));
+        const C nu = nl_term_e<R>(A0[p + g.sT], cmul(wt[gidx(g, lc + 1, r)], eb0));
+        const C nd = nl_term_e<R>(A0[p - g.sT], cmul(wt[gidx(g, lc - 1, r)], eb0));
+        K[p] = k_term<R>(nc, nu, nd, gk, cc);
       }
       OFX_SYNC();
-      // stage 2: NL(A0 + K1) on columns 1..LC-2
-      for (int idx = OFX_TID; idx < (g.LC - 2) * n2; idx += OFX_NT) {
-        const int lc = 1 + (idx >> g.lg2), r = idx & (n2 - 1);
-        const int p = lc * g.sT + r;
-        const int j = gidx(g, lc, r);
-        X[p] = nl_term_e<R>(cadd(A0[p], K[p]), cmul(wt[j], eb1));
-      }
-      OFX_SYNC();
-      // K2, corrector, error, absorber on own columns; E overwrites K
+      // Stage 2 on own columns: K2 from NL(A1 = A0 + K1) at j and j+-1, the
+      // corrector E = A0 + K1/2 + K2/2, the error quotient and the absorber.
+      // E goes to X (neighbours still read K1).
       double lmax = -1.0;
       auto corrector = [&](int idx, R f) {
         const int lc = g.HW + (idx >> g.lg2), r = idx & (n2 - 1);
         const int p = lc * g.sT + r;
-        C k2 = k_term<R>(X[p], X[p + g.sT], X[p - g.sT], gk, cc);
-        C k1 = K[p];
-        C a0 = A0[p];
-        C a1 = cadd(a0, k1);
+        const C k1 = K[p];
+        const C a0 = A0[p];
+        const C a1 = cadd(a0, k1);
+        const C nc = nl_term_e<R>(a1, cmul(wt[gidx(g, lc, r)], eb1));
+        const C nu = nl_term_e<R>(cadd(A0[p + g.sT], K[p + g.sT]), cmul(wt[gidx(g, lc + 1, r)], eb1));
+        const C nd = nl_term_e<R>(cadd(A0[p - g.sT], K[p - g.sT]), cmul(wt[gidx(g, lc - 1, r)], eb1));
+        const C k2 = k_term<R>(nc, nu, nd, gk, cc);
         C e = cadd(cadd(a0, cscale(k1, R(0.5))), cscale(k2, R(0.5)));
         const double ex = (double)e.x - (double)a1.x, ey = (double)e.y - (double)a1.y;
         const double err = sqrt(ex * ex + ey * ey) /
                            (sqrt((double)e.x * (double)e.x + (double)e.y * (double)e.y) + 1.0);
         if (err > lmax) lmax = err;
         if (f != R(1)) e = cscale(e, f);
-        K[p] = e;
+        X[p] = e;
       };
-#pragma unroll
-      for (int q = 0; q < MP; q++) {
-        const int idx = OFX_TID + q * OFX_NT;
-        if (idx < g.W * n2) corrector(idx, abreg[q]);
-      }
-      for (int idx = OFX_TID + MP * OFX_NT; idx < g.W * n2; idx += OFX_NT)
+      for (int idx = OFX_TID; idx < g.W * n2; idx += OFX_NT)
         corrector(idx, ab[gidx(g, g.HW + (idx >> g.lg2), idx & (n2 - 1))]);
       OFX_SYNC();
       const int par = (int)(nbar & 1u);
       publish_slots(cd, par, rank, 0, lmax, (double)bad, 0.0, g.coh);
       // speculative forward column transforms of E
-      fwd_cols_to_t1<R>(g, K, X, t1, tw, nmax);
+      fwd_cols_to_t1<R>(g, X, K, t1, tw, nmax);
       nbar++;
       if (!cluster_barrier(cd, nbar * (unsigned)g.Cn, a.timeout_ticks, sflag, g.coh)) {
         status = ST_BARRIER_TIMEOUT;
         break;
       }
       double gr[3];
-      gather_slots(cd, par, 0, gr, red, g.coh);
+      gather_err_and_rows<R>(cd, par, g, t1, gr, red);
       const double gmax = gr[0], gbad = gr[1];
       attempts++;
       trip_att++;
@@ -679,8 +709,8 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
       int cend = chunk;  // chunks [chunk, cend) end with this step
       while ((double)cend <= nChunks && !(knew < cend * sel && knew <= nsteps)) cend++;
       // frequency phase: rows, dispersion, snapshots, inverse rows
-      C* F = t1_to_rows_fwd<R>(g, t1, g.B0, g.B1, tw, nmax);
-      C* Fo = (F == g.B0) ? g.B1 : g.B0;
+      C* F = lds_fft<R, -1>(g.B1, g.B2, g.lg1, g.Rr, g.sF, tw, nmax >> g.lg1);
+      C* Fo = (F == g.B1) ? g.B2 : g.B1;
       auto disperse = [&](int idx, double beta, R al) {
         const int k2l = idx >> g.lg1, k1 = idx & (n1 - 1);
         const int kk = (g.k20 + k2l) + n2 * k1;
