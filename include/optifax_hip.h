@@ -154,6 +154,39 @@ int ofx_opo_geometry(const ofx_opo* run, int inst, int* cluster, int* n1, int* b
 /* hardware XCC id of every launched workgroup of the last launch (diagnostic) */
 int ofx_opo_xcc_ids(ofx_opo* run, int* ids, int cap, int* count);
 
+/* ---- spectral surrogate of one crystal pass (MLP, BF16 MFMA / FP32) ------
+ * No reference counterpart: the reference has no learned component.  Network
+ * n_in -> width -> width -> width -> n_out, GELU after the first three layers,
+ * weights [out][in] (PyTorch Linear order).  Public parameter vector, FP32:
+ *   W0 [width][n_in], b0 [width], W1 [width][width], b1, W2, b2,
+ *   W3 [n_out][width], b3 [n_out].
+ * Loss: mean over the batch of ||y - t||_2 / ||t||_2.  Optimiser: Adam
+ * (beta1 0.9, beta2 0.999, eps 1e-8).  width and n_out: multiples of 64;
+ * batch: a multiple of 64 up to max_batch.  Calls on one object are
+ * synchronous and must not overlap. */
+typedef struct ofx_mlp ofx_mlp;
+int ofx_mlp_create(int device, int n_in, int width, int n_out, int max_batch, ofx_mlp** out);
+void ofx_mlp_destroy(ofx_mlp* m);
+const char* ofx_mlp_last_error(const ofx_mlp* m);
+long long ofx_mlp_num_params(const ofx_mlp* m);
+/* load parameters (public layout) and reset the Adam state */
+int ofx_mlp_set_params(ofx_mlp* m, const float* params);
+int ofx_mlp_get_params(ofx_mlp* m, float* params);
+/* inference: x [batch][n_in] -> y [batch][n_out] (host arrays) */
+int ofx_mlp_forward(ofx_mlp* m, int batch, const float* x, float* y);
+/* loss and its gradient w.r.t. the parameters (public layout); no update */
+int ofx_mlp_gradients(ofx_mlp* m, int batch, const float* x, const float* t, float* loss,
+                      float* grads);
+/* one Adam step on the given batch; *loss = batch loss before the update */
+int ofx_mlp_step(ofx_mlp* m, int batch, const float* x, const float* t, float lr, float* loss);
+/* keep a training set on the device: X [n][n_in], T [n][n_out] */
+int ofx_mlp_upload_dataset(ofx_mlp* m, int n, const float* X, const float* T);
+/* `steps` Adam steps; step s uses dataset rows idx[s*batch .. s*batch+batch-1];
+ * losses [steps] (may be NULL) */
+int ofx_mlp_train(ofx_mlp* m, int batch, int steps, const int* idx, float lr, float* losses);
+/* device time of the last ofx_mlp_forward / ofx_mlp_train, milliseconds */
+double ofx_mlp_last_ms(const ofx_mlp* m);
+
 #ifdef __cplusplus
 }
 #endif

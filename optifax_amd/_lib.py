@@ -48,6 +48,9 @@ EXPORTED = [
     "ofx_opo_create", "ofx_opo_destroy", "ofx_opo_launch", "ofx_opo_sync",
     "ofx_opo_graph_replay", "ofx_opo_last_ms", "ofx_opo_read", "ofx_opo_write_field",
     "ofx_opo_geometry", "ofx_opo_xcc_ids",
+    "ofx_mlp_create", "ofx_mlp_destroy", "ofx_mlp_last_error", "ofx_mlp_num_params",
+    "ofx_mlp_set_params", "ofx_mlp_get_params", "ofx_mlp_forward", "ofx_mlp_gradients",
+    "ofx_mlp_step", "ofx_mlp_upload_dataset", "ofx_mlp_train", "ofx_mlp_last_ms",
 ]
 
 
@@ -116,6 +119,22 @@ def load(path: str | None = None):
     L.ofx_opo_write_field.argtypes = [vp, ctypes.c_int, _DP]
     L.ofx_opo_geometry.argtypes = [vp, ctypes.c_int, _IP, _IP, _IP, _IP, _IP]
     L.ofx_opo_xcc_ids.argtypes = [vp, _IP, ctypes.c_int, _IP]
+    L.ofx_mlp_create.argtypes = [ctypes.c_int] * 5 + [ctypes.POINTER(vp)]
+    L.ofx_mlp_destroy.argtypes = [vp]
+    L.ofx_mlp_destroy.restype = None
+    L.ofx_mlp_last_error.argtypes = [vp]
+    L.ofx_mlp_last_error.restype = ctypes.c_char_p
+    L.ofx_mlp_num_params.argtypes = [vp]
+    L.ofx_mlp_num_params.restype = ctypes.c_longlong
+    L.ofx_mlp_set_params.argtypes = [vp, _FP]
+    L.ofx_mlp_get_params.argtypes = [vp, _FP]
+    L.ofx_mlp_forward.argtypes = [vp, ctypes.c_int, _FP, _FP]
+    L.ofx_mlp_gradients.argtypes = [vp, ctypes.c_int, _FP, _FP, _FP, _FP]
+    L.ofx_mlp_step.argtypes = [vp, ctypes.c_int, _FP, _FP, ctypes.c_float, _FP]
+    L.ofx_mlp_upload_dataset.argtypes = [vp, ctypes.c_int, _FP, _FP]
+    L.ofx_mlp_train.argtypes = [vp, ctypes.c_int, ctypes.c_int, _IP, ctypes.c_float, _FP]
+    L.ofx_mlp_last_ms.argtypes = [vp]
+    L.ofx_mlp_last_ms.restype = ctypes.c_double
     if path is None:
         _lib = L
     return L

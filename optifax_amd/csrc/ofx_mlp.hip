@@ -1,0 +1,716 @@
+// Spectral surrogate of one crystal pass: MLP training and inference on gfx950.
+//
+// Network (PyTorch Linear convention, weight [out][in]):
+//   z1 = x W0^T + b0, h1 = gelu(z1); z2 = h1 W1^T + b1, h2 = gelu(z2);
+//   z3 = h2 W2^T + b2, h3 = gelu(z3); y = h3 W3^T + b3
+// Loss: mean over the batch of the relative L2 error ||y - t|| / ||t||.
+//
+// Precision: GEMM operands BF16 (weights, activations, output gradients),
+// accumulation FP32 on the matrix cores (v_mfma_f32_32x32x16_bf16); master
+// weights, pre-activations, weight gradients and Adam moments FP32.
+//
+// Every product is one "NT" GEMM, C[m][n] = sum_k A[m][k] B[n][k], with both
+// operands contiguous in k, so tiles stage into LDS with 16-byte loads and
+// stores and feed the MFMA without transposition:
+//   forward        Z   = H  . Wt^T      A = H   [batch][in],   B = Wt  [out][in]
+//   data grad      dH  = dZ . Wn^T      A = dZ  [batch][out],  B = Wn  [in][out]
+//   weight grad    dWt = dZ^T . H       A = dZT [out][batch],  B = HT  [in][batch]
+// which needs a BF16 copy of each weight in both orientations (Wt, Wn; built
+// after each optimiser step) and transposed copies of H and dZ (written by
+// the producing GEMM's epilogue, 8-byte stores of 4 consecutive rows).
+//
+// Tiles: 64 x 64 outputs per workgroup of 4 waves (each wave one 32 x 32 MFMA
+// block), K in slices of 64 double-buffered in LDS with the next slice's
+// global loads in flight during the current slice's MFMAs.  For the 512-row
+// batch and width 2048, a layer is 8 x 32 = 256 workgroups: one per CU.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/optifax_hip.h"
+
+namespace ofxm {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+enum { EPI_GELU = 0, EPI_BIAS = 1, EPI_DGELU = 2, EPI_F32 = 3 };
+
+constexpr int TM = 64, TN = 64, TK = 64;
+constexpr int LDK = TK + 8;  // LDS row stride (bf16): 144 B, staggers the 16-B reads over banks
+
+__device__ __forceinline__ float gelu(float z) {
+  return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f));
+}
+__device__ __forceinline__ float gelu_d(float z) {
+  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
+         z * 0.39894228040143268f * __expf(-0.5f * z * z);
+}
+
+struct GemmArgs {
+  const bf16* A;  // [M][lda], k-contiguous
+  const bf16* B;  // [N][ldb], k-contiguous
+  int M, N, K, lda, ldb;
+  const float* bias;  // [N] (GELU, BIAS)
+  float* Cf;          // FP32 out [M][ldc]: Z (GELU), y (BIAS), dW (F32)
+  int ldc;
+  bf16* Cb;           // BF16 out [M][ldcb]: H (GELU), dZ (DGELU)
+  int ldcb;
+  bf16* Ct;           // BF16 transposed out [N][ldct]: H^T, dZ^T (optional)
+  int ldct;
+  const float* Zin;   // pre-activation [M][ldz] (DGELU)
+  int ldz;
+};
+
+template <int EPI>
+__global__ void __launch_bounds__(256) gemm_nt(GemmArgs g) {
+  __shared__ __align__(16) bf16 As[2][TM * LDK];
+  __shared__ __align__(16) bf16 Bs[2][TN * LDK];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wr = w >> 1, wc = w & 1;
+  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+  // staging: a 64 x 64 slice is 512 chunks of 8 bf16; thread takes rows r and r + 32
+  const int r = tid >> 3, c = (tid & 7) * 8;
+  const bf16* Ap = g.A + (size_t)(m0 + r) * g.lda + c;
+  const bf16* Bp = g.B + (size_t)(n0 + r) * g.ldb + c;
+  const size_t a32 = (size_t)32 * g.lda, b32 = (size_t)32 * g.ldb;
+  const int so0 = r * LDK + c, so1 = (r + 32) * LDK + c;
+  bf16x8 ra0 = *reinterpret_cast<const bf16x8*>(Ap);
+  bf16x8 ra1 = *reinterpret_cast<const bf16x8*>(Ap + a32);
+  bf16x8 rb0 = *reinterpret_cast<const bf16x8*>(Bp);
+  bf16x8 rb1 = *reinterpret_cast<const bf16x8*>(Bp + b32);
+  *reinterpret_cast<bf16x8*>(&As[0][so0]) = ra0;
+  *reinterpret_cast<bf16x8*>(&As[0][so1]) = ra1;
+  *reinterpret_cast<bf16x8*>(&Bs[0][so0]) = rb0;
+  *reinterpret_cast<bf16x8*>(&Bs[0][so1]) = rb1;
+  __syncthreads();
+  f32x16 acc;
+#pragma unroll
+  for (int i = 0; i < 16; i++) acc[i] = 0.0f;
+  // MFMA operands: lane l supplies row (A) / column (B) l % 32, k = 8 (l / 32) .. + 7
+  const int ao = (32 * wr + (lane & 31)) * LDK + 8 * (lane >> 5);
+  const int bo = (32 * wc + (lane & 31)) * LDK + 8 * (lane >> 5);
+  const int nk = g.K / TK;
+  for (int kt = 0; kt < nk; kt++) {
+    const int cur = kt & 1;
+    const bool more = kt + 1 < nk;
+    if (more) {
+      const size_t ko = (size_t)(kt + 1) * TK;
+      ra0 = *reinterpret_cast<const bf16x8*>(Ap + ko);
+      ra1 = *reinterpret_cast<const bf16x8*>(Ap + a32 + ko);
+      rb0 = *reinterpret_cast<const bf16x8*>(Bp + ko);
+      rb1 = *reinterpret_cast<const bf16x8*>(Bp + b32 + ko);
+    }
+    const bf16* as = &As[cur][ao];
+    const bf16* bs = &Bs[cur][bo];
+#pragma unroll
+    for (int ks = 0; ks < TK; ks += 16) {
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(as + ks);
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(bs + ks);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    if (more) {
+      *reinterpret_cast<bf16x8*>(&As[cur ^ 1][so0]) = ra0;
+      *reinterpret_cast<bf16x8*>(&As[cur ^ 1][so1]) = ra1;
+      *reinterpret_cast<bf16x8*>(&Bs[cur ^ 1][so0]) = rb0;
+      *reinterpret_cast<bf16x8*>(&Bs[cur ^ 1][so1]) = rb1;
+    }
+    __syncthreads();
+  }
+  // accumulator j of lane l holds row 8 (j / 4) + 4 (l / 32) + j % 4, column l % 32
+  const int col = n0 + 32 * wc + (lane & 31);
+  const int rbase = m0 + 32 * wr + 4 * (lane >> 5);
+  float bcol = 0.0f;
+  if (EPI == EPI_GELU || EPI == EPI_BIAS) bcol = g.bias[col];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int row0 = rbase + 8 * q;
+    bf16x4 tv;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int row = row0 + e;
+      const float v = acc[4 * q + e];
+      if (EPI == EPI_GELU) {
+        const float z = v + bcol;
+        g.Cf[(size_t)row * g.ldc + col] = z;
+        const bf16 hb = (bf16)gelu(z);
+        g.Cb[(size_t)row * g.ldcb + col] = hb;
+        tv[e] = hb;
+      } else if (EPI == EPI_BIAS) {
+        g.Cf[(size_t)row * g.ldc + col] = v + bcol;
+      } else if (EPI == EPI_DGELU) {
+        const bf16 db = (bf16)(v * gelu_d(g.Zin[(size_t)row * g.ldz + col]));
+        g.Cb[(size_t)row * g.ldcb + col] = db;
+        tv[e] = db;
+      } else {
+        g.Cf[(size_t)row * g.ldc + col] = v;
+      }
+    }
+    if ((EPI == EPI_GELU || EPI == EPI_DGELU) && g.Ct)
+      *reinterpret_cast<bf16x4*>(&g.Ct[(size_t)col * g.ldct + row0]) = tv;
+  }
+}
+
+// Relative-L2 loss of each row; dL/dy = (y - t) / (B ||y - t|| ||t||) as BF16,
+// row-major and transposed.  One workgroup (256 threads) per row.
+__global__ void __launch_bounds__(256) rel_l2_loss(const float* __restrict__ y,
+                                                   const float* __restrict__ t, int n, float inv_b,
+                                                   int batch, bf16* __restrict__ dy,
+                                                   bf16* __restrict__ dyT,
+                                                   float* __restrict__ row_loss) {
+  __shared__ float red[2][4];
+  const int r = blockIdx.x;
+  const float* yr = y + (size_t)r * n;
+  const float* tr = t + (size_t)r * n;
+  float se = 0.f, st = 0.f;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const float d = yr[i] - tr[i];
+    se += d * d;
+    st += tr[i] * tr[i];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    se += __shfl_xor(se, o);
+    st += __shfl_xor(st, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = se;
+    red[1][threadIdx.x >> 6] = st;
+  }
+  __syncthreads();
+  const float E = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+  const float T = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+  const float ne = sqrtf(E), nt = T > 0.f ? sqrtf(T) : 1.f;
+  if (threadIdx.x == 0) row_loss[r] = ne / nt;
+  if (!dy) return;
+  const float sc = ne > 0.f ? inv_b / (ne * nt) : 0.f;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const bf16 v = (bf16)(sc * (yr[i] - tr[i]));
+    dy[(size_t)r * n + i] = v;
+    dyT[(size_t)i * batch + r] = v;
+  }
+}
+
+// mean of the row losses into hist[slot] (one wave, fixed order)
+__global__ void loss_mean(const float* __restrict__ row_loss, int batch, float* __restrict__ hist,
+                          int slot) {
+  float s = 0.f;
+  for (int i = threadIdx.x; i < batch; i += 64) s += row_loss[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (threadIdx.x == 0) hist[slot] = s / (float)batch;
+}
+
+// bias gradient: db[n] = sum_b dZT[n][b], one wave per n
+__global__ void __launch_bounds__(256) row_sum(const bf16* __restrict__ dT, int rows, int cols,
+                                               float* __restrict__ out) {
+  const int n = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (n >= rows) return;
+  float s = 0.f;
+  for (int b = lane; b < cols; b += 64) s += (float)dT[(size_t)n * cols + b];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (lane == 0) out[n] = s;
+}
+
+// Adam (Kingma & Ba) on the FP32 masters
+__global__ void adam_step(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                          float* __restrict__ v, size_t n, float lr, float b1, float b2, float eps,
+                          float c1, float c2) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  }
+}
+
+// BF16 copies of one weight matrix W [rows][cols] (FP32): Wt = W, Wn = W^T
+// (optional).  64 x 64 tiles through LDS; rows, cols multiples of 64.
+__global__ void __launch_bounds__(256) weight_copies(const float* __restrict__ W, int rows,
+                                                     int cols, bf16* __restrict__ Wt,
+                                                     bf16* __restrict__ Wn) {
+  __shared__ bf16 tile[64][65];
+  const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  for (int i = threadIdx.x; i < 4096; i += 256) {
+    const int rr = i >> 6, cc = i & 63;
+    const bf16 v = (bf16)W[(size_t)(r0 + rr) * cols + c0 + cc];
+    Wt[(size_t)(r0 + rr) * cols + c0 + cc] = v;
+    tile[rr][cc] = v;
+  }
+  if (!Wn) return;
+  __syncthreads();
+  for (int i = threadIdx.x; i < 4096; i += 256) {
+    const int cc = i >> 6, rr = i & 63;
+    Wn[(size_t)(c0 + cc) * rows + r0 + rr] = tile[rr][cc];
+  }
+}
+
+// input rows (FP32, n_in wide) -> BF16 [batch][ld_in] and [ld_in][batch], zero padded;
+// rows come from a device dataset through idx (or in order when idx is null)
+__global__ void __launch_bounds__(256) load_inputs(const float* __restrict__ X,
+                                                   const int* __restrict__ idx, int batch, int n_in,
+                                                   int ld_in, bf16* __restrict__ h0,
+                                                   bf16* __restrict__ h0T) {
+  const int r = blockIdx.x;
+  const size_t src = idx ? (size_t)idx[r] : (size_t)r;
+  for (int c = threadIdx.x; c < ld_in; c += 256) {
+    const bf16 v = (bf16)(c < n_in ? X[src * n_in + c] : 0.f);
+    h0[(size_t)r * ld_in + c] = v;
+    if (h0T) h0T[(size_t)c * batch + r] = v;
+  }
+}
+
+__global__ void __launch_bounds__(256) load_targets(const float* __restrict__ T,
+                                                    const int* __restrict__ idx, int n_out,
+                                                    float* __restrict__ t) {
+  const int r = blockIdx.x;
+  const size_t src = idx ? (size_t)idx[r] : (size_t)r;
+  for (int c = threadIdx.x; c < n_out; c += 256) t[(size_t)r * n_out + c] = T[src * n_out + c];
+}
+
+}  // namespace ofxm
+
+using namespace ofxm;
+
+struct ofx_mlp {
+  int device = 0;
+  int n_in = 0, ld_in = 0, width = 0, n_out = 0, max_batch = 0;
+  int kin[4] = {0, 0, 0, 0}, kout[4] = {0, 0, 0, 0};  // kin[0] is ld_in on the device
+  size_t np_dev = 0, np_pub = 0;                      // parameters: device (padded) / public
+  size_t off_w[4] = {0, 0, 0, 0}, off_b[4] = {0, 0, 0, 0};
+  float *p = nullptr, *g = nullptr, *am = nullptr, *av = nullptr;
+  bf16* wt[4] = {nullptr, nullptr, nullptr, nullptr};  // [out][in]
+  bf16* wn[4] = {nullptr, nullptr, nullptr, nullptr};  // [in][out], layers 1..3
+  bf16* h[4] = {nullptr, nullptr, nullptr, nullptr};   // h[0] = input
+  bf16* hT[4] = {nullptr, nullptr, nullptr, nullptr};
+  float* z[4] = {nullptr, nullptr, nullptr, nullptr};  // z[1..3]
+  bf16* dz[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // dz[4] = dL/dy
+  bf16* dzT[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  float *y = nullptr, *t = nullptr, *row_loss = nullptr, *hist = nullptr;
+  float *xin = nullptr;  // host-batch staging (FP32 inputs)
+  int hist_cap = 0;
+  float *dsX = nullptr, *dsT = nullptr;
+  int ds_n = 0;
+  int* idx = nullptr;
+  int idx_cap = 0;
+  long long adam_t = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  double last_ms = 0.0;
+  std::string err;
+};
+
+static int mlp_fail(ofx_mlp* m, const char* what, hipError_t e) {
+  if (m) m->err = std::string(what) + ": " + hipGetErrorString(e);
+  return e == hipErrorOutOfMemory ? OFX_E_NOMEM : OFX_E_HIP;
+}
+#define MCHK(m, call)                                        \
+  do {                                                       \
+    hipError_t _e = (call);                                  \
+    if (_e != hipSuccess) return mlp_fail((m), #call, _e);   \
+  } while (0)
+#define MLAUNCH(m) MCHK(m, hipGetLastError())
+
+static int arg_err(ofx_mlp* m, const char* msg) {
+  if (m) m->err = msg;
+  return OFX_E_ARG;
+}
+
+template <int EPI>
+static int gemm(ofx_mlp* m, const GemmArgs& a) {
+  // every dimension is a multiple of the tile; checked once at create
+  if (a.M % TM || a.N % TN || a.K % TK || a.M <= 0 || a.N <= 0 || a.K <= 0)
+    return arg_err(m, "GEMM dimensions are not multiples of 64");
+  hipLaunchKernelGGL(gemm_nt<EPI>, dim3(a.N / TN, a.M / TM), dim3(256), 0, m->stream, a);
+  MLAUNCH(m);
+  return OFX_OK;
+}
+
+static int refresh_weights(ofx_mlp* m) {
+  for (int l = 0; l < 4; l++) {
+    hipLaunchKernelGGL(weight_copies, dim3(m->kin[l] / 64, m->kout[l] / 64), dim3(256), 0,
+                       m->stream, m->p + m->off_w[l], m->kout[l], m->kin[l], m->wt[l], m->wn[l]);
+    MLAUNCH(m);
+  }
+  return OFX_OK;
+}
+
+static int forward(ofx_mlp* m, int B) {
+  for (int l = 0; l < 4; l++) {
+    GemmArgs a{};
+    a.A = m->h[l];
+    a.lda = m->kin[l];
+    a.B = m->wt[l];
+    a.ldb = m->kin[l];
+    a.M = B;
+    a.N = m->kout[l];
+    a.K = m->kin[l];
+    a.bias = m->p + m->off_b[l];
+    int st;
+    if (l < 3) {
+      a.Cf = m->z[l + 1];
+      a.ldc = m->width;
+      a.Cb = m->h[l + 1];
+      a.ldcb = m->width;
+      a.Ct = m->hT[l + 1];
+      a.ldct = B;
+      st = gemm<EPI_GELU>(m, a);
+    } else {
+      a.Cf = m->y;
+      a.ldc = m->n_out;
+      st = gemm<EPI_BIAS>(m, a);
+    }
+    if (st) return st;
+  }
+  return OFX_OK;
+}
+
+// loss, gradients into m->g (device layout), mean loss into hist[slot]
+static int backward(ofx_mlp* m, int B, int slot) {
+  hipLaunchKernelGGL(rel_l2_loss, dim3(B), dim3(256), 0, m->stream, m->y, m->t, m->n_out,
+                     1.0f / (float)B, B, m->dz[4], m->dzT[4], m->row_loss);
+  MLAUNCH(m);
+  hipLaunchKernelGGL(loss_mean, dim3(1), dim3(64), 0, m->stream, m->row_loss, B, m->hist, slot);
+  MLAUNCH(m);
+  for (int l = 3; l >= 0; l--) {
+    // dWt_l [out][in] = dZ^T . H : A = dzT[l+1] [out][B], B = hT[l] [in][B]
+    GemmArgs a{};
+    a.A = m->dzT[l + 1];
+    a.lda = B;
+    a.B = m->hT[l];
+    a.ldb = B;
+    a.M = m->kout[l];
+    a.N = m->kin[l];
+    a.K = B;
+    a.Cf = m->g + m->off_w[l];
+    a.ldc = m->kin[l];
+    int st = gemm<EPI_F32>(m, a);
+    if (st) return st;
+    hipLaunchKernelGGL(row_sum, dim3((m->kout[l] + 3) / 4), dim3(256), 0, m->stream, m->dzT[l + 1],
+                       m->kout[l], B, m->g + m->off_b[l]);
+    MLAUNCH(m);
+    if (l == 0) break;
+    // dZ_l = (dZ_{l+1} . Wn_l^T) * gelu'(z_l) : A = dz[l+1] [B][out], B = wn[l] [in][out]
+    GemmArgs d{};
+    d.A = m->dz[l + 1];
+    d.lda = m->kout[l];
+    d.B = m->wn[l];
+    d.ldb = m->kout[l];
+    d.M = B;
+    d.N = m->kin[l];
+    d.K = m->kout[l];
+    d.Zin = m->z[l];
+    d.ldz = m->width;
+    d.Cb = m->dz[l];
+    d.ldcb = m->width;
+    d.Ct = m->dzT[l];
+    d.ldct = B;
+    st = gemm<EPI_DGELU>(m, d);
+    if (st) return st;
+  }
+  return OFX_OK;
+}
+
+static int adam(ofx_mlp* m, float lr) {
+  m->adam_t++;
+  const double b1 = 0.9, b2 = 0.999;
+  const float c1 = (float)(1.0 - pow(b1, (double)m->adam_t));
+  const float c2 = (float)(1.0 - pow(b2, (double)m->adam_t));
+  hipLaunchKernelGGL(adam_step, dim3(2048), dim3(256), 0, m->stream, m->p, m->g, m->am, m->av,
+                     m->np_dev, lr, (float)b1, (float)b2, 1e-8f, c1, c2);
+  MLAUNCH(m);
+  return refresh_weights(m);
+}
+
+// host batch -> device inputs (and targets)
+static int stage_batch(ofx_mlp* m, int B, const float* x, const float* t) {
+  MCHK(m, hipMemcpyAsync(m->xin, x, sizeof(float) * (size_t)B * m->n_in, hipMemcpyHostToDevice,
+                         m->stream));
+  hipLaunchKernelGGL(load_inputs, dim3(B), dim3(256), 0, m->stream, m->xin, (const int*)nullptr, B,
+                     m->n_in, m->ld_in, m->h[0], m->hT[0]);
+  MLAUNCH(m);
+  if (t)
+    MCHK(m, hipMemcpyAsync(m->t, t, sizeof(float) * (size_t)B * m->n_out, hipMemcpyHostToDevice,
+                           m->stream));
+  return OFX_OK;
+}
+
+static int check_batch(ofx_mlp* m, int B) {
+  if (B < 64 || B > m->max_batch || B % 64) return arg_err(m, "batch must be a multiple of 64 in [64, max_batch]");
+  return OFX_OK;
+}
+
+// public <-> device parameter layouts differ only in layer 0's padded columns
+static int copy_params(ofx_mlp* m, float* dst_dev, const float* src_host, float* dst_host,
+                       const float* src_dev) {
+  const size_t w0 = (size_t)m->width * m->n_in;
+  const hipMemcpyKind k = src_host ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+  if (src_host) {
+    MCHK(m, hipMemsetAsync(dst_dev, 0, sizeof(float) * (size_t)m->width * m->ld_in, m->stream));
+    MCHK(m, hipMemcpy2DAsync(dst_dev, sizeof(float) * m->ld_in, src_host, sizeof(float) * m->n_in,
+                             sizeof(float) * m->n_in, m->width, k, m->stream));
+    MCHK(m, hipMemcpyAsync(dst_dev + m->off_b[0], src_host + w0,
+                           sizeof(float) * (m->np_pub - w0), k, m->stream));
+  } else {
+    MCHK(m, hipMemcpy2DAsync(dst_host, sizeof(float) * m->n_in, src_dev, sizeof(float) * m->ld_in,
+                             sizeof(float) * m->n_in, m->width, k, m->stream));
+    MCHK(m, hipMemcpyAsync(dst_host + w0, src_dev + m->off_b[0],
+                           sizeof(float) * (m->np_pub - w0), k, m->stream));
+  }
+  MCHK(m, hipStreamSynchronize(m->stream));
+  return OFX_OK;
+}
+
+extern "C" {
+
+void ofx_mlp_destroy(ofx_mlp* m);
+
+int ofx_mlp_create(int device, int n_in, int width, int n_out, int max_batch, ofx_mlp** out) {
+  if (!out) return OFX_E_ARG;
+  *out = nullptr;
+  if (n_in < 1 || width < 64 || width % 64 || n_out < 64 || n_out % 64 || max_batch < 64 ||
+      max_batch % 64)
+    return OFX_E_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return OFX_E_NODEVICE;
+  if (device < 0 || device >= ndev) return OFX_E_ARG;
+  if (hipSetDevice(device) != hipSuccess) return OFX_E_HIP;
+  ofx_mlp* m = new ofx_mlp();
+  m->device = device;
+  m->n_in = n_in;
+  m->ld_in = (n_in + 63) / 64 * 64;
+  m->width = width;
+  m->n_out = n_out;
+  m->max_batch = max_batch;
+  const int kin[4] = {m->ld_in, width, width, width};
+  const int kout[4] = {width, width, width, n_out};
+  size_t off = 0, pub = 0;
+  for (int l = 0; l < 4; l++) {
+    m->kin[l] = kin[l];
+    m->kout[l] = kout[l];
+    m->off_w[l] = off;
+    off += (size_t)kin[l] * kout[l];
+    m->off_b[l] = off;
+    off += kout[l];
+    pub += (size_t)(l == 0 ? n_in : kin[l]) * kout[l] + kout[l];
+  }
+  m->np_dev = off;
+  m->np_pub = pub;
+  hipError_t e = hipSuccess;
+  auto al = [&](void** ptr, size_t bytes) {
+    if (e == hipSuccess) e = hipMalloc(ptr, bytes);
+  };
+  const size_t Bm = (size_t)max_batch;
+  al((void**)&m->p, 4 * off);
+  al((void**)&m->g, 4 * off);
+  al((void**)&m->am, 4 * off);
+  al((void**)&m->av, 4 * off);
+  for (int l = 0; l < 4; l++) {
+    al((void**)&m->wt[l], 2 * (size_t)kin[l] * kout[l]);
+    if (l > 0) al((void**)&m->wn[l], 2 * (size_t)kin[l] * kout[l]);
+    al((void**)&m->h[l], 2 * Bm * kin[l]);
+    al((void**)&m->hT[l], 2 * Bm * kin[l]);
+  }
+  for (int l = 1; l < 4; l++) {
+    al((void**)&m->z[l], 4 * Bm * width);
+    al((void**)&m->dz[l], 2 * Bm * width);
+    al((void**)&m->dzT[l], 2 * Bm * width);
+  }
+  al((void**)&m->dz[4], 2 * Bm * n_out);
+  al((void**)&m->dzT[4], 2 * Bm * n_out);
+  al((void**)&m->y, 4 * Bm * n_out);
+  al((void**)&m->t, 4 * Bm * n_out);
+  al((void**)&m->row_loss, 4 * Bm);
+  al((void**)&m->xin, 4 * Bm * n_in);
+  m->hist_cap = 1024;
+  al((void**)&m->hist, 4 * (size_t)m->hist_cap);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&m->e0);
+  if (e == hipSuccess) e = hipEventCreate(&m->e1);
+  if (e == hipSuccess) e = hipMemsetAsync(m->p, 0, 4 * off, m->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(m->am, 0, 4 * off, m->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(m->av, 0, 4 * off, m->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+  if (e != hipSuccess) {
+    const int code = e == hipErrorOutOfMemory ? OFX_E_NOMEM : OFX_E_HIP;
+    ofx_mlp_destroy(m);
+    return code;
+  }
+  if (refresh_weights(m) || hipStreamSynchronize(m->stream) != hipSuccess) {
+    ofx_mlp_destroy(m);
+    return OFX_E_HIP;
+  }
+  *out = m;
+  return OFX_OK;
+}
+
+void ofx_mlp_destroy(ofx_mlp* m) {
+  if (!m) return;
+  (void)hipSetDevice(m->device);
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  void* ptrs[] = {m->p,      m->g,      m->am,     m->av,     m->wt[0],  m->wt[1],  m->wt[2],
+                  m->wt[3],  m->wn[1],  m->wn[2],  m->wn[3],  m->h[0],   m->h[1],   m->h[2],
+                  m->h[3],   m->hT[0],  m->hT[1],  m->hT[2],  m->hT[3],  m->z[1],   m->z[2],
+                  m->z[3],   m->dz[1],  m->dz[2],  m->dz[3],  m->dz[4],  m->dzT[1], m->dzT[2],
+                  m->dzT[3], m->dzT[4], m->y,      m->t,      m->row_loss, m->xin,  m->hist,
+                  m->dsX,    m->dsT,    m->idx};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  if (m->e0) (void)hipEventDestroy(m->e0);
+  if (m->e1) (void)hipEventDestroy(m->e1);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+}
+
+const char* ofx_mlp_last_error(const ofx_mlp* m) { return m ? m->err.c_str() : "null mlp"; }
+
+long long ofx_mlp_num_params(const ofx_mlp* m) { return m ? (long long)m->np_pub : 0; }
+
+int ofx_mlp_set_params(ofx_mlp* m, const float* params) {
+  if (!m || !params) return OFX_E_ARG;
+  MCHK(m, hipSetDevice(m->device));
+  int st = copy_params(m, m->p, params, nullptr, nullptr);
+  if (st) return st;
+  MCHK(m, hipMemsetAsync(m->am, 0, 4 * m->np_dev, m->stream));
+  MCHK(m, hipMemsetAsync(m->av, 0, 4 * m->np_dev, m->stream));
+  m->adam_t = 0;
+  st = refresh_weights(m);
+  if (st) return st;
+  MCHK(m, hipStreamSynchronize(m->stream));
+  return OFX_OK;
+}
+
+int ofx_mlp_get_params(ofx_mlp* m, float* params) {
+  if (!m || !params) return OFX_E_ARG;
+  MCHK(m, hipSetDevice(m->device));
+  return copy_params(m, nullptr, nullptr, params, m->p);
+}
+
+int ofx_mlp_forward(ofx_mlp* m, int batch, const float* x, float* y) {
+  if (!m || !x || !y) return OFX_E_ARG;
+  int st = check_batch(m, batch);
+  if (st) return st;
+  MCHK(m, hipSetDevice(m->device));
+  MCHK(m, hipEventRecord(m->e0, m->stream));
+  st = stage_batch(m, batch, x, nullptr);
+  if (!st) st = forward(m, batch);
+  if (st) return st;
+  MCHK(m, hipMemcpyAsync(y, m->y, sizeof(float) * (size_t)batch * m->n_out, hipMemcpyDeviceToHost,
+                         m->stream));
+  MCHK(m, hipEventRecord(m->e1, m->stream));
+  MCHK(m, hipStreamSynchronize(m->stream));
+  float ms = 0.f;
+  MCHK(m, hipEventElapsedTime(&ms, m->e0, m->e1));
+  m->last_ms = ms;
+  return OFX_OK;
+}
+
+int ofx_mlp_gradients(ofx_mlp* m, int batch, const float* x, const float* t, float* loss,
+                      float* grads) {
+  if (!m || !x || !t) return OFX_E_ARG;
+  int st = check_batch(m, batch);
+  if (st) return st;
+  MCHK(m, hipSetDevice(m->device));
+  st = stage_batch(m, batch, x, t);
+  if (!st) st = forward(m, batch);
+  if (!st) st = backward(m, batch, 0);
+  if (st) return st;
+  if (loss) MCHK(m, hipMemcpyAsync(loss, m->hist, sizeof(float), hipMemcpyDeviceToHost, m->stream));
+  MCHK(m, hipStreamSynchronize(m->stream));
+  if (grads) return copy_params(m, nullptr, nullptr, grads, m->g);
+  return OFX_OK;
+}
+
+int ofx_mlp_step(ofx_mlp* m, int batch, const float* x, const float* t, float lr, float* loss) {
+  if (!m || !x || !t || !(lr >= 0.f)) return OFX_E_ARG;
+  int st = check_batch(m, batch);
+  if (st) return st;
+  MCHK(m, hipSetDevice(m->device));
+  st = stage_batch(m, batch, x, t);
+  if (!st) st = forward(m, batch);
+  if (!st) st = backward(m, batch, 0);
+  if (!st) st = adam(m, lr);
+  if (st) return st;
+  if (loss) MCHK(m, hipMemcpyAsync(loss, m->hist, sizeof(float), hipMemcpyDeviceToHost, m->stream));
+  MCHK(m, hipStreamSynchronize(m->stream));
+  return OFX_OK;
+}
+
+int ofx_mlp_upload_dataset(ofx_mlp* m, int n, const float* X, const float* T) {
+  if (!m || n < 1 || !X || !T) return OFX_E_ARG;
+  MCHK(m, hipSetDevice(m->device));
+  MCHK(m, hipStreamSynchronize(m->stream));
+  if (m->dsX) (void)hipFree(m->dsX);
+  if (m->dsT) (void)hipFree(m->dsT);
+  m->dsX = m->dsT = nullptr;
+  m->ds_n = 0;
+  MCHK(m, hipMalloc((void**)&m->dsX, sizeof(float) * (size_t)n * m->n_in));
+  MCHK(m, hipMalloc((void**)&m->dsT, sizeof(float) * (size_t)n * m->n_out));
+  MCHK(m, hipMemcpy(m->dsX, X, sizeof(float) * (size_t)n * m->n_in, hipMemcpyHostToDevice));
+  MCHK(m, hipMemcpy(m->dsT, T, sizeof(float) * (size_t)n * m->n_out, hipMemcpyHostToDevice));
+  m->ds_n = n;
+  return OFX_OK;
+}
+
+// `steps` Adam steps on rows idx[s*batch .. s*batch+batch-1] of the uploaded
+// dataset (every index in [0, n)); losses[s] = mean batch loss at step s
+// (before its update).  The device time of the span is kept for ofx_mlp_last_ms.
+int ofx_mlp_train(ofx_mlp* m, int batch, int steps, const int* idx, float lr, float* losses) {
+  if (!m || !idx || steps < 1 || !(lr >= 0.f)) return OFX_E_ARG;
+  int st = check_batch(m, batch);
+  if (st) return st;
+  if (!m->dsX) return arg_err(m, "no dataset uploaded");
+  for (size_t i = 0; i < (size_t)steps * batch; i++)
+    if (idx[i] < 0 || idx[i] >= m->ds_n) return arg_err(m, "dataset index out of range");
+  MCHK(m, hipSetDevice(m->device));
+  MCHK(m, hipStreamSynchronize(m->stream));
+  if (m->idx_cap < steps * batch) {
+    if (m->idx) (void)hipFree(m->idx);
+    m->idx = nullptr;
+    m->idx_cap = 0;
+    MCHK(m, hipMalloc((void**)&m->idx, sizeof(int) * (size_t)steps * batch));
+    m->idx_cap = steps * batch;
+  }
+  if (m->hist_cap < steps) {
+    (void)hipFree(m->hist);
+    m->hist = nullptr;
+    m->hist_cap = 0;
+    MCHK(m, hipMalloc((void**)&m->hist, sizeof(float) * (size_t)steps));
+    m->hist_cap = steps;
+  }
+  MCHK(m, hipMemcpyAsync(m->idx, idx, sizeof(int) * (size_t)steps * batch, hipMemcpyHostToDevice,
+                         m->stream));
+  MCHK(m, hipEventRecord(m->e0, m->stream));
+  for (int s = 0; s < steps; s++) {
+    const int* ix = m->idx + (size_t)s * batch;
+    hipLaunchKernelGGL(load_inputs, dim3(batch), dim3(256), 0, m->stream, m->dsX, ix, batch,
+                       m->n_in, m->ld_in, m->h[0], m->hT[0]);
+    MLAUNCH(m);
+    hipLaunchKernelGGL(load_targets, dim3(batch), dim3(256), 0, m->stream, m->dsT, ix, m->n_out,
+                       m->t);
+    MLAUNCH(m);
+    st = forward(m, batch);
+    if (!st) st = backward(m, batch, s);
+    if (!st) st = adam(m, lr);
+    if (st) return st;
+  }
+  MCHK(m, hipEventRecord(m->e1, m->stream));
+  MCHK(m, hipEventSynchronize(m->e1));
+  float ms = 0.f;
+  MCHK(m, hipEventElapsedTime(&ms, m->e0, m->e1));
+  m->last_ms = ms;
+  if (losses)
+    MCHK(m, hipMemcpy(losses, m->hist, sizeof(float) * (size_t)steps, hipMemcpyDeviceToHost));
+  return OFX_OK;
+}
+
+double ofx_mlp_last_ms(const ofx_mlp* m) { return m ? m->last_ms : 0.0; }
+
+}  // extern "C"
