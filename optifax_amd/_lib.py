@@ -119,22 +119,23 @@ def load(path: str | None = None):
     L.ofx_opo_write_field.argtypes = [vp, ctypes.c_int, _DP]
     L.ofx_opo_geometry.argtypes = [vp, ctypes.c_int, _IP, _IP, _IP, _IP, _IP]
     L.ofx_opo_xcc_ids.argtypes = [vp, _IP, ctypes.c_int, _IP]
-    L.ofx_mlp_create.argtypes = [ctypes.c_int] * 5 + [ctypes.POINTER(vp)]
-    L.ofx_mlp_destroy.argtypes = [vp]
-    L.ofx_mlp_destroy.restype = None
-    L.ofx_mlp_last_error.argtypes = [vp]
-    L.ofx_mlp_last_error.restype = ctypes.c_char_p
-    L.ofx_mlp_num_params.argtypes = [vp]
-    L.ofx_mlp_num_params.restype = ctypes.c_longlong
-    L.ofx_mlp_set_params.argtypes = [vp, _FP]
-    L.ofx_mlp_get_params.argtypes = [vp, _FP]
-    L.ofx_mlp_forward.argtypes = [vp, ctypes.c_int, _FP, _FP]
-    L.ofx_mlp_gradients.argtypes = [vp, ctypes.c_int, _FP, _FP, _FP, _FP]
-    L.ofx_mlp_step.argtypes = [vp, ctypes.c_int, _FP, _FP, ctypes.c_float, _FP]
-    L.ofx_mlp_upload_dataset.argtypes = [vp, ctypes.c_int, _FP, _FP]
-    L.ofx_mlp_train.argtypes = [vp, ctypes.c_int, ctypes.c_int, _IP, ctypes.c_float, _FP]
-    L.ofx_mlp_last_ms.argtypes = [vp]
-    L.ofx_mlp_last_ms.restype = ctypes.c_double
+    if hasattr(L, "ofx_mlp_create"):   # the CPU emulation library has no surrogate
+        L.ofx_mlp_create.argtypes = [ctypes.c_int] * 5 + [ctypes.POINTER(vp)]
+        L.ofx_mlp_destroy.argtypes = [vp]
+        L.ofx_mlp_destroy.restype = None
+        L.ofx_mlp_last_error.argtypes = [vp]
+        L.ofx_mlp_last_error.restype = ctypes.c_char_p
+        L.ofx_mlp_num_params.argtypes = [vp]
+        L.ofx_mlp_num_params.restype = ctypes.c_longlong
+        L.ofx_mlp_set_params.argtypes = [vp, _FP]
+        L.ofx_mlp_get_params.argtypes = [vp, _FP]
+        L.ofx_mlp_forward.argtypes = [vp, ctypes.c_int, _FP, _FP]
+        L.ofx_mlp_gradients.argtypes = [vp, ctypes.c_int, _FP, _FP, _FP, _FP]
+        L.ofx_mlp_step.argtypes = [vp, ctypes.c_int, _FP, _FP, ctypes.c_float, _FP]
+        L.ofx_mlp_upload_dataset.argtypes = [vp, ctypes.c_int, _FP, _FP]
+        L.ofx_mlp_train.argtypes = [vp, ctypes.c_int, ctypes.c_int, _IP, ctypes.c_float, _FP]
+        L.ofx_mlp_last_ms.argtypes = [vp]
+        L.ofx_mlp_last_ms.restype = ctypes.c_double
     if path is None:
         _lib = L
     return L
