@@ -126,6 +126,8 @@ typedef struct {
   const double* pump;    /* [2n]  pump spectrum in the crystal, or NULL        */
   const double* cav;     /* [2n]  crystal-out -> next crystal-in transfer, or NULL (single passes) */
   const double* ocm;     /* [2n]  crystal-out -> output-coupled spectrum, or NULL */
+  int cluster;           /* this instance's workgroups (power of two), 0 = batch tuning / automatic */
+  int n1;                /* this instance's four-step split, 0 = batch tuning / automatic */
 } ofx_opo_spec;
 
 int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_spec* specs,

@@ -80,7 +80,7 @@ _LLP = ctypes.POINTER(ctypes.c_longlong)
 class OpoSpec(ctypes.Structure):
     _fields_ = [("pass_", PassParams), ("delay", ctypes.c_double), ("beta", _DP),
                 ("alpha", _DP), ("G", _DP), ("field", _DP), ("pump", _DP), ("cav", _DP),
-                ("ocm", _DP)]
+                ("ocm", _DP), ("cluster", ctypes.c_int), ("n1", ctypes.c_int)]
 
 
 _lib = None

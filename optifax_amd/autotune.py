@@ -1,0 +1,178 @@
+"""Launch-geometry autotuner for the persistent crystal-pass kernel.
+
+For a grid size N, a precision and the number of instances B of that size
+that will share the GPU, the tunable schedule of the kernel is
+
+  cluster  C   workgroups per instance (the field is split over C LDS tiles)
+  n1           four-step split N = n1 * n2 (row FFTs of n1 and n2 points,
+               radix-4 Stockham passes with a radix-2 tail when log2 is odd)
+  block        threads per workgroup
+
+Every candidate that fits the LDS limit is timed on the caller's own
+instance, truncated to its first `steps` minimal steps, with k copies in one
+launch, k = min(B, slots // C) (the number that run at once).  The makespan
+of B instances is then ceil(B / k) * t_step(C, n1, block, k) per step.  The
+lowest makespan wins.  Results are cached per (device, N, precision, B) in
+memory and in a JSON file (OFX_TUNE_CACHE, default ~/.cache/optifax_amd/
+tune.json).
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import time
+
+import numpy as np
+
+from . import _lib
+
+_MEM: dict = {}
+
+
+def _cache_path():
+    return os.environ.get("OFX_TUNE_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "optifax_amd",
+                                       "tune.json"))
+
+
+def _load_cache():
+    p = _cache_path()
+    try:
+        with open(p) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store_cache(key, value):
+    p = _cache_path()
+    try:
+        os.makedirs(os.path.dirname(p), exist_ok=True)
+        d = _load_cache()
+        d[key] = value
+        tmp = p + f".{os.getpid()}.tmp"
+        with open(tmp, "w") as f:
+            json.dump(d, f, indent=1, sort_keys=True)
+        os.replace(tmp, p)
+    except OSError:
+        pass
+
+
+def candidates(n, max_cluster=64):
+    """(cluster, n1, block) triples to try for grid size n."""
+    lg = int(round(math.log2(n)))
+    if 1 << lg != n:
+        raise ValueError("n must be a power of two")
+    splits = sorted({(lg + 1) // 2, (lg + 1) // 2 + 1, lg // 2})
+    out = []
+    for s1 in splits:
+        n1 = 1 << s1
+        n2 = n // n1
+        if n1 < 4 or n2 < 4:
+            continue
+        C = 1
+        while C <= min(n1, n2, max_cluster):
+            for block in (256, 512):
+                out.append((C, n1, block))
+            C *= 2
+    return out
+
+
+def _truncate(inst, steps):
+    s = dict(inst["solver"])
+    ns = int(s["nSteps"])
+    k = max(1, min(ns, int(steps)))
+    s["nSteps"] = k
+    s["G33"] = np.asarray(s["G33"])[:k]
+    # alpha_w = T^(1/NSteps): keep the per-step loss of the full crystal
+    return dict(solver=s, field0=inst["field0"], delay=inst.get("delay", 0.0))
+
+
+def measure(inst, cand, k, precision="single", steps=1500, ctx=None, repeats=2):
+    """Seconds per adaptive step of one pass with k concurrent copies, or None
+    when the candidate cannot launch."""
+    from .solver import OPOBatch
+    C, n1, block = cand
+    proto = _truncate(inst, steps)
+    proto.update(cluster=C, n1=n1)
+    try:
+        run = OPOBatch([proto] * k, precision=precision, tune={"block": block}, ctx=ctx)
+    except _lib.OfxError:
+        return None
+    try:
+        best = None
+        for _ in range(repeats):
+            run.launch(0)
+            run.sync(allow_nonfinite=True)
+            ms = run.last_ms()
+            r = run.read(0)
+            att = max(1, r["attempts"])
+            t = ms * 1e-3 / att
+            best = t if best is None else min(best, t)
+        return best
+    finally:
+        run.close()
+
+
+def tune(inst, batch=1, precision="single", steps=1500, ctx=None, use_cache=True, verbose=False):
+    """Best (cluster, n1, block) for `batch` instances shaped like `inst`."""
+    from .solver import default_context
+    ctx = ctx or default_context()
+    info = ctx.info()
+    n = int(np.asarray(inst["solver"]["beta_op"]).size)
+    key = f"{info['name']}|n={n}|{precision}|B={int(batch)}"
+    if use_cache:
+        if key in _MEM:
+            return dict(_MEM[key])
+        disk = _load_cache()
+        if key in disk:
+            _MEM[key] = disk[key]
+            return dict(disk[key])
+    slots = int(info["cus"])
+    results = []
+    t0 = time.time()
+    for cand in candidates(n):
+        C = cand[0]
+        k = max(1, min(int(batch), slots // C))
+        t = measure(inst, cand, k, precision, steps, ctx)
+        if t is None:
+            continue
+        waves = math.ceil(batch / k)
+        results.append((waves * t, t, k, cand))
+        if verbose:
+            print(f"  C={C:3d} n1={cand[1]:5d} block={cand[2]} k={k:3d}: "
+                  f"{t * 1e6:8.2f} us/step, makespan {waves * t * 1e6:8.2f} us/step", flush=True)
+    if not results:
+        raise RuntimeError(f"no launchable geometry for N={n}")
+    results.sort(key=lambda r: r[0])
+    span, t, k, (C, n1, block) = results[0]
+    best = dict(cluster=C, n1=n1, block=block, us_per_step=t * 1e6, concurrent=k,
+                makespan_us_per_step=span * 1e6, candidates=len(results),
+                tuning_s=time.time() - t0)
+    _MEM[key] = best
+    if use_cache:
+        _store_cache(key, best)
+    return dict(best)
+
+
+def tune_batch(instances, precision="single", steps=1500, ctx=None, verbose=False):
+    """Per-instance geometry for an uneven batch: instances are grouped by N,
+    each group tuned for its own size and count (first instance of the group
+    as the prototype).  Returns (instances with cluster/n1 set, block)."""
+    groups: dict = {}
+    for i, inst in enumerate(instances):
+        n = int(np.asarray(inst["solver"]["beta_op"]).size)
+        groups.setdefault(n, []).append(i)
+    out = [dict(inst) for inst in instances]
+    blocks = {}
+    for n, ids in groups.items():
+        best = tune(instances[ids[0]], batch=len(ids), precision=precision, steps=steps, ctx=ctx,
+                    verbose=verbose)
+        for i in ids:
+            out[i]["cluster"] = best["cluster"]
+            out[i]["n1"] = best["n1"]
+        blocks[n] = (len(ids), best["block"])
+    # one block size per launch: the one chosen for the most instances
+    block = max(blocks.values())[1]
+    return out, block

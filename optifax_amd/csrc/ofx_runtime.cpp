@@ -279,6 +279,7 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
     nmax = std::max(nmax, n);
     int n1 = 1 << ((lg + 1) / 2);
     if (tune && tune->n1) n1 = tune->n1;
+    if (specs[i].n1) n1 = specs[i].n1;
     if (ilog2i(n1) < 2 || n % n1 || n / n1 < 4) {
       delete r;
       return fail(ctx, OFX_E_ARG, "n1=%d does not split N=%d into two powers of two >= 4", n1, n);
@@ -297,8 +298,9 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
                   ctx->info.lds_max);
     }
     int C = cfit;
-    if (tune && tune->cluster) {
-      C = tune->cluster;
+    const int want_C = specs[i].cluster ? specs[i].cluster : (tune ? tune->cluster : 0);
+    if (want_C) {
+      C = want_C;
       if (ilog2i(C) < 0 || C > std::min(n1, n2) ||
           (int)lds_bytes_for(n1, n2, C, halo, r->csz) > ctx->info.lds_max - 1024) {
         delete r;

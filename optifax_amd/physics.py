@@ -220,16 +220,37 @@ def energy_of(simwin: SimWindow, E, area, n_medium=1.0):
     return float(np.sum(np.abs(Ek) ** 2 * (C0 * n_medium * EPS0 / 2) * area * simwin.dt / simwin.n))
 
 
+def smoothed_top_hat(x, a, b, w):
+    """0.5 (tanh((x - a)/w) - tanh((x - b)/w)) (smoothedTopHat.m, tanh form)."""
+    return 0.5 * (np.tanh((x - a) / w) - np.tanh((x - b) / w))
+
+
 @dataclass
 class OutputCoupler:
-    """Cavity output coupler in the "R" regime (Optic.m; OpticalPulse.m:163-188):
-    the intracavity field keeps sqrt(R) e^{-i phi}, the output gets sqrt(T) e^{-i phi}."""
+    """Resonator feedback and output coupling.
+
+    The cavity keeps R(lambda) of the power and couples out 1 - R:
+      R = (1 - transmission) * smoothed_top_hat(lambda, band, edge)
+    i.e. a singly resonant cavity: only the signal band is fed back; the
+    depleted pump and the idler leave every trip.  This is the reference's
+    OPO setup (toolbox/scripts/simulation_script_template.m:44-48: OC coating
+    0.5*smoothedTopHat(lam, 1.2e-6, 1.8e-6, 30e-9)) with the coupler's
+    transmission as a parameter.  band=None gives a flat coupler (everything
+    fed back), the "R" regime of Optic.m / OpticalPulse.m:163-188."""
     transmission: float = 0.05
     phase: float = 0.0
+    band: tuple | None = (1.2e-6, 1.8e-6)
+    edge: float = 30e-9
 
     def masks(self, simwin: SimWindow):
-        T = np.full(simwin.n, self.transmission)
-        R = 1.0 - T
+        lam = simwin.wavelengths
+        if self.band is None:
+            R = np.full(simwin.n, 1.0 - self.transmission)
+        else:
+            R = (1.0 - self.transmission) * smoothed_top_hat(lam, self.band[0], self.band[1],
+                                                            self.edge)
+            R = np.where(simwin.omegas > 0, R, 0.0)
+        T = 1.0 - R
         ph = np.full(simwin.n, self.phase)
         return R, T, ph
 
@@ -275,6 +296,7 @@ class OPOConfig:
     pump_energy: float = 2e-9       # 2 W at 1 GHz
     area: float = math.pi * (50e-6) ** 2
     oc_transmission: float = 0.05
+    oc_band: tuple | None = (1.2e-6, 1.8e-6)     # fed-back (signal) band; None = flat
     delay: float = 0.3e-12
     step: float = 1e-7
     error_bounds: tuple = (2.5e-3, 5e-2)   # OpticalSim.m:69 [min, max] percent
@@ -288,9 +310,10 @@ class OPOConfig:
         si = xt.solver_inputs(sw, self.step)
         pump = sech_pump(sw, self.pump_fwhm, self.pump_energy, self.area)
         noise = quantum_noise(sw, self.area, self.noise_seed)
-        ops = cavity_operators(sw, si["n_xtal"], OutputCoupler(self.oc_transmission), pump, noise)
+        oc = OutputCoupler(self.oc_transmission, band=self.oc_band)
+        ops = cavity_operators(sw, si["n_xtal"], oc, pump, noise)
         sel = math.ceil(si["nSteps"] / (self.progress_plots - 1))   # OpticalSim.m:214
         si.update(maxErr=self.error_bounds[1], minErr=self.error_bounds[0], sel=float(sel),
                   n_cols=self.progress_plots - 1)
         return dict(simwin=sw, crystal=xt, solver=si, pump=pump, noise=noise, ops=ops,
-                    delay=self.delay)
+                    delay=self.delay, oc=oc)

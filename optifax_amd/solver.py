@@ -170,12 +170,22 @@ class OPOBatch:
     Each instance is a dict with keys: solver (the dict of
     physics.NonlinearCrystal.solver_inputs plus maxErr, minErr, sel, n_cols),
     field0 (fftshifted crystal input, complex), and optionally pump, cav, ocm
-    (FFT-order complex spectra) and delay."""
+    (FFT-order complex spectra), delay, and cluster / n1 (this instance's
+    launch geometry; 0 or absent = batch tuning or automatic).
+    tune: None (built-in heuristic), a dict of cluster / n1 / block for the
+    whole batch, or "auto" (timed per grid size by optifax_amd.autotune)."""
 
     def __init__(self, instances, precision="single", trip_cap=0, tune=None,
                  ctx: Context | None = None):
         self.ctx = ctx or default_context()
         self.lib = self.ctx.lib
+        if isinstance(tune, str):
+            if tune != "auto":
+                raise ValueError("tune must be None, a dict or 'auto'")
+            from .autotune import tune_batch
+            instances, block = tune_batch(instances, precision=precision, ctx=self.ctx)
+            tune = {"block": block}
+        self.tuning = tune
         self.n_inst = len(instances)
         self.prec = _lib.OFX_F64 if precision == "double" else _lib.OFX_F32
         self.trip_cap = int(trip_cap)
@@ -196,6 +206,8 @@ class OPOBatch:
             specs[i].pump = self._c(inst["pump"]) if inst.get("pump") is not None else None
             specs[i].cav = self._c(inst["cav"]) if inst.get("cav") is not None else None
             specs[i].ocm = self._c(inst["ocm"]) if inst.get("ocm") is not None else None
+            specs[i].cluster = int(inst.get("cluster", 0))
+            specs[i].n1 = int(inst.get("n1", 0))
             self.meta.append(dict(n=n, nsteps=int(s["nSteps"]), n_cols=int(s.get("n_cols", 0)),
                                   cavity=inst.get("cav") is not None))
         h = ctypes.c_void_p()

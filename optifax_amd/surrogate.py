@@ -180,3 +180,69 @@ class SurrogateMLP:
 def _close_live():
     for m in list(_LIVE):
         m.close()
+
+
+# ---------------------------------------------------------------------------
+# Spectral features: 1024 bins of amplitude and unwrapped phase
+# ---------------------------------------------------------------------------
+class SpectralBins:
+    """A decimated spectral window of a SimWindow grid.
+
+    Bin j is the grid frequency with SimWindow (centred) index
+    k0 + D*j, j = 0..nbins-1, so the bins span nbins*D grid points.  D is a
+    power of two: sampling a spectrum every D points is exact for a field
+    confined to |t| < T/(2D) (T = window), and `to_grid` inverts it by
+    band-limited interpolation (periodised time field, one period kept).
+
+    Features of a spectrum X (FFT order, as the solver holds it):
+      amplitude  sqrt(|X_j| / a_ref)    (compressed: 4th root of power)
+      phase      unwrap(angle(X_j)) / pi (MATLAB unwrap along the bins,
+                 as OpticalPulse.m:318-322 computes SpectralPhase)
+    """
+
+    def __init__(self, simwin, f_lo_hz, nbins=1024, D=None, a_ref=1.0):
+        n = simwin.n
+        if D is None:
+            D = max(1, n // (4 * nbins))        # 16 ps window, 2^14 grid: D = 4
+        if D & (D - 1) or nbins * D > n:
+            raise ValueError("D must be a power of two with nbins*D <= n")
+        f = simwin.rel_frequencies + C0_HZ / simwin.reference_wave
+        k0 = int(np.searchsorted(f, f_lo_hz))
+        k0 = min(max(k0, 0), n - nbins * D)
+        self.n, self.nbins, self.D, self.k0 = n, int(nbins), int(D), k0
+        self.kc = k0 + D * np.arange(nbins)                  # centred indices
+        self.kf = (self.kc - n // 2) % n                     # FFT-order indices
+        self.freqs = f[self.kc]
+        self.a_ref = float(a_ref)
+        self.window = simwin.temporal_range
+
+    def bins(self, Xk):
+        """FFT-order spectra [..., n] -> complex bins [..., nbins]."""
+        return np.asarray(Xk)[..., self.kf]
+
+    def features(self, Xb):
+        """complex bins -> (amplitude, phase) feature arrays."""
+        amp = np.sqrt(np.abs(Xb) / self.a_ref)
+        ph = np.unwrap(np.angle(Xb), axis=-1) / np.pi
+        return amp, ph
+
+    def from_features(self, amp, ph):
+        return (self.a_ref * np.asarray(amp, np.float64) ** 2) * np.exp(1j * np.pi * np.asarray(ph))
+
+    def to_grid(self, Xb):
+        """complex bins -> FFT-order spectrum on the full grid (zero outside
+        the window band), band-limited: the time field of the bins is periodic
+        with period T/D; one period centred on t = 0 is kept."""
+        n, D = self.n, self.D
+        Xc = np.zeros(np.shape(Xb)[:-1] + (n,), np.complex128)
+        Xc[..., self.kc] = Xb
+        Xf = np.fft.ifftshift(Xc, axes=-1)                   # FFT order
+        e = np.fft.ifft(Xf, axis=-1)                         # time, FFT order (t = 0 first)
+        m = np.fft.fftfreq(n) * n                            # sample offsets from t = 0
+        keep = np.abs(m) < n / (2 * D)
+        edge = np.abs(m) == n / (2 * D)
+        w = keep * 1.0 + edge * 0.5
+        return np.fft.fft(e * (D * w), axis=-1)
+
+
+C0_HZ = 299792458.0

@@ -44,9 +44,9 @@ def test_round_trips_fp64(gpu_ctx):
     cfg, b = opo_case()
     si, ops, sw = b["solver"], b["ops"], b["simwin"]
     trips = 3
-    o = oracle.opo_run(trips, b["noise"], b["pump"], np.ones(sw.n), si["n_xtal"],
-                       np.full(sw.n, 1 - cfg.oc_transmission), np.full(sw.n, cfg.oc_transmission),
-                       np.zeros(sw.n), cfg.delay, sw.rel_omegas, si)
+    R, T, phi = b["oc"].masks(sw)
+    o = oracle.opo_run(trips, b["noise"], b["pump"], np.ones(sw.n), si["n_xtal"], R, T, phi,
+                       cfg.delay, sw.rel_omegas, si)
     for C in (1, 4):
         run = OPOBatch([dict(solver=si, field0=ops["field0"], pump=ops["pump"], cav=ops["cav"],
                              ocm=ops["ocm"], delay=cfg.delay)], precision="double",

@@ -67,7 +67,9 @@ def test_cavity_operators_compose_the_reference_trip():
     X = np.fft.fftshift(ph.SimWindow.k2t(np.random.default_rng(3).standard_normal(512) + 0j))
     Xn = np.fft.ifftshift(X)                        # natural order field in the crystal
     Ek = ph.SimWindow.t2k(Xn) * np.sqrt(si["n_xtal"])            # refract to air (n_air = 1)
-    Ek = Ek * math.sqrt(1 - cfg.oc_transmission)                 # intracavity OC copy
+    R, _, _ = b["oc"].masks(sw)
+    assert R.max() > 0.9 and R.min() < 1e-10        # signal band fed back, the rest dumped
+    Ek = Ek * np.sqrt(R)                                         # intracavity OC copy
     Ek = Ek * np.exp(-1j * cfg.delay * sw.rel_omegas)            # applyGD
     Ek = (Ek + ph.SimWindow.t2k(b["pump"])) * np.sqrt(1 / si["n_xtal"])   # pump, refract in
     lit = ph.SimWindow.fft_order(Ek)
