@@ -28,7 +28,7 @@ struct InstDesc {
   int n, n1, n2, lg1, lg2;
   int nsteps;      // NSteps (number of minimal steps h)
   int n_cols;      // ApFT columns written (0 = none)
-  int pad0;
+  int method;      // METHOD_HEUN or METHOD_RK23
   double w0;       // reference angular frequency [rad/s]
   double bdiffw0;  // beta0 - beta1*w0 [1/m]
   double h;        // minimal step [m]
@@ -72,6 +72,7 @@ struct ClusterDesc {
   unsigned* bar;   // barrier counter (zeroed before every launch)
   int* abort_flag; // set when a barrier times out
   int* xcc;        // C ints: hardware XCC id of each member (placement diagnostic)
+  void* kx;        // RK23 only: [C][2 sides][3 columns][n2] complex, boundary columns of K4
 };
 
 struct LaunchArgs {

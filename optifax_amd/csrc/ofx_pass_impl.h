@@ -404,7 +404,7 @@ struct Geo {
   int n, n1, n2, lg1, lg2, lgn;
   int Cn, lgC, rank, W, lgW, Rr, lgR, HW, LC, sT, sF, c0, k20;
   int coh;  // exchange accesses: 1 device-scope atomics (always used on the GPU)
-  cpx<R>*B0, *B1, *B2;
+  cpx<R>*B0, *B1, *B2, *B3, *B4;  // B3, B4: RK23 only
 };
 
 template <typename R>
@@ -535,6 +535,218 @@ OFX_DEV void t2_to_cols_inv(const Geo<R>& g, const cpx<R>* t2, const cpx<R>* tw,
   OFX_SYNC();
 }
 
+
+// ------------------------------------------------------------------------
+// One adaptive RK23 pass: the Bogacki-Shampine 3(2) step and the step
+// control of ode23/OPO_CUDA_ODE23.cu:375-557 (restated in the oracle's
+// orc_rk23), with FSAL (K1 <- K4 of the accepted step, computed before the
+// absorber and the dispersion).  Buffers (halo HW = 4 columns):
+//   B0 A0 (all columns), B1 E, B2 K2, B3 K3, B4 K1/K4.
+// Validity per attempt: K1 [1, LC-1), K2 [2, LC-2), K3 and E [3, LC-3), K4
+// on the own columns.  K4's three boundary columns on each side go through
+// cd.kx at the first barrier and fill K1's halo for the next step.
+// Returns ST_RUNNING or a terminal status; counts attempts / accepted steps.
+// ------------------------------------------------------------------------
+template <typename R, int MP>
+OFX_DEV int rk23_pass(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc& d, Geo<R>& g,
+                      int rank, double* red, int* sflag, unsigned& nbar, const double* breg,
+                      const R* areg, long long& att, long long& acc) {
+  typedef cpx<R> C;
+  const int n = g.n, n1 = g.n1, n2 = g.n2;
+  const C* tw = (const C*)a.twiddle;
+  const int nmax = a.nmax;
+  C* t1 = (C*)cd.t1;
+  C* t2 = (C*)cd.t2;
+  C* kx = (C*)cd.kx;
+  const C* wt = (const C*)d.wt;
+  const R* ab = (const R*)d.absorb;
+  const R cc = (R)(0.5 / d.dt / d.w0);
+  C* A0 = g.B0;
+  C* Eb = g.B1;
+  C* K2 = g.B2;
+  C* K3 = g.B3;
+  C* K1 = g.B4;
+  const int HW = g.HW, LC = g.LC, W = g.W;
+  const int Ns = d.nsteps;
+  const int sel = (int)d.sel;
+  const int nchunks = Ns / sel;
+  const double A1c = 1.0 / 2.0, A2c = 3.0 / 4.0;
+  const R a1 = (R)A1c, a2 = (R)A2c, a3 = (R)(4.0 / 9.0), a4 = (R)(1.0 / 3.0),
+          a5 = (R)(2.0 / 9.0);
+  const R b1 = (R)(1.0 / 8.0), b2 = (R)(1.0 / 3.0), b3 = (R)(1.0 / 4.0), b4 = (R)(7.0 / 24.0);
+  // K of the nonlinear term of (A0 + cin*kin) on columns [lo, hi)
+  auto kstage = [&](C* out, const C* kin, R cin, int lo, int hi, double bdwz, R gk) {
+    const C eb = expi(-bdwz, R(0));
+    for (int idx = OFX_TID; idx < (hi - lo) * n2; idx += OFX_NT) {
+      const int lc = lo + (idx >> g.lg2), r = idx & (n2 - 1);
+      const int p = lc * g.sT + r;
+      C sc = A0[p], su = A0[p + g.sT], sd = A0[p - g.sT];
+      if (kin) {
+        sc = cadd(sc, cscale(kin[p], cin));
+        su = cadd(su, cscale(kin[p + g.sT], cin));
+        sd = cadd(sd, cscale(kin[p - g.sT], cin));
+      }
+      const C nc = nl_term_e<R>(sc, cmul(wt[gidx(g, lc, r)], eb));
+      const C nu = nl_term_e<R>(su, cmul(wt[gidx(g, lc + 1, r)], eb));
+      const C nd = nl_term_e<R>(sd, cmul(wt[gidx(g, lc - 1, r)], eb));
+      out[p] = k_term<R>(nc, nu, nd, gk, cc);
+    }
+    OFX_SYNC();
+  };
+  double k = 0.0, stepprev = 4.0, stepmod = 4.0;
+  kstage(K1, nullptr, R(0), 1, LC - 1, d.bdiffw0 * k * d.h, (R)(stepmod * d.h * d.G[0]));
+  for (int chunk = 0; chunk < nchunks; chunk++) {
+    stepmod = stepprev;
+    const double cend = (double)(chunk + 1) * (double)sel;
+    while (k < cend) {
+      if ((int)k >= Ns) return ST_BAD_ARG;
+      const double Gk = d.G[(int)k];
+      const R gk = (R)(stepmod * d.h * Gk);
+      // K2 from A1 = A0 + K1/2
+      kstage(K2, K1, a1, 2, LC - 2, d.bdiffw0 * (k + A1c * stepmod) * d.h, gk);
+      // K3 from A0 + 3/4 K2, then E = A0 + (4/9 K3 + 1/3 K2) + 2/9 K1
+      {
+        const C eb = expi(-(d.bdiffw0 * (k + A2c * stepmod) * d.h), R(0));
+        for (int idx = OFX_TID; idx < (LC - 6) * n2; idx += OFX_NT) {
+          const int lc = 3 + (idx >> g.lg2), r = idx & (n2 - 1);
+          const int p = lc * g.sT + r;
+          const C sc = cadd(A0[p], cscale(K2[p], a2));
+          const C su = cadd(A0[p + g.sT], cscale(K2[p + g.sT], a2));
+          const C sd = cadd(A0[p - g.sT], cscale(K2[p - g.sT], a2));
+          const C nc = nl_term_e<R>(sc, cmul(wt[gidx(g, lc, r)], eb));
+          const C nu = nl_term_e<R>(su, cmul(wt[gidx(g, lc + 1, r)], eb));
+          const C nd = nl_term_e<R>(sd, cmul(wt[gidx(g, lc - 1, r)], eb));
+          const C k3 = k_term<R>(nc, nu, nd, gk, cc);
+          K3[p] = k3;
+          Eb[p] = cadd(cadd(A0[p], cadd(cscale(k3, a3), cscale(K2[p], a4))), cscale(K1[p], a5));
+        }
+        OFX_SYNC();
+      }
+      // own columns: K4 from NL(E), the 2nd-order solution, the error quotient;
+      // K4 replaces K1 (FSAL), the absorbed E goes to K2 for the transforms
+      double lmax = -1.0;
+      int bad = 0;
+      {
+        const C eb = expi(-(d.bdiffw0 * (k + stepmod) * d.h), R(0));
+        for (int idx = OFX_TID; idx < W * n2; idx += OFX_NT) {
+          const int lc = HW + (idx >> g.lg2), r = idx & (n2 - 1);
+          const int p = lc * g.sT + r;
+          const C e = Eb[p];
+          const C nc = nl_term_e<R>(e, cmul(wt[gidx(g, lc, r)], eb));
+          const C nu = nl_term_e<R>(Eb[p + g.sT], cmul(wt[gidx(g, lc + 1, r)], eb));
+          const C nd = nl_term_e<R>(Eb[p - g.sT], cmul(wt[gidx(g, lc - 1, r)], eb));
+          const C k4 = k_term<R>(nc, nu, nd, gk, cc);
+          const C k1 = K1[p], k2 = K2[p], k3 = K3[p];
+          const C a1s = cadd(cadd(cadd(A0[p], cadd(cscale(k4, b1), cscale(k3, b2))), cscale(k2, b3)),
+                             cscale(k1, b4));
+          const double ex = (double)e.x, ey = (double)e.y;
+          const double aE = sqrt(ex * ex + ey * ey);
+          if (aE > 0.0) {
+            const double dx = (double)a1s.x - ex, dy = (double)a1s.y - ey;
+            const double q = 0.5 * sqrt(dx * dx + dy * dy) / (aE + 1.0);
+            if (q > lmax) lmax = q;
+          }
+          if (isnan(ex) || isnan(ey)) bad = 1;
+          K1[p] = k4;
+          const R f = ab[gidx(g, lc, r)];
+          K2[p] = f != R(1) ? cscale(e, f) : e;
+        }
+        OFX_SYNC();
+      }
+      const int par = (int)(nbar & 1u);
+      publish_slots(cd, par, rank, 0, lmax, (double)bad, 0.0, g.coh);
+      for (int idx = OFX_TID; idx < 6 * n2; idx += OFX_NT) {
+        const int sc3 = idx >> g.lg2, r = idx & (n2 - 1);
+        const int side = sc3 / 3, col = sc3 - 3 * side;
+        const int lc = side == 0 ? HW + col : HW + W - 3 + col;
+        xst(&kx[((size_t)(rank * 2 + side) * 3 + col) * n2 + r], K1[lc * g.sT + r], g.coh);
+      }
+      fwd_cols_to_t1<R>(g, K2, K3, t1, tw, nmax);
+      nbar++;
+      if (!cluster_barrier(cd, nbar * (unsigned)g.Cn, a.timeout_ticks, sflag, g.coh))
+        return ST_BARRIER_TIMEOUT;
+      double gr[3];
+      gather_err_and_rows<R>(cd, par, g, t1, gr, red);
+      att++;
+      if (gr[1] > 0.0) return ST_NONFINITE;
+      const double pmax = gr[0] < 0.0 ? -1.0 : 4.0 * 100.0 * gr[0];
+      const bool flag0 = pmax >= d.max_err;
+      const bool flag1 = !flag0 && pmax >= d.min_err;
+      if (flag0 && stepmod > 1.0 && (k - stepmod) > 0.0) {
+        // reject: back by the failed step, one minimal step smaller, K1 from A0
+        k = k - stepmod;
+        stepmod = stepmod - 1.0;
+        stepprev = stepmod;
+        kstage(K1, nullptr, R(0), 1, LC - 1, d.bdiffw0 * (k + stepmod) * d.h,
+               (R)(stepmod * d.h * d.G[(int)k]));
+        k = k + stepmod;
+        continue;
+      }
+      // ---------------- accept ----------------
+      acc++;
+      if (rank == 0 && d.stepmods && (int)k < Ns) d.stepmods[(int)k] = (float)stepmod;
+      // K1 halo <- neighbours' K4 boundary columns (row shift across the wrap)
+      {
+        const int left = (rank + g.Cn - 1) & (g.Cn - 1), right = (rank + 1) & (g.Cn - 1);
+        for (int idx = OFX_TID; idx < 6 * n2; idx += OFX_NT) {
+          const int sc3 = idx >> g.lg2, r = idx & (n2 - 1);
+          const int side = sc3 / 3, col = sc3 - 3 * side;
+          if (side == 0) {
+            const int gc = g.c0 - 3 + col;
+            const int rr = gc < 0 ? ((r - 1) & (n2 - 1)) : r;
+            K1[(1 + col) * g.sT + r] = xld(&kx[((size_t)(left * 2 + 1) * 3 + col) * n2 + rr], g.coh);
+          } else {
+            const int gc = g.c0 + W + col;
+            const int rr = gc >= n1 ? ((r + 1) & (n2 - 1)) : r;
+            K1[(HW + W + col) * g.sT + r] =
+                xld(&kx[((size_t)(right * 2 + 0) * 3 + col) * n2 + rr], g.coh);
+          }
+        }
+      }
+      // frequency phase: rows, dispersion (step just taken), snapshot, inverse rows
+      const double s_used = stepmod;
+      C* F = lds_fft<R, -1>(g.B1, g.B2, g.lg1, g.Rr, g.sF, tw, nmax >> g.lg1);
+      C* Fo = (F == g.B1) ? g.B2 : g.B1;
+      auto disperse = [&](int idx, double beta, R al) {
+        const int k2l = idx >> g.lg1, k1i = idx & (n1 - 1);
+        const int kk = (g.k20 + k2l) + n2 * k1i;
+        C v = F[k2l * g.sF + k1i];
+        const C ph = expi(-beta * s_used, R(0));
+        const R am = al == R(1) ? R(1) : (al == R(0) ? R(0) : (R)pow(al, (R)s_used));
+        v = cscale(cmul(v, ph), am);
+        F[k2l * g.sF + k1i] = v;
+        if (d.apft && chunk < d.n_cols) ((C*)d.apft)[(size_t)chunk * n + kk] = v;
+      };
+#pragma unroll
+      for (int q = 0; q < MP; q++) {
+        const int idx = OFX_TID + q * OFX_NT;
+        if (idx < g.Rr * n1) disperse(idx, breg[q], areg[q]);
+      }
+      for (int idx = OFX_TID + MP * OFX_NT; idx < g.Rr * n1; idx += OFX_NT) {
+        const int kk = (g.k20 + (idx >> g.lg1)) + n2 * (idx & (n1 - 1));
+        disperse(idx, d.beta[kk], (R)d.alpha[kk]);
+      }
+      OFX_SYNC();
+      rows_inv_to_t2<R>(g, F, Fo, t2, tw, nmax);
+      nbar++;
+      if (!cluster_barrier(cd, nbar * (unsigned)g.Cn, a.timeout_ticks, sflag, g.coh))
+        return ST_BARRIER_TIMEOUT;
+      t2_to_cols_inv<R>(g, t2, tw, nmax);
+      // step control after an accepted step (OPO_CUDA_ODE23.cu order)
+      if (k + stepmod >= cend && k < cend - 1.0) {
+        const double stepend = (cend - k) - 1.0;
+        stepprev = stepmod;
+        stepmod = stepend;
+      } else {
+        if (!flag1 && !flag0 && (k + stepmod + stepmod) < cend - 1.0) stepmod = stepmod * 2.0;
+        stepprev = stepmod;
+      }
+      k = k + stepmod;
+    }
+  }
+  return ST_RUNNING;
+}
+
 // ------------------------------------------------------------------------
 // One instance: single pass (trips == 0) or round trips.
 // ------------------------------------------------------------------------
@@ -571,6 +783,8 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
     g.B0 = lds;
     g.B1 = lds + bs;
     g.B2 = lds + 2 * bs;
+    g.B3 = lds + 3 * bs;  // RK23 only (the launch sizes LDS for 5 buffers then)
+    g.B4 = lds + 4 * bs;
   }
   const int n = g.n, n1 = g.n1, n2 = g.n2;
   const C* tw = (const C*)a.twiddle;
@@ -620,6 +834,13 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
     long long trip_att = 0, trip_acc = 0;
     double k = 1.0, stepmod = 4.0;
     int chunk = 1;
+    if (d.method == METHOD_RK23) {
+      status = rk23_pass<R, MP>(a, cd, d, g, rank, red, sflag, nbar, breg, areg, trip_att,
+                                trip_acc);
+      attempts += trip_att;
+      accepted += trip_acc;
+      chunk = (int)nChunks + 1;  // skip the Heun loop
+    }
     // ---------------- adaptive Heun pass (NEE_CPU.m:39-94) ----------------
     while ((double)chunk <= nChunks) {
       const double Gk = (double)d.G[(int)k - 1];

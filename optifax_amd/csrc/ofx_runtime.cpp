@@ -92,10 +92,21 @@ int ilog2i(long long n) {
   return ((1LL << l) == n) ? l : -1;
 }
 
-size_t lds_bytes_for(int n1, int n2, int C, int halo, size_t csz) {
-  const int W = n1 / C, Rr = n2 / C, LC = W + 2 * halo;
+// Heun keeps A0, E and K in LDS with 2 halo columns; RK23 keeps A0, E, K1,
+// K2, K3 with 4 (three stages deep, see rk23_pass).
+int halo_of(int method) { return method == OFX_METHOD_RK23 ? 4 : 2; }
+int nbuf_of(int method) { return method == OFX_METHOD_RK23 ? 5 : 3; }
+
+size_t lds_bytes_for(int n1, int n2, int C, int method, size_t csz) {
+  const int W = n1 / C, Rr = n2 / C, LC = W + 2 * halo_of(method);
   const size_t b1 = (size_t)LC * (size_t)(n2 + 1), b2 = (size_t)Rr * (size_t)(n1 + 1);
-  return 3 * std::max(b1, b2) * csz;
+  return (size_t)nbuf_of(method) * std::max(b1, b2) * csz;
+}
+
+// largest cluster size the method allows (RK23 exchanges 3 boundary columns
+// of K4 with each neighbour, so each workgroup owns at least 4 columns)
+int max_cluster(int method, int n1, int n2) {
+  return method == OFX_METHOD_RK23 ? std::min(n1 / 4, n2) : std::min(n1, n2);
 }
 
 int dev_alloc(ofx_opo* r, void** p, size_t bytes) {
@@ -192,8 +203,10 @@ int check_pass(ofx_context* ctx, const ofx_pass_params* p) {
   if (!isfinite(p->bdiffw0) || !isfinite(p->max_err) || !isfinite(p->min_err))
     return fail(ctx, OFX_E_ARG, "bdiffw0, maxErr, minErr must be finite");
   if (p->n_cols < 0) return fail(ctx, OFX_E_ARG, "n_cols must be >= 0");
-  if (p->method != OFX_METHOD_HEUN)
+  if (p->method != OFX_METHOD_HEUN && p->method != OFX_METHOD_RK23)
     return fail(ctx, OFX_E_UNSUPPORTED, "method %d is not available on the device path", p->method);
+  if (p->method == OFX_METHOD_RK23 && (int)p->sel > p->nsteps)
+    return fail(ctx, OFX_E_ARG, "RK23: sel=%g exceeds NSteps=%d (no chunk)", p->sel, p->nsteps);
   return OFX_OK;
 }
 
@@ -266,7 +279,6 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
   r->n_inst = n_inst;
   r->trip_cap = trip_cap;
   const size_t rsz = precision == OFX_F64 ? 8 : 4;
-  const int halo = 2;
   std::vector<std::vector<char>> keep;
   int st = OFX_OK;
 
@@ -285,9 +297,11 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
       return fail(ctx, OFX_E_ARG, "n1=%d does not split N=%d into two powers of two >= 4", n1, n);
     }
     const int n2 = n / n1;
+    const int meth = specs[i].pass.method;
+    const int cmax = max_cluster(meth, n1, n2);
     int cfit = 0;
-    for (int C = 1; C <= std::min(n1, n2); C <<= 1) {
-      if ((int)lds_bytes_for(n1, n2, C, halo, r->csz) <= ctx->info.lds_max - 1024) {
+    for (int C = 1; C <= cmax; C <<= 1) {
+      if ((int)lds_bytes_for(n1, n2, C, meth, r->csz) <= ctx->info.lds_max - 1024) {
         cfit = C;
         break;
       }
@@ -301,8 +315,8 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
     const int want_C = specs[i].cluster ? specs[i].cluster : (tune ? tune->cluster : 0);
     if (want_C) {
       C = want_C;
-      if (ilog2i(C) < 0 || C > std::min(n1, n2) ||
-          (int)lds_bytes_for(n1, n2, C, halo, r->csz) > ctx->info.lds_max - 1024) {
+      if (ilog2i(C) < 0 || C > cmax ||
+          (int)lds_bytes_for(n1, n2, C, meth, r->csz) > ctx->info.lds_max - 1024) {
         delete r;
         return fail(ctx, OFX_E_ARG, "cluster size %d is invalid for N=%d (n1=%d)", C, n, n1);
       }
@@ -311,7 +325,7 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
       // the per-workgroup share until it falls to ~512 points (measured on
       // N = 2^15: 33 / 27 / 24 us per step at 16 / 32 / 64 workgroups).
       const int slots_est = ctx->info.cus;
-      while (C * 2 <= std::min(n1, n2) && (long long)n_inst * C * 2 <= slots_est &&
+      while (C * 2 <= cmax && (long long)n_inst * C * 2 <= slots_est &&
              C * 2 <= 64 && n / (C * 2) >= 512)
         C *= 2;
     }
@@ -327,7 +341,7 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
   size_t lds = 0;
   for (int i = 0; i < n_inst; i++) {
     const int n1 = r->inst_n1[i], n2 = specs[i].pass.n / n1;
-    lds = std::max(lds, lds_bytes_for(n1, n2, r->inst_C[i], halo, r->csz));
+    lds = std::max(lds, lds_bytes_for(n1, n2, r->inst_C[i], specs[i].pass.method, r->csz));
   }
   r->lds = lds;
   const int bpc = be_blocks_per_cu(ctx->be, precision, r->block, lds);
@@ -337,14 +351,15 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
   }
   const int slots = ctx->info.cus * bpc;
 
-  // ---- clusters: group instances by (C, n, n1); LPT over cost ----
-  std::map<std::tuple<int, int, int>, std::vector<int>> groups;
+  // ---- clusters: group instances by (C, n, n1, method); LPT over cost ----
+  std::map<std::tuple<int, int, int, int>, std::vector<int>> groups;
   for (int i = 0; i < n_inst; i++)
-    groups[std::make_tuple(r->inst_C[i], specs[i].pass.n, r->inst_n1[i])].push_back(i);
+    groups[std::make_tuple(r->inst_C[i], specs[i].pass.n, r->inst_n1[i], specs[i].pass.method)]
+        .push_back(i);
   long long want = 0;
   for (auto& g : groups) want += (long long)std::get<0>(g.first) * (long long)g.second.size();
   std::vector<std::vector<int>> cl_members;
-  std::vector<int> cl_C;
+  std::vector<int> cl_C, cl_method;
   for (auto& g : groups) {
     const int C = std::get<0>(g.first);
     std::vector<int> ids = g.second;
@@ -368,6 +383,7 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
       if (m.empty()) continue;
       cl_members.push_back(m);
       cl_C.push_back(C);
+      cl_method.push_back(std::get<3>(g.first));
     }
   }
   long long used = 0;
@@ -489,6 +505,7 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
     d.lg2 = ilog2i(d.n2);
     d.nsteps = p.nsteps;
     d.n_cols = p.n_cols;
+    d.method = p.method;
     d.w0 = p.w0;
     d.bdiffw0 = p.bdiffw0;
     d.h = p.h;
@@ -525,11 +542,12 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
     cd.C = cl_C[c];
     cd.first_inst = (int)r->order_h.size();
     cd.n_inst = (int)cl_members[c].size();
-    cd.halo = halo;
-    int nc = 0;
+    cd.halo = halo_of(cl_method[c]);
+    int nc = 0, n2c = 0;
     for (int id : cl_members[c]) {
       r->order_h.push_back(id);
       nc = std::max(nc, specs[id].pass.n);
+      n2c = std::max(n2c, specs[id].pass.n / r->inst_n1[id]);
     }
     st = dev_alloc(r, &cd.t1, (size_t)nc * r->csz);
     if (!st) st = dev_alloc(r, &cd.t2, (size_t)nc * r->csz);
@@ -537,6 +555,8 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
     if (!st) st = dev_alloc(r, (void**)&cd.bar, 64);
     if (!st) st = dev_alloc(r, (void**)&cd.abort_flag, 64);
     if (!st) st = dev_alloc(r, (void**)&cd.xcc, sizeof(int) * (size_t)cd.C);
+    if (!st && cl_method[c] == OFX_METHOD_RK23)
+      st = dev_alloc(r, &cd.kx, (size_t)cd.C * 6 * (size_t)n2c * r->csz);
     if (st) break;
     r->zero_ptrs.push_back(cd.bar);
     r->zero_words.push_back(1);

@@ -220,6 +220,37 @@ def energy_of(simwin: SimWindow, E, area, n_medium=1.0):
     return float(np.sum(np.abs(Ek) ** 2 * (C0 * n_medium * EPS0 / 2) * area * simwin.dt / simwin.n))
 
 
+def qpm_signal_wavelength(lambda_p, period, temperature=20.0, lo=1.2e-6, hi=2.0e-6):
+    """Signal wavelength of first-order QPM, k_p - k_s - k_i = 2 pi / period,
+    with the Gayer extraordinary index (bisection in [lo, hi])."""
+    def dk(ls):
+        li = 1.0 / (1.0 / lambda_p - 1.0 / ls)
+        n = gayer_n(np.array([lambda_p, ls, li]) * 1e6, temperature, 1)
+        return 2 * np.pi * (n[0] / lambda_p - n[1] / ls - n[2] / li - 1.0 / period)
+    a, b = lo, hi
+    fa = dk(a)
+    if fa * dk(b) > 0:
+        raise ValueError("no QPM signal wavelength in the search interval")
+    for _ in range(80):
+        m = 0.5 * (a + b)
+        fm = dk(m)
+        if fa * fm <= 0:
+            b = m
+        else:
+            a, fa = m, fm
+    return 0.5 * (a + b)
+
+
+def sync_delay(simwin, length, temperature, lambda_s):
+    """Cavity delay that cancels the signal's group delay relative to the
+    crystal frame (which moves with the pump at the reference wavelength):
+    -GD_rel(lambda_s) of one pass (phi_calc + phi2GD)."""
+    phi_rel, _ = phi_calc(length, temperature, simwin)
+    gd = phi2gd(phi_rel, simwin.d_omega)
+    k = int(np.argmin(np.abs(simwin.wavelengths - lambda_s)))
+    return -float(gd[k])
+
+
 def smoothed_top_hat(x, a, b, w):
     """0.5 (tanh((x - a)/w) - tanh((x - b)/w)) (smoothedTopHat.m, tanh form)."""
     return 0.5 * (np.tanh((x - a) / w) - np.tanh((x - b) / w))
@@ -229,14 +260,17 @@ def smoothed_top_hat(x, a, b, w):
 class OutputCoupler:
     """Resonator feedback and output coupling.
 
-    The cavity keeps R(lambda) of the power and couples out 1 - R:
-      R = (1 - transmission) * smoothed_top_hat(lambda, band, edge)
-    i.e. a singly resonant cavity: only the signal band is fed back; the
-    depleted pump and the idler leave every trip.  This is the reference's
-    OPO setup (toolbox/scripts/simulation_script_template.m:44-48: OC coating
-    0.5*smoothedTopHat(lam, 1.2e-6, 1.8e-6, 30e-9)) with the coupler's
-    transmission as a parameter.  band=None gives a flat coupler (everything
-    fed back), the "R" regime of Optic.m / OpticalPulse.m:163-188."""
+    With top = smoothed_top_hat(lambda, band, edge), the cavity keeps
+      R = (1 - transmission) * top
+    of the power, and the output port carries
+      T = transmission * top;
+    the remaining 1 - top (the depleted pump and the idler) is dumped at the
+    band filter every trip.  This is a singly resonant cavity as in the
+    reference's OPO setup (toolbox/scripts/simulation_script_template.m:44-48:
+    OC coating 0.5*smoothedTopHat(lam, 1.2e-6, 1.8e-6, 30e-9), pump filter
+    smoothedTopHat(lam, 0.9e-6, 5.5e-6, 30e-9)), with the coupler's
+    transmission as a parameter.  band=None gives a flat coupler: R = 1 - t,
+    T = t everywhere (the "R" regime of Optic.m / OpticalPulse.m:163-188)."""
     transmission: float = 0.05
     phase: float = 0.0
     band: tuple | None = (1.2e-6, 1.8e-6)
@@ -247,9 +281,11 @@ class OutputCoupler:
         if self.band is None:
             R = np.full(simwin.n, 1.0 - self.transmission)
         else:
-            R = (1.0 - self.transmission) * smoothed_top_hat(lam, self.band[0], self.band[1],
-                                                            self.edge)
-            R = np.where(simwin.omegas > 0, R, 0.0)
+            top = smoothed_top_hat(lam, self.band[0], self.band[1], self.edge)
+            top = np.where(simwin.omegas > 0, top, 0.0)
+            R = (1.0 - self.transmission) * top
+            T = self.transmission * top
+            return R, T, np.full(simwin.n, self.phase)
         T = 1.0 - R
         ph = np.full(simwin.n, self.phase)
         return R, T, ph
@@ -298,6 +334,9 @@ class OPOConfig:
     oc_transmission: float = 0.05
     oc_band: tuple | None = (1.2e-6, 1.8e-6)     # fed-back (signal) band; None = flat
     delay: float = 0.3e-12
+    # "absolute": delay is the per-trip cavity delay; "sync": delay is the
+    # detuning from the synchronous delay of the QPM signal (sync_delay)
+    delay_mode: str = "absolute"
     step: float = 1e-7
     error_bounds: tuple = (2.5e-3, 5e-2)   # OpticalSim.m:69 [min, max] percent
     progress_plots: int = 5                 # OpticalSim.m:23
@@ -315,5 +354,11 @@ class OPOConfig:
         sel = math.ceil(si["nSteps"] / (self.progress_plots - 1))   # OpticalSim.m:214
         si.update(maxErr=self.error_bounds[1], minErr=self.error_bounds[0], sel=float(sel),
                   n_cols=self.progress_plots - 1)
+        delay = self.delay
+        if self.delay_mode == "sync":
+            ls = qpm_signal_wavelength(self.lambda_ref, self.period, self.temperature)
+            delay = sync_delay(sw, self.length, self.temperature, ls) + self.delay
+        elif self.delay_mode != "absolute":
+            raise ValueError("delay_mode must be 'absolute' or 'sync'")
         return dict(simwin=sw, crystal=xt, solver=si, pump=pump, noise=noise, ops=ops,
-                    delay=self.delay, oc=oc)
+                    delay=delay, oc=oc)

@@ -86,6 +86,14 @@ def _tuning(tune):
     return ctypes.byref(t)
 
 
+def _method(m):
+    if m in (0, "heun"):
+        return _lib.OFX_METHOD_HEUN
+    if m in (1, "rk23"):
+        return _lib.OFX_METHOD_RK23
+    raise ValueError(f"method must be 'heun' or 'rk23', got {m!r}")
+
+
 def _params(n, nsteps, w0, bdiffw0, h, dt, maxErr, minErr, sel, n_cols, method=0):
     return PassParams(int(n), int(nsteps), float(w0), float(bdiffw0), float(h), float(dt),
                       float(maxErr), float(minErr), float(sel), int(n_cols), int(method))
@@ -93,7 +101,7 @@ def _params(n, nsteps, w0, bdiffw0, h, dt, maxErr, minErr, sel, n_cols, method=0
 
 def OPOmexBatch(Et, alpha_w, G, w0, bdiffw0, h, NSteps, dt, beta_op, maxErr, minErr, sel, ApFT,
                 stepMods, *, ctx: Context | None = None, precision: str = "single", tune=None,
-                return_status: bool = False):
+                return_status: bool = False, method: str = "heun"):
     """[Et, ApFT, stepMods] = OPOmexBatch(Et, alpha_w, G, w0, bdiffw0, h, NSteps, dt,
     beta_op, maxErr, minErr, sel, ApFT, stepMods).
 
@@ -101,7 +109,9 @@ def OPOmexBatch(Et, alpha_w, G, w0, bdiffw0, h, NSteps, dt, beta_op, maxErr, min
     G: (NSteps,) shared or (NSteps, B) per instance (OPO_CUDA_BATCH_XTAL.cu:222).
     ApFT: (N, P-1) or (N, P-1, B) complex; stepMods: (NSteps,) or (1, NSteps).
     Inputs are not modified; new arrays are returned (OPO_mex_BATCH.cpp:42-56,120-122).
-    precision "single" (the MEX contract) or "double" (FP64 fields)."""
+    precision "single" (the MEX contract) or "double" (FP64 fields).
+    method "heun" (OPOmexBatch / NEE_CPU.m) or "rk23" (the ode23 variant,
+    ode23/OPO_CUDA_ODE23.cu)."""
     ctx = ctx or default_context()
     f64 = precision == "double"
     rdt = np.float64 if f64 else np.float32
@@ -137,7 +147,7 @@ def OPOmexBatch(Et, alpha_w, G, w0, bdiffw0, h, NSteps, dt, beta_op, maxErr, min
         if sm_in.size != nsteps:
             raise ValueError("stepMods must have NSteps elements")
         smf = np.ascontiguousarray(np.tile(sm_in, (B, 1)))
-    p = _params(N, nsteps, w0, bdiffw0, h, dt, maxErr, minErr, sel, n_cols)
+    p = _params(N, nsteps, w0, bdiffw0, h, dt, maxErr, minErr, sel, n_cols, _method(method))
     status = np.zeros(B, np.int32)
     ptr = (lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) if f64 else \
         (lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
@@ -196,7 +206,8 @@ class OPOBatch:
             s = inst["solver"]
             n = int(np.asarray(s["beta_op"]).size)
             p = _params(n, s["nSteps"], s["w0"], s["bdiffw0"], s["h"], s["dt"], s["maxErr"],
-                        s["minErr"], s["sel"], s.get("n_cols", 0))
+                        s["minErr"], s["sel"], s.get("n_cols", 0),
+                        _method(inst.get("method", s.get("method", "heun"))))
             specs[i].pass_ = p
             specs[i].delay = float(inst.get("delay", 0.0))
             specs[i].beta = self._d(s["beta_op"])

@@ -121,3 +121,54 @@ def test_argument_errors(emu_ctx):
     with pytest.raises(OfxError):
         OPOmexBatch(E[:100], args[0][:100], args[1], *args[2:7], args[7][:100], *args[8:], None,
                     None, ctx=emu_ctx)      # N not a power of two
+
+
+@pytest.mark.parametrize("n,C,window", [(256, 1, 2e-12), (256, 2, 2e-12), (1024, 4, 4e-12),
+                                        (1024, 8, 4e-12)])
+def test_rk23_pass_fp64_matches_oracle(emu_ctx, n, C, window):
+    """RK23 (ode23 variant): same accept/reject sequence and FP64 fields as orc_rk23,
+    including the K4 boundary exchange between workgroups (C > 1)."""
+    sw, si, E, args = crystal_case(n=n, window=window)
+    Eo, apo, smo, cnt = oracle.rk23(E, *args, n_cols=4)
+    assert cnt[0] > cnt[1]                          # the case exercises rejections
+    Eg, apg, smg = OPOmexBatch(E, *args, np.zeros((n, 4), complex), np.zeros(args[5]),
+                               ctx=emu_ctx, precision="double", tune={"cluster": C},
+                               method="rk23")
+    assert np.array_equal(smg, smo)
+    assert rel_l2(Eg, Eo) < 1e-10
+    assert rel_l2(apg, apo) < 1e-10
+
+
+def test_rk23_round_trips_match_literal_opticalsim(emu_ctx):
+    cfg, b = opo_case()
+    si, ops, sw = b["solver"], b["ops"], b["simwin"]
+    trips = 2
+    R, T, phi = b["oc"].masks(sw)
+    o = oracle.opo_run(trips, b["noise"], b["pump"], np.ones(sw.n), si["n_xtal"], R, T, phi,
+                       cfg.delay, sw.rel_omegas, si, method=1)
+    run = OPOBatch([dict(solver=si, field0=ops["field0"], pump=ops["pump"], cav=ops["cav"],
+                         ocm=ops["ocm"], delay=cfg.delay, method="rk23")], precision="double",
+                   trip_cap=trips, ctx=emu_ctx, tune={"cluster": 2})
+    run.launch(trips)
+    run.sync()
+    r = run.read(0)
+    assert np.array_equal(r["trip_log"][:, 2:].astype(int), o["steps"])
+    assert rel_l2(r["field"], oracle_next_input(b, o["E_cav"])) < 1e-10
+    run.close()
+
+
+def test_rk23_mixed_with_heun_in_one_launch(emu_ctx):
+    """Heun and RK23 instances share one launch (separate clusters, own halos)."""
+    sw, si, E, args = crystal_case(n=256, window=2e-12)
+    inst = lambda m: dict(solver=dict(si, maxErr=args[8], minErr=args[9], sel=args[10]),
+                          field0=E, method=m)
+    run = OPOBatch([inst("heun"), inst("rk23"), inst("heun")], precision="double", ctx=emu_ctx,
+                   tune={"cluster": 2})
+    run.launch(0)
+    run.sync()
+    Eh, _, _, _ = oracle.nee_heun(E, *args)
+    Er, _, _, _ = oracle.rk23(E, *args)
+    assert rel_l2(run.read(0)["field"], Eh) < 1e-10
+    assert rel_l2(run.read(1)["field"], Er) < 1e-10
+    assert rel_l2(run.read(2)["field"], Eh) < 1e-10
+    run.close()

@@ -138,3 +138,58 @@ def test_config2_full_size_invariants(gpu_ctx):
     S32, S64 = np.abs(s["oc_spec"]) ** 2, np.abs(d["oc_spec"]) ** 2
     assert rel_l2(S32, S64) < 1e-2
     assert abs(s["trip_log"][1, 0] - d["trip_log"][1, 0]) < 1e-2
+
+
+# ---------------------------------------------------------------- RK23 (ode23 variant)
+def _run_rk(ctx, E, args, n, precision, C=None, cols=4):
+    return OPOmexBatch(E, *args, np.zeros((n, cols), complex), np.zeros(args[5]), ctx=ctx,
+                       precision=precision, tune={"cluster": C} if C else None, method="rk23")
+
+
+@pytest.mark.parametrize("n,C,window", [(256, 1, 2e-12), (256, 2, 2e-12), (1024, 4, 4e-12),
+                                        (4096, 8, 8e-12), (8192, 16, 10e-12)])
+def test_rk23_pass_fp64_exact(gpu_ctx, n, C, window):
+    sw, si, E, args = crystal_case(n=n, window=window)
+    Eo, apo, smo, cnt = oracle.rk23(E, *args, n_cols=4)
+    assert cnt[0] > cnt[1]
+    Eg, apg, smg = _run_rk(gpu_ctx, E, args, n, "double", C)
+    assert np.array_equal(smg, smo)
+    assert rel_l2(Eg, Eo) < 1e-9
+    assert rel_l2(apg, apo) < 1e-9
+
+
+def test_rk23_config1_grid_fp64_and_fp32(gpu_ctx):
+    """Config-1 grid and pump (2^13 points, 10 ps, 1040 nm, 200 fs sech^2,
+    d_eff 14 pm/V, 5 um^2) over the first 0.2 mm of the 1 mm crystal at 100 pJ
+    (1 nJ in 5 um^2 drives the FP64 oracle itself to NaN)."""
+    sw, si, E, args = crystal_case(n=8192, window=10e-12, L=0.2e-3, energy=1e-10, area=5e-12,
+                                   period=30e-6)
+    args = list(args)
+    args[8], args[9] = 1e-2, 5e-4
+    Eo, apo, smo, cnt = oracle.rk23(E, *args, n_cols=4)
+    Eg, apg, smg = _run_rk(gpu_ctx, E, args, 8192, "double")
+    assert np.array_equal(smg, smo)
+    assert rel_l2(Eg, Eo) < 1e-9
+    Ef, apf, smf = _run_rk(gpu_ctx, E, args, 8192, "single")
+    print(f"RK23 config-1 grid: FP32 rel L2 {rel_l2(Ef, Eo):.2e}, steps equal "
+          f"{np.array_equal(smf, smo)}")
+    assert rel_l2(Ef, Eo) < 1e-4
+
+
+def test_rk23_round_trips_fp64(gpu_ctx):
+    cfg, b = opo_case()
+    si, ops, sw = b["solver"], b["ops"], b["simwin"]
+    trips = 3
+    R, T, phi = b["oc"].masks(sw)
+    o = oracle.opo_run(trips, b["noise"], b["pump"], np.ones(sw.n), si["n_xtal"], R, T, phi,
+                       cfg.delay, sw.rel_omegas, si, method=1)
+    for C in (1, 4):
+        run = OPOBatch([dict(solver=si, field0=ops["field0"], pump=ops["pump"], cav=ops["cav"],
+                             ocm=ops["ocm"], delay=cfg.delay, method="rk23")],
+                       precision="double", trip_cap=trips, ctx=gpu_ctx, tune={"cluster": C})
+        run.launch(trips)
+        run.sync()
+        r = run.read(0)
+        assert np.array_equal(r["trip_log"][:, 2:].astype(int), o["steps"])
+        assert rel_l2(r["field"], oracle_next_input(b, o["E_cav"])) < 1e-9
+        run.close()
