@@ -135,7 +135,9 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
 void ofx_opo_destroy(ofx_opo* run);
 /* enqueue `trips` round trips (0 = one plain crystal pass) on the context
  * stream; instances stop early once the trip-to-trip relative L2 change of
- * the output-coupled power spectrum is below conv_tol (<= 0: never) */
+ * the output-coupled power spectrum is below conv_tol (<= 0: never).  With
+ * conv_tol > 0, an instance that converged in an earlier launch is skipped
+ * until ofx_opo_write_field gives it a new field. */
 int ofx_opo_launch(ofx_opo* run, int trips, double conv_tol);
 int ofx_opo_sync(ofx_opo* run);
 /* capture `trips` round trips as a hipGraph (once) and replay it `replays` times */
@@ -148,7 +150,8 @@ int ofx_opo_last_ms(ofx_opo* run, double* ms);
 int ofx_opo_read(ofx_opo* run, int inst, double* field, double* oc_spec, double* trip_log,
                  int* status, int* trips_done, long long* counters, double* apft,
                  double* stepmods);
-/* overwrite the crystal-input field of one instance (warm start / checkpoint restore) */
+/* overwrite the crystal-input field of one instance (warm start / checkpoint
+ * restore); resets its status to running */
 int ofx_opo_write_field(ofx_opo* run, int inst, const double* field);
 /* geometry actually used: cluster size, n1, block, grid, LDS bytes */
 int ofx_opo_geometry(const ofx_opo* run, int inst, int* cluster, int* n1, int* block, int* grid,

@@ -57,6 +57,7 @@ struct InstDesc {
   int pad1;
   int* status;          // ST_* per instance
   int* trips_done;      // per instance, cumulative (also the trip-log base)
+  int* fresh;           // per instance: 1 after a field rewrite (no previous spectrum yet)
   long long* counters;  // [0] attempts, [1] accepted (cumulative over the launch)
 };
 

@@ -755,6 +755,8 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
                           int rank, cpx<R>* lds, double* red, int* sflag, unsigned& nbar,
                           int coh) {
   typedef cpx<R> C;
+  // converged in an earlier launch (and not rewritten since): nothing to do
+  if (a.conv_tol > 0.0 && a.trips > 0 && d.status[0] == ST_CONVERGED) return;
   Geo<R> g;
   g.n = d.n;
   g.n1 = d.n1;
@@ -804,6 +806,7 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
   int status = ST_RUNNING;
   long long attempts = 0, accepted = 0;
   const int trip_base = d.trips_done[0];
+  const bool fresh = d.fresh[0] != 0;
 
   // load the crystal-input field with halo columns
   for (int idx = OFX_TID; idx < g.LC * n2; idx += OFX_NT) {
@@ -988,7 +991,7 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
       const C* cav = (const C*)d.cav;
       const C* ocm = (const C*)d.ocm;
       C* ocs = (C*)d.oc_spec;
-      const bool have_prev = (trip_base + trip) > 0;
+      const bool have_prev = (trip_base + trip) > 0 && !(fresh && trip == 0);
       double acc[3] = {0.0, 0.0, 0.0};  // sum (S-Sp)^2, sum S^2, sum S
       for (int idx = OFX_TID; idx < g.Rr * n1; idx += OFX_NT) {
         const int k2l = idx >> g.lg1, k1 = idx & (n1 - 1);
@@ -1047,6 +1050,7 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
   }
   if (rank == 0 && OFX_TID == 0) {
     d.status[0] = status;
+    if (trip > 0) d.fresh[0] = 0;
     d.trips_done[0] += trip;
     d.counters[0] += attempts;
     d.counters[1] += accepted;

@@ -61,6 +61,7 @@ struct ofx_opo {
   int* xcc_d = nullptr;
   int* status_d = nullptr;
   int* trips_d = nullptr;
+  int* fresh_d = nullptr;
   long long* counters_d = nullptr;
   void* twiddle = nullptr;
   int nmax = 0;
@@ -428,6 +429,7 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
   // ---- device buffers ----
   st = dev_alloc(r, (void**)&r->status_d, sizeof(int) * n_inst);
   if (!st) st = dev_alloc(r, (void**)&r->trips_d, sizeof(int) * n_inst);
+  if (!st) st = dev_alloc(r, (void**)&r->fresh_d, sizeof(int) * n_inst);
   if (!st) st = dev_alloc(r, (void**)&r->counters_d, sizeof(long long) * 2 * n_inst);
   if (!st) st = dev_alloc(r, (void**)&r->xcc_d, sizeof(int) * r->grid);
   if (st) { ofx_opo_destroy(r); return st; }
@@ -435,6 +437,7 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
     std::vector<char> z(std::max(sizeof(long long) * 2 * n_inst, sizeof(int) * (size_t)r->grid), 0);
     upload(r, r->status_d, z.data(), sizeof(int) * n_inst);
     upload(r, r->trips_d, z.data(), sizeof(int) * n_inst);
+    upload(r, r->fresh_d, z.data(), sizeof(int) * n_inst);
     upload(r, r->counters_d, z.data(), sizeof(long long) * 2 * n_inst);
     upload(r, r->xcc_d, z.data(), sizeof(int) * r->grid);
     be_sync(ctx->be);
@@ -530,6 +533,7 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
     d.trip_cap = trip_cap;
     d.status = r->status_d + i;
     d.trips_done = r->trips_d + i;
+    d.fresh = r->fresh_d + i;
     d.counters = r->counters_d + 2 * i;
   }
   if (st) { be_sync(ctx->be); ofx_opo_destroy(r); return st; }
@@ -710,6 +714,9 @@ int ofx_opo_write_field(ofx_opo* r, int inst, const double* field) {
   std::vector<std::vector<char>> keep;
   const size_t n = (size_t)r->pass[inst].n;
   int st = upload(r, r->bufs[inst].field, convert(field, 2 * n, r->prec, keep), n * r->csz);
+  const int running = ST_RUNNING, one = 1;
+  if (!st) st = upload(r, r->status_d + inst, &running, sizeof(int));
+  if (!st) st = upload(r, r->fresh_d + inst, &one, sizeof(int));
   if (!st && be_sync(r->ctx->be)) st = fail(r->ctx, OFX_E_HIP, "upload failed");
   return st;
 }

@@ -1,0 +1,204 @@
+"""Surrogate-warm-started OPO steady state.
+
+The cavity is singly resonant (physics.OutputCoupler): only the signal band
+is fed back, the pump is injected fresh and the idler is dumped every trip.
+The state that carries from one trip to the next is therefore the crystal
+input inside the resonated band, and one round trip is
+
+    X_{n+1} = D(tau) . F(X_n + P) + P,    F = band feedback of one crystal pass
+
+with P the pump, D(tau) = exp(-i tau dw) the cavity delay and F the
+crystal pass seen through the resonator band, F(X) = cav * pass(X).  The
+surrogate learns F on 1024 spectral bins of that band, conditioned on pump
+energy and poling period, and replaces it for the first trips of a solve:
+
+    1. surrogate trips: iterate the bin-level map from a weak smooth seed
+       (up to `max_surrogate_trips`, or until the bin-level change between
+       trips drops below `surrogate_tol`); one batched inference per trip;
+    2. handover: the bins become a full-grid field (band-limited
+       interpolation) plus the exact pump;
+    3. judge: the first plain trip's band output is compared with the
+       surrogate's prediction for the same input; above `judge_tol` the
+       surrogate state is discarded and the instance restarts plainly from its
+       noise seed (fallback);
+    4. plain FP32 round trips until the trip-to-trip change of the
+       output-coupled spectrum is below `conv_tol`, within the trip budget.
+
+The output-coupled spectrum is sqrt(T/R) * F in the band (the coupler's
+masks share the band shape), so the convergence test needs no extra pass.
+
+Features (SpectralBins over the band, D = 2 grid points per bin):
+  input  [amplitude(X) 1024, phase(X) 1024, pump energy / 4 nJ, (period - 30 um) / 1 um]
+  output [amplitude(F) 1024, phase(F) 1024]
+with amplitude sqrt(|.|/a_ref), phase unwrap(angle)/pi over bins holding at
+least 1e-3 of the peak amplitude (0 elsewhere), after rotating the input by
+its energy-weighted mean phase (the map commutes with a global signal phase;
+the output is rotated back).
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+
+from . import physics as ph
+from .surrogate import SpectralBins, SurrogateMLP
+from .solver import OPOBatch
+
+NBINS = 1024
+N_IN = 2 * NBINS + 2
+N_OUT = 2 * NBINS
+PUMP_REF = 4e-9
+PERIOD_REF = 30e-6
+
+
+# ---------------------------------------------------------------------------
+# band, features
+# ---------------------------------------------------------------------------
+def band_bins(simwin, f_lo=150e12, a_ref=1.0):
+    """1024 bins, every second grid frequency from f_lo (150-278 THz on a 16 ps
+    window): the resonated band 1.2-1.8 um (167-250 THz) with margin."""
+    return SpectralBins(simwin, f_lo, nbins=NBINS, D=2, a_ref=a_ref)
+
+
+def canonical_phase(Xb):
+    """Energy-weighted mean phase of each row of complex bins."""
+    Xb = np.atleast_2d(Xb)
+    z = np.sum(Xb * np.abs(Xb), axis=-1)
+    return np.where(np.abs(z) > 0, np.angle(z), 0.0)
+
+
+def _amp_phase(Xb, a_ref):
+    amp = np.sqrt(np.abs(Xb) / a_ref)
+    mag = np.abs(Xb)
+    occ = mag >= 1e-3 * np.max(mag, axis=-1, keepdims=True)
+    ph_ = np.where(occ, np.angle(Xb), 0.0)
+    ph_ = np.unwrap(ph_, axis=-1)
+    return amp, np.where(occ, ph_, 0.0) / np.pi
+
+
+def input_features(Xb, pump_energy, period, a_ref):
+    """Rows of complex band bins -> network inputs [B, 2050] and the rotation
+    applied (to undo on the output)."""
+    Xb = np.atleast_2d(Xb)
+    th = canonical_phase(Xb)
+    Xr = Xb * np.exp(-1j * th)[:, None]
+    amp, phs = _amp_phase(Xr, a_ref)
+    B = Xb.shape[0]
+    pe = np.broadcast_to(np.asarray(pump_energy, np.float64) / PUMP_REF, (B,))
+    pr = np.broadcast_to((np.asarray(period, np.float64) - PERIOD_REF) / 1e-6, (B,))
+    x = np.concatenate([amp, phs, pe[:, None], pr[:, None]], axis=1)
+    return x.astype(np.float32), th
+
+
+def output_features(Fb, th, a_ref):
+    """Rows of complex band outputs -> targets [B, 2048] in the rotated frame."""
+    Fr = np.atleast_2d(Fb) * np.exp(-1j * np.asarray(th))[:, None]
+    amp, phs = _amp_phase(Fr, a_ref)
+    return np.concatenate([amp, phs], axis=1).astype(np.float32)
+
+
+def from_output_features(y, th, a_ref):
+    y = np.asarray(y, np.float64)
+    amp, phs = y[:, :NBINS], y[:, NBINS:]
+    return (a_ref * np.maximum(amp, 0.0) ** 2) * np.exp(1j * np.pi * phs) * \
+        np.exp(1j * np.asarray(th))[:, None]
+
+
+# ---------------------------------------------------------------------------
+# instances
+# ---------------------------------------------------------------------------
+def opo_instance(cfg: ph.OPOConfig):
+    """Device instance dict + the pieces the warm start needs."""
+    b = cfg.build()
+    ops = b["ops"]
+    inst = dict(solver=b["solver"], field0=ops["field0"], pump=ops["pump"], cav=ops["cav"],
+                ocm=ops["ocm"], delay=b["delay"])
+    return inst, b
+
+
+def smooth_seed(simwin, cfg: ph.OPOConfig, rel_amp=1e-3, fwhm=300e-15, t0=0.0):
+    """A weak transform-limited Gaussian at the QPM signal wavelength, fftshifted
+    time; rel_amp relative to the pump's peak field."""
+    ls = ph.qpm_signal_wavelength(cfg.lambda_ref, cfg.period, cfg.temperature)
+    fs = ph.C0 / ls - ph.C0 / cfg.lambda_ref
+    t = simwin.times
+    pump = ph.sech_pump(simwin, cfg.pump_fwhm, cfg.pump_energy, cfg.area)
+    env = np.exp(-2 * math.log(2) * ((t - t0) / fwhm) ** 2)
+    return rel_amp * np.abs(pump).max() * env * np.exp(2j * np.pi * fs * t)
+
+
+class BandCavity:
+    """Bin-level cavity of a batch of instances (same SimWindow)."""
+
+    def __init__(self, sb: SpectralBins, insts, builds):
+        self.sb = sb
+        self.cav = np.stack([sb.bins(i["cav"]) for i in insts])
+        self.ocm = np.stack([sb.bins(i["ocm"]) for i in insts])
+        self.pump = np.stack([sb.bins(i["pump"]) for i in insts])
+        dw = 2 * np.pi * (sb.freqs - ph.C0 / builds[0]["simwin"].reference_wave)
+        self.dphase = np.exp(-1j * np.outer([i["delay"] for i in insts], dw))
+        with np.errstate(divide="ignore", invalid="ignore"):
+            r = np.abs(self.ocm) / np.abs(self.cav)
+        self.oc_over_fb = np.where(np.abs(self.cav) > 0, r, 0.0)
+
+    def next_input(self, Fb):
+        return self.dphase * Fb + self.pump
+
+    def band_out(self, Xin_next):
+        """F from a next crystal input: exp(+i tau dw) (X_next - P)."""
+        return (Xin_next - self.pump) / self.dphase
+
+    def oc_power(self, Fb):
+        return (self.oc_over_fb * np.abs(Fb)) ** 2
+
+
+def conv_metric(S, Sp):
+    num = np.sqrt(np.sum((S - Sp) ** 2, axis=-1))
+    den = np.sqrt(np.sum(S ** 2, axis=-1))
+    return np.where(den > 0, num / np.where(den > 0, den, 1.0), np.inf)
+
+
+# ---------------------------------------------------------------------------
+# training data from device trajectories
+# ---------------------------------------------------------------------------
+def generate_pairs(cfgs, trips, seeds_rel=(1e-6, 1e-2), rng_seed=0, log=print):
+    """Run each configuration from a smooth seed of random strength and phase
+    for `trips` plain round trips (one persistent launch per trip, all
+    instances together) and return (inputs X [B*trips, 1024] complex bins,
+    band outputs F, pump energies, periods) for every trip."""
+    rng = np.random.default_rng(rng_seed)
+    insts, builds = [], []
+    for cfg in cfgs:
+        inst, b = opo_instance(cfg)
+        sw = b["simwin"]
+        amp = math.exp(rng.uniform(math.log(seeds_rel[0]), math.log(seeds_rel[1])))
+        seed = smooth_seed(sw, cfg, amp, t0=rng.uniform(-0.3e-12, 0.3e-12)) * \
+            np.exp(1j * rng.uniform(0, 2 * np.pi))
+        Xk = ph.SimWindow.fft_order(ph.SimWindow.t2k(seed)) + inst["pump"]
+        inst["field0"] = np.fft.ifft(Xk)
+        insts.append(inst)
+        builds.append(b)
+    sb = band_bins(builds[0]["simwin"])
+    cav = BandCavity(sb, insts, builds)
+    run = OPOBatch(insts, trip_cap=trips)
+    B = len(insts)
+    X = np.empty((trips, B, NBINS), np.complex128)
+    F = np.empty((trips, B, NBINS), np.complex128)
+    cur = np.stack([sb.bins(np.fft.fft(i["field0"])) for i in insts])
+    t0 = time.time()
+    for t in range(trips):
+        run.launch(1)
+        run.sync(allow_nonfinite=True)
+        nxt = np.stack([sb.bins(np.fft.fft(run.read(i)["field"])) for i in range(B)])
+        X[t] = cur
+        F[t] = cav.band_out(nxt)
+        cur = nxt
+        if log and (t % 10 == 9 or t == trips - 1):
+            log(f"  data: trip {t + 1}/{trips}, {time.time() - t0:.0f} s")
+    run.close()
+    pe = np.array([c.pump_energy for c in cfgs])
+    pr = np.array([c.period for c in cfgs])
+    return (X.reshape(-1, NBINS), F.reshape(-1, NBINS), np.tile(pe, trips), np.tile(pr, trips),
+            sb)

@@ -172,3 +172,25 @@ def test_rk23_mixed_with_heun_in_one_launch(emu_ctx):
     assert rel_l2(run.read(1)["field"], Er) < 1e-10
     assert rel_l2(run.read(2)["field"], Eh) < 1e-10
     run.close()
+
+
+def test_converged_instances_are_skipped_until_rewritten(emu_ctx):
+    cfg, b = opo_case()
+    si, ops = b["solver"], b["ops"]
+    run = OPOBatch([dict(solver=si, field0=ops["field0"], pump=ops["pump"], cav=ops["cav"],
+                         ocm=ops["ocm"], delay=cfg.delay)], precision="double", trip_cap=8,
+                   ctx=emu_ctx, tune={"cluster": 2})
+    run.launch(4, 1e30)                 # any finite change counts as converged (trip 2)
+    run.sync()
+    r = run.read(0)
+    assert r["status"] == 2 and r["trips_done"] == 2
+    run.launch(3, 1e30)                 # converged earlier: skipped
+    run.sync()
+    r2 = run.read(0)
+    assert r2["trips_done"] == 2 and np.array_equal(r2["field"], r["field"])
+    run.write_field(0, r["field"])      # rewritten: runs again, first trip has no previous
+    run.launch(3, 1e30)
+    run.sync()
+    r3 = run.read(0)
+    assert r3["trips_done"] == 4 and np.isinf(r3["trip_log"][2, 0])
+    run.close()
