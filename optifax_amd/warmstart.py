@@ -202,3 +202,155 @@ def generate_pairs(cfgs, trips, seeds_rel=(1e-6, 1e-2), rng_seed=0, log=print):
     pr = np.array([c.period for c in cfgs])
     return (X.reshape(-1, NBINS), F.reshape(-1, NBINS), np.tile(pe, trips), np.tile(pr, trips),
             sb)
+
+
+# ---------------------------------------------------------------------------
+# training
+# ---------------------------------------------------------------------------
+def make_dataset(X, F, pe, pr, a_ref=None):
+    """Network inputs / targets from complex band pairs; a_ref defaults to the
+    largest band-output amplitude (so amplitudes are <= 1)."""
+    if a_ref is None:
+        a_ref = float(np.max(np.abs(F)))
+    x, th = input_features(X, pe, pr, a_ref)
+    y = output_features(F, th, a_ref)
+    return x, y, a_ref
+
+
+class BandSurrogate:
+    """Trained network + the band it was trained on."""
+
+    def __init__(self, net: SurrogateMLP, a_ref: float):
+        self.net = net
+        self.a_ref = float(a_ref)
+
+    def __call__(self, Xb, pump_energy, period):
+        x, th = input_features(Xb, pump_energy, period, self.a_ref)
+        y = self.net.forward(x)
+        return from_output_features(y, th, self.a_ref)
+
+
+def train_surrogate(x, y, steps, batch=512, lr=1e-3, width=2048, seed=0, val_frac=0.05,
+                    log=print):
+    """Adam on the device kernels; returns (net, train losses, validation loss,
+    device ms).  The last val_frac of the (shuffled) rows is held out."""
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(x.shape[0])
+    nv = max(64, int(val_frac * x.shape[0]) // 64 * 64)
+    vi, ti = perm[:nv], perm[nv:]
+    net = SurrogateMLP(N_IN, width, N_OUT, max_batch=batch, seed=seed)
+    net.upload_dataset(x[ti], y[ti])
+    losses, ms = [], 0.0
+    done = 0
+    chunk = 200
+    while done < steps:
+        k = min(chunk, steps - done)
+        # cosine decay from lr to lr/20 over the run
+        frac = done / max(1, steps)
+        lr_k = lr * (0.05 + 0.95 * 0.5 * (1 + math.cos(math.pi * frac)))
+        l, t = net.train(k, batch=batch, lr=lr_k, seed=seed + done)
+        losses.extend(l.tolist())
+        ms += t
+        done += k
+        if log:
+            log(f"  train: step {done}/{steps} loss {np.mean(l[-20:]):.4f} lr {lr_k:.2e}")
+    yv = net.forward(x[vi])
+    num = np.linalg.norm(yv - y[vi], axis=1)
+    den = np.linalg.norm(y[vi], axis=1)
+    val = float(np.mean(num / np.where(den > 0, den, 1.0)))
+    return net, np.array(losses), val, ms
+
+
+# ---------------------------------------------------------------------------
+# solves
+# ---------------------------------------------------------------------------
+def _run_to_convergence(run, n_inst, conv_tol, budget, chunk, log, tag):
+    done = 0
+    t0 = time.time()
+    while done < budget:
+        k = min(chunk, budget - done)
+        run.launch(k, conv_tol)
+        run.sync(allow_nonfinite=True)
+        done += k
+        st = [run.read(i)["status"] for i in range(n_inst)]
+        nconv = sum(1 for x in st if x == 2)
+        if log:
+            log(f"  {tag}: {done} trips launched, {nconv}/{n_inst} converged, "
+                f"{time.time() - t0:.1f} s")
+        if all(x in (2, 3) for x in st):
+            break
+    return time.time() - t0
+
+
+def solve_plain(insts, conv_tol=1e-4, budget=1000, chunk=25, log=print):
+    """Plain FP32 round trips from each instance's own initial field until
+    converged (or the budget).  Returns dict(seconds, trips[], status[])."""
+    run = OPOBatch(insts, trip_cap=budget)
+    secs = _run_to_convergence(run, len(insts), conv_tol, budget, chunk, log, "plain")
+    rs = [run.read(i) for i in range(len(insts))]
+    run.close()
+    return dict(seconds=secs, trips=np.array([r["trips_done"] for r in rs]),
+                status=np.array([r["status"] for r in rs]),
+                fields=[r["field"] for r in rs], oc=[r["oc_spec"] for r in rs])
+
+
+def solve_warm(cfgs, insts, builds, sur: BandSurrogate, conv_tol=1e-4, budget=1000,
+               max_surrogate_trips=100, surrogate_tol=1e-3, judge_tol=0.2, seed_rel=1e-3,
+               chunk=25, log=print):
+    """Surrogate trips on the band bins, handover, judged plain trips.
+    Returns dict(seconds, surrogate_seconds, surrogate_trips[], trips[],
+    status[], judge_err[], fallback[])."""
+    t0 = time.time()
+    sw = builds[0]["simwin"]
+    sb = band_bins(sw)
+    cav = BandCavity(sb, insts, builds)
+    B = len(insts)
+    pe = np.array([c.pump_energy for c in cfgs])
+    pr = np.array([c.period for c in cfgs])
+    # smooth seeds in the band (+ pump), bin level
+    seeds = [smooth_seed(b["simwin"], c, seed_rel) for c, b in zip(cfgs, builds)]
+    X = np.stack([sb.bins(ph.SimWindow.fft_order(ph.SimWindow.t2k(s))) for s in seeds]) + cav.pump
+    active = np.ones(B, bool)
+    s_trips = np.zeros(B, int)
+    S_prev = np.zeros((B, NBINS))
+    for t in range(max_surrogate_trips):
+        idx = np.where(active)[0]
+        if idx.size == 0:
+            break
+        F = sur(X[idx], pe[idx], pr[idx])
+        X[idx] = cav.dphase[idx] * F + cav.pump[idx]
+        S = (cav.oc_over_fb[idx] * np.abs(F)) ** 2
+        s_trips[idx] += 1
+        if t > 0:
+            done = conv_metric(S, S_prev[idx]) < surrogate_tol
+            active[idx[done]] = False
+        S_prev[idx] = S
+    t_sur = time.time() - t0
+    # handover: full-grid crystal inputs = pump + band-limited signal
+    fields = []
+    for i in range(B):
+        Xk = insts[i]["pump"] + sb.to_grid(X[i] - cav.pump[i])
+        fields.append(np.fft.ifft(Xk))
+    warm = [dict(inst, field0=f) for inst, f in zip(insts, fields)]
+    run = OPOBatch(warm, trip_cap=budget)
+    # judge: one plain trip vs the surrogate's prediction for the same input
+    F_pred = sur(X, pe, pr)
+    run.launch(1)
+    run.sync(allow_nonfinite=True)
+    nxt = np.stack([sb.bins(np.fft.fft(run.read(i)["field"])) for i in range(B)])
+    F_plain = cav.band_out(nxt)
+    jerr = np.linalg.norm(F_pred - F_plain, axis=1) / np.maximum(np.linalg.norm(F_plain, axis=1),
+                                                                 1e-300)
+    fallback = jerr > judge_tol
+    for i in np.where(fallback)[0]:
+        run.write_field(i, insts[i]["field0"])      # restart plainly from the noise seed
+    if log:
+        log(f"  warm: {t_sur:.2f} s of surrogate trips (max {s_trips.max()}), judge error "
+            f"median {np.median(jerr):.3f}, fallbacks {int(fallback.sum())}/{B}")
+    secs = _run_to_convergence(run, B, conv_tol, budget - 1, chunk, log, "warm")
+    rs = [run.read(i) for i in range(B)]
+    run.close()
+    return dict(seconds=time.time() - t0, surrogate_seconds=t_sur, surrogate_trips=s_trips,
+                trips=np.array([r["trips_done"] for r in rs]),
+                status=np.array([r["status"] for r in rs]), judge_err=jerr, fallback=fallback,
+                fields=[r["field"] for r in rs], oc=[r["oc_spec"] for r in rs])
