@@ -40,6 +40,28 @@ def workload():
     return cfg, cfg.build()
 
 
+def batch_workload(n_inst=256, seed=3):
+    """256 uneven OPO instances drawn from a seed: poling period 28-32 um,
+    crystal length 1-10 mm, grid 2^14-2^16 (window scaled with the grid),
+    detuning +-50 fs around the synchronous delay."""
+    from optifax_amd import physics as ph
+    rng = np.random.default_rng(seed)
+    insts, meta = [], []
+    for i in range(n_inst):
+        lg = int(rng.integers(14, 17))
+        cfg = ph.OPOConfig(n=2 ** lg, window=12e-12 * 2 ** (lg - 15) if lg < 15 else 12e-12,
+                           period=float(rng.uniform(28e-6, 32e-6)),
+                           length=float(rng.uniform(1e-3, 10e-3)),
+                           delay=float(rng.uniform(-50e-15, 50e-15)), delay_mode="sync",
+                           noise_seed=int(rng.integers(1 << 30)))
+        b = cfg.build()
+        ops = b["ops"]
+        insts.append(dict(solver=b["solver"], field0=ops["field0"], pump=ops["pump"],
+                          cav=ops["cav"], ocm=ops["ocm"], delay=b["delay"]))
+        meta.append((2 ** lg, cfg.length))
+    return insts, meta
+
+
 def cpu_baseline(b, attempts_per_trip):
     """Oracle (FP64, one core) on the first 1 mm of the 5 mm pass; the time per
     adaptive step is scaled by the steps of one full round trip."""
@@ -73,12 +95,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=2, help="untimed round trips")
     ap.add_argument("--precision", default="single", choices=["single", "double"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="ring", choices=["ring", "batch"],
+                    help="ring: one config-2 instance (the headline line); batch: 256 uneven "
+                         "instances in one persistent launch")
+    ap.add_argument("--tune", default="heuristic", choices=["heuristic", "auto"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(instances shard across GPUs by design, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "batch":
+        return main_batch(args)
     from optifax_amd.solver import Context, OPOBatch
     cfg, b = workload()
     si, ops = b["solver"], b["ops"]
@@ -140,6 +168,75 @@ def main():
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(b, att / args.steps)
+    run.close()
+    ctx.close()
+    print(json.dumps(out))
+
+
+def main_batch(args):
+    """Config 3: all instances in one persistent launch per round trip."""
+    from optifax_amd.solver import Context, OPOBatch
+    ctx = Context(0)
+    t_build = time.perf_counter()
+    insts, meta = batch_workload()
+    t_build = time.perf_counter() - t_build
+    trip_cap = args.warmup + args.steps + 1
+    tune = "auto" if args.tune == "auto" else None
+    t_tune = time.perf_counter()
+    run = OPOBatch(insts, precision=args.precision, trip_cap=trip_cap, ctx=ctx, tune=tune)
+    t_tune = time.perf_counter() - t_tune
+    print(f"batch: {len(insts)} instances built in {t_build:.1f} s, batch created in "
+          f"{t_tune:.1f} s, grid {run.geometry(0)['grid']}", file=sys.stderr, flush=True)
+    if args.warmup > 0:
+        run.launch(args.warmup)
+        run.sync(allow_nonfinite=True)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        run.launch(1)
+        run.sync(allow_nonfinite=True)
+        print(f"batch: timed trip {k + 1}/{args.steps} {time.perf_counter() - t0:.1f} s",
+              file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    att = acc = 0
+    bytes_total = 0.0
+    for i, (n, L) in enumerate(meta):
+        log = run.read(i)["trip_log"][args.warmup:args.warmup + args.steps]
+        a_i, c_i = int(log[:, 2].sum()), int(log[:, 3].sum())
+        att += a_i
+        acc += c_i
+        bytes_total += algorithmic_bytes(n, c_i, a_i - c_i, args.steps)
+    inst_trips = len(insts) * args.steps
+    achieved = bytes_total / wall / 1e9
+    out = {
+        "metric": METRIC,
+        "value": inst_trips / wall,
+        "unit": "instance round trips/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision == "single" else "f64",
+        "data": "synthetic (sech^2 pump + seeded one-photon-per-mode noise)",
+        "config": {
+            "workload": "256 uneven OPO instances in one persistent launch: period 28-32 um, "
+                        "length 1-10 mm, grid 2^14-2^16, detuning +-50 fs (seed 3)",
+            "instances": len(insts),
+            "adaptive_steps_per_instance_trip": att / inst_trips,
+            "tune": args.tune,
+            "grid_workgroups": run.geometry(0)["grid"],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+        },
+    }
     run.close()
     ctx.close()
     print(json.dumps(out))

@@ -68,27 +68,40 @@ struct GemmArgs {
   int ldz;
 };
 
+// 512 threads = two halves of 4 waves.  Half h accumulates the K slices
+// kt = 2i + h of the same 64 x 64 tile (its own double-buffered LDS), so a CU
+// holds two waves per SIMD with independent load streams; at the end half 1
+// hands its accumulators to half 0 through LDS, which runs the epilogue.
 template <int EPI>
-__global__ void __launch_bounds__(256) gemm_nt(GemmArgs g) {
-  __shared__ __align__(16) bf16 As[2][TM * LDK];
-  __shared__ __align__(16) bf16 Bs[2][TN * LDK];
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+__global__ void __launch_bounds__(512) gemm_nt(GemmArgs g) {
+  __shared__ __align__(16) bf16 As[2][2][TM * LDK];  // [half][stage]
+  __shared__ __align__(16) bf16 Bs[2][2][TN * LDK];
+  const int tid = threadIdx.x, half = tid >> 8, th = tid & 255;
+  const int lane = th & 63, w = th >> 6;
   const int wr = w >> 1, wc = w & 1;
   const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
-  // staging: a 64 x 64 slice is 512 chunks of 8 bf16; thread takes rows r and r + 32
-  const int r = tid >> 3, c = (tid & 7) * 8;
+  const int r = th >> 3, c = (th & 7) * 8;
   const bf16* Ap = g.A + (size_t)(m0 + r) * g.lda + c;
   const bf16* Bp = g.B + (size_t)(n0 + r) * g.ldb + c;
   const size_t a32 = (size_t)32 * g.lda, b32 = (size_t)32 * g.ldb;
   const int so0 = r * LDK + c, so1 = (r + 32) * LDK + c;
-  bf16x8 ra0 = *reinterpret_cast<const bf16x8*>(Ap);
-  bf16x8 ra1 = *reinterpret_cast<const bf16x8*>(Ap + a32);
-  bf16x8 rb0 = *reinterpret_cast<const bf16x8*>(Bp);
-  bf16x8 rb1 = *reinterpret_cast<const bf16x8*>(Bp + b32);
-  *reinterpret_cast<bf16x8*>(&As[0][so0]) = ra0;
-  *reinterpret_cast<bf16x8*>(&As[0][so1]) = ra1;
-  *reinterpret_cast<bf16x8*>(&Bs[0][so0]) = rb0;
-  *reinterpret_cast<bf16x8*>(&Bs[0][so1]) = rb1;
+  const int nk = g.K / TK;
+  const int iters = (nk + 1) >> 1;  // both halves run the same number of iterations
+  bf16x8 ra0, ra1, rb0, rb1;
+  {
+    const int kt = half;
+    if (kt < nk) {
+      const size_t ko = (size_t)kt * TK;
+      ra0 = *reinterpret_cast<const bf16x8*>(Ap + ko);
+      ra1 = *reinterpret_cast<const bf16x8*>(Ap + a32 + ko);
+      rb0 = *reinterpret_cast<const bf16x8*>(Bp + ko);
+      rb1 = *reinterpret_cast<const bf16x8*>(Bp + b32 + ko);
+      *reinterpret_cast<bf16x8*>(&As[half][0][so0]) = ra0;
+      *reinterpret_cast<bf16x8*>(&As[half][0][so1]) = ra1;
+      *reinterpret_cast<bf16x8*>(&Bs[half][0][so0]) = rb0;
+      *reinterpret_cast<bf16x8*>(&Bs[half][0][so1]) = rb1;
+    }
+  }
   __syncthreads();
   f32x16 acc;
 #pragma unroll
@@ -96,33 +109,46 @@ __global__ void __launch_bounds__(256) gemm_nt(GemmArgs g) {
   // MFMA operands: lane l supplies row (A) / column (B) l % 32, k = 8 (l / 32) .. + 7
   const int ao = (32 * wr + (lane & 31)) * LDK + 8 * (lane >> 5);
   const int bo = (32 * wc + (lane & 31)) * LDK + 8 * (lane >> 5);
-  const int nk = g.K / TK;
-  for (int kt = 0; kt < nk; kt++) {
-    const int cur = kt & 1;
-    const bool more = kt + 1 < nk;
+  for (int it = 0; it < iters; it++) {
+    const int cur = it & 1;
+    const int kt = 2 * it + half;
+    const bool have = kt < nk;
+    const bool more = kt + 2 < nk;
     if (more) {
-      const size_t ko = (size_t)(kt + 1) * TK;
+      const size_t ko = (size_t)(kt + 2) * TK;
       ra0 = *reinterpret_cast<const bf16x8*>(Ap + ko);
       ra1 = *reinterpret_cast<const bf16x8*>(Ap + a32 + ko);
       rb0 = *reinterpret_cast<const bf16x8*>(Bp + ko);
       rb1 = *reinterpret_cast<const bf16x8*>(Bp + b32 + ko);
     }
-    const bf16* as = &As[cur][ao];
-    const bf16* bs = &Bs[cur][bo];
+    if (have) {
+      const bf16* as = &As[half][cur][ao];
+      const bf16* bs = &Bs[half][cur][bo];
 #pragma unroll
-    for (int ks = 0; ks < TK; ks += 16) {
-      const bf16x8 a = *reinterpret_cast<const bf16x8*>(as + ks);
-      const bf16x8 b = *reinterpret_cast<const bf16x8*>(bs + ks);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+      for (int ks = 0; ks < TK; ks += 16) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(as + ks);
+        const bf16x8 b = *reinterpret_cast<const bf16x8*>(bs + ks);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+      }
     }
     if (more) {
-      *reinterpret_cast<bf16x8*>(&As[cur ^ 1][so0]) = ra0;
-      *reinterpret_cast<bf16x8*>(&As[cur ^ 1][so1]) = ra1;
-      *reinterpret_cast<bf16x8*>(&Bs[cur ^ 1][so0]) = rb0;
-      *reinterpret_cast<bf16x8*>(&Bs[cur ^ 1][so1]) = rb1;
+      *reinterpret_cast<bf16x8*>(&As[half][cur ^ 1][so0]) = ra0;
+      *reinterpret_cast<bf16x8*>(&As[half][cur ^ 1][so1]) = ra1;
+      *reinterpret_cast<bf16x8*>(&Bs[half][cur ^ 1][so0]) = rb0;
+      *reinterpret_cast<bf16x8*>(&Bs[half][cur ^ 1][so1]) = rb1;
     }
     __syncthreads();
   }
+  // half 1 -> half 0 through LDS (the A buffers: 256 threads x 16 floats = 16 KB)
+  float* xch = reinterpret_cast<float*>(&As[0][0][0]);
+  if (half == 1) {
+#pragma unroll
+    for (int i = 0; i < 16; i++) xch[i * 256 + th] = acc[i];
+  }
+  __syncthreads();
+  if (half == 1) return;
+#pragma unroll
+  for (int i = 0; i < 16; i++) acc[i] += xch[i * 256 + th];
   // accumulator j of lane l holds row 8 (j / 4) + 4 (l / 32) + j % 4, column l % 32
   const int col = n0 + 32 * wc + (lane & 31);
   const int rbase = m0 + 32 * wr + 4 * (lane >> 5);
@@ -328,7 +354,7 @@ static int gemm(ofx_mlp* m, const GemmArgs& a) {
   // every dimension is a multiple of the tile; checked once at create
   if (a.M % TM || a.N % TN || a.K % TK || a.M <= 0 || a.N <= 0 || a.K <= 0)
     return arg_err(m, "GEMM dimensions are not multiples of 64");
-  hipLaunchKernelGGL(gemm_nt<EPI>, dim3(a.N / TN, a.M / TM), dim3(256), 0, m->stream, a);
+  hipLaunchKernelGGL(gemm_nt<EPI>, dim3(a.N / TN, a.M / TM), dim3(512), 0, m->stream, a);
   MLAUNCH(m);
   return OFX_OK;
 }

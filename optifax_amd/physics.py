@@ -220,17 +220,22 @@ def energy_of(simwin: SimWindow, E, area, n_medium=1.0):
     return float(np.sum(np.abs(Ek) ** 2 * (C0 * n_medium * EPS0 / 2) * area * simwin.dt / simwin.n))
 
 
-def qpm_signal_wavelength(lambda_p, period, temperature=20.0, lo=1.2e-6, hi=2.0e-6):
+def qpm_signal_wavelength(lambda_p, period, temperature=20.0, lo=None, hi=None):
     """Signal wavelength of first-order QPM, k_p - k_s - k_i = 2 pi / period,
-    with the Gayer extraordinary index (bisection in [lo, hi])."""
+    with the Gayer extraordinary index (bisection between 1.05 lambda_p and
+    degeneracy, 2 lambda_p, unless lo / hi are given).  Without a root in the
+    interval, the end with the smaller phase mismatch."""
+    lo = 1.05 * lambda_p if lo is None else lo
+    hi = 2.0 * lambda_p if hi is None else hi
     def dk(ls):
         li = 1.0 / (1.0 / lambda_p - 1.0 / ls)
         n = gayer_n(np.array([lambda_p, ls, li]) * 1e6, temperature, 1)
         return 2 * np.pi * (n[0] / lambda_p - n[1] / ls - n[2] / li - 1.0 / period)
     a, b = lo, hi
-    fa = dk(a)
-    if fa * dk(b) > 0:
-        raise ValueError("no QPM signal wavelength in the search interval")
+    fa, fb = dk(a), dk(b)
+    if fa * fb > 0:
+        # no exact match (e.g. a grating past degeneracy): the best-matched end
+        return a if abs(fa) < abs(fb) else b
     for _ in range(80):
         m = 0.5 * (a + b)
         fm = dk(m)
