@@ -62,6 +62,113 @@ def batch_workload(n_inst=256, seed=3):
     return insts, meta
 
 
+def warmstart_configs(n_inst=64, seed=5):
+    """Config 5 at a reduced grid: 64 cavity detunings (+40 .. +100 fs from
+    synchronism) of an 8 mm fan-out PPLN (local period 29-31 um, one position
+    per instance), 4 W at 1 GHz of 1 ps sech^2 pump in pi (50 um)^2, 16 ps
+    window on 2^14 points, Heun with fixed 0.4 um steps (error control off:
+    maxErr = inf, minErr = 0)."""
+    from optifax_amd import physics as ph
+    rng = np.random.default_rng(seed)
+    periods = rng.permutation(np.linspace(29e-6, 31e-6, n_inst))
+    dets = np.linspace(40e-15, 100e-15, n_inst)
+    return [ph.OPOConfig(n=2 ** 14, window=16e-12, length=8e-3, period=float(p),
+                         pump_energy=4e-9, pump_fwhm=1e-12, area=np.pi * (50e-6) ** 2,
+                         delay=float(d), delay_mode="sync", error_bounds=(0.0, 1e30),
+                         noise_seed=1000 + i)
+            for i, (p, d) in enumerate(zip(periods, dets))]
+
+
+def training_configs(n_inst=64, seed=11):
+    """Trajectories for the surrogate's data, drawn apart from the solve's
+    instances: period 29-31 um, detuning +30 .. +110 fs, pump 3.5-4.5 nJ."""
+    from optifax_amd import physics as ph
+    rng = np.random.default_rng(seed)
+    return [ph.OPOConfig(n=2 ** 14, window=16e-12, length=8e-3,
+                         period=float(rng.uniform(29e-6, 31e-6)),
+                         pump_energy=float(rng.uniform(3.5e-9, 4.5e-9)), pump_fwhm=1e-12,
+                         area=np.pi * (50e-6) ** 2, delay=float(rng.uniform(30e-15, 110e-15)),
+                         delay_mode="sync", error_bounds=(0.0, 1e30), noise_seed=5000 + i)
+            for i in range(n_inst)]
+
+
+def main_warmstart(args):
+    """Time to steady state (trip-to-trip change of the output-coupled
+    spectrum < 1e-4, 1000-trip budget), plain FP32 round trips from noise vs
+    surrogate warm start (data generation and training reported apart)."""
+    from optifax_amd import warmstart as ws
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    cfgs = warmstart_configs(args.ws_instances)
+    pairs = [ws.opo_instance(c) for c in cfgs]
+    insts = [p[0] for p in pairs]
+    builds = [p[1] for p in pairs]
+    log(f"warmstart: {len(insts)} instances")
+    plain = ws.solve_plain(insts, conv_tol=1e-4, budget=1000, log=log)
+    log(f"plain: {plain['seconds']:.1f} s, trips median {np.median(plain['trips'])}")
+    t0 = time.perf_counter()
+    X, F, pe, pr, sb = ws.generate_pairs(training_configs(args.ws_train_instances),
+                                         args.ws_train_trips, log=log)
+    t_data = time.perf_counter() - t0
+    x, y, a_ref = ws.make_dataset(X, F, pe, pr)
+    t0 = time.perf_counter()
+    net, losses, val, ms = ws.train_surrogate(x, y, args.ws_train_steps, log=log)
+    t_train = time.perf_counter() - t0
+    sur = ws.BandSurrogate(net, a_ref)
+    warm = ws.solve_warm(cfgs, insts, builds, sur, conv_tol=1e-4, budget=1000, log=log)
+    conv_p = plain["status"] == 2
+    conv_w = warm["status"] == 2
+    # agreement of the two steady states (output-coupled power spectra)
+    errs = []
+    for i in np.where(conv_p & conv_w)[0]:
+        Sp = np.abs(plain["oc"][i]) ** 2
+        Sw = np.abs(warm["oc"][i]) ** 2
+        errs.append(np.linalg.norm(Sw - Sp) / np.linalg.norm(Sp))
+    out = {
+        "metric": METRIC,
+        "value": plain["seconds"] / warm["seconds"],
+        "unit": "x (time to steady state, plain / warm start)",
+        "n_gpus": 1,
+        "steps": int(np.max(plain["trips"])),
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 fields, bf16 surrogate",
+        "data": "synthetic (sech^2 pump + seeded one-photon-per-mode noise; surrogate "
+                "trained on solver trajectories)",
+        "config": {
+            "workload": "config 5 at 2^14 points: 64 detunings +40..+100 fs, 8 mm fan-out "
+                        "PPLN 29-31 um, 4 W @ 1 GHz 1 ps sech^2, 5% OC, fixed 0.4 um steps",
+            "instances": len(insts),
+            "plain_seconds": plain["seconds"],
+            "warm_seconds": warm["seconds"],
+            "surrogate_seconds": warm["surrogate_seconds"],
+            "plain_trips_median": float(np.median(plain["trips"])),
+            "warm_trips_median": float(np.median(warm["trips"])),
+            "surrogate_trips_median": float(np.median(warm["surrogate_trips"])),
+            "converged_plain": int(conv_p.sum()),
+            "converged_warm": int(conv_w.sum()),
+            "fallbacks": int(warm["fallback"].sum()),
+            "judge_err_median": float(np.median(warm["judge_err"])),
+            "steady_state_rel_l2_warm_vs_plain_median": float(np.median(errs)) if errs else None,
+            "train_pairs": int(x.shape[0]),
+            "train_seconds": t_train,
+            "data_seconds": t_data,
+            "train_loss_last": float(np.mean(losses[-50:])),
+            "val_loss": val,
+            "mlp_train_ms": ms,
+        },
+    }
+    if args.ws_out:
+        np.savez(args.ws_out, plain_trips=plain["trips"], warm_trips=warm["trips"],
+                 plain_status=plain["status"], warm_status=warm["status"],
+                 sur_trips=warm["surrogate_trips"], judge_err=warm["judge_err"],
+                 losses=losses, steady_err=np.array(errs))
+    net.close()
+    print(json.dumps(out))
+
+
 def cpu_baseline(b, attempts_per_trip):
     """Oracle (FP64, one core) on the first 1 mm of the 5 mm pass; the time per
     adaptive step is scaled by the steps of one full round trip."""
@@ -95,11 +202,28 @@ def main():
     ap.add_argument("--warmup", type=int, default=2, help="untimed round trips")
     ap.add_argument("--precision", default="single", choices=["single", "double"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="ring", choices=["ring", "batch"],
+    ap.add_argument("--ws-instances", type=int, default=64)
+    ap.add_argument("--ws-train-instances", type=int, default=64)
+    ap.add_argument("--ws-train-trips", type=int, default=150)
+    ap.add_argument("--ws-train-steps", type=int, default=3000)
+    ap.add_argument("--ws-out", default="")
+    ap.add_argument("--workload", default="ring", choices=["ring", "batch", "warmstart"],
                     help="ring: one config-2 instance (the headline line); batch: 256 uneven "
                          "instances in one persistent launch")
     ap.add_argument("--tune", default="heuristic", choices=["heuristic", "auto"])
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device copy of 256 MiB three times (the known-size "
+                         "reference for the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": x.numel() * 4, "copies": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(instances shard across GPUs by design, see DESIGN.md)", file=sys.stderr)
@@ -107,6 +231,8 @@ def main():
 
     if args.workload == "batch":
         return main_batch(args)
+    if args.workload == "warmstart":
+        return main_warmstart(args)
     from optifax_amd.solver import Context, OPOBatch
     cfg, b = workload()
     si, ops = b["solver"], b["ops"]

@@ -163,7 +163,7 @@ def conv_metric(S, Sp):
 # ---------------------------------------------------------------------------
 # training data from device trajectories
 # ---------------------------------------------------------------------------
-def generate_pairs(cfgs, trips, seeds_rel=(1e-6, 1e-2), rng_seed=0, log=print):
+def generate_pairs(cfgs, trips, seeds_rel=(1e-6, 1e-2), rng_seed=0, log=print, ctx=None):
     """Run each configuration from a smooth seed of random strength and phase
     for `trips` plain round trips (one persistent launch per trip, all
     instances together) and return (inputs X [B*trips, 1024] complex bins,
@@ -182,7 +182,7 @@ def generate_pairs(cfgs, trips, seeds_rel=(1e-6, 1e-2), rng_seed=0, log=print):
         builds.append(b)
     sb = band_bins(builds[0]["simwin"])
     cav = BandCavity(sb, insts, builds)
-    run = OPOBatch(insts, trip_cap=trips)
+    run = OPOBatch(insts, trip_cap=trips, ctx=ctx)
     B = len(insts)
     X = np.empty((trips, B, NBINS), np.complex128)
     F = np.empty((trips, B, NBINS), np.complex128)
@@ -282,10 +282,10 @@ def _run_to_convergence(run, n_inst, conv_tol, budget, chunk, log, tag):
     return time.time() - t0
 
 
-def solve_plain(insts, conv_tol=1e-4, budget=1000, chunk=25, log=print):
+def solve_plain(insts, conv_tol=1e-4, budget=1000, chunk=25, log=print, ctx=None):
     """Plain FP32 round trips from each instance's own initial field until
     converged (or the budget).  Returns dict(seconds, trips[], status[])."""
-    run = OPOBatch(insts, trip_cap=budget)
+    run = OPOBatch(insts, trip_cap=budget, ctx=ctx)
     secs = _run_to_convergence(run, len(insts), conv_tol, budget, chunk, log, "plain")
     rs = [run.read(i) for i in range(len(insts))]
     run.close()
@@ -296,7 +296,7 @@ def solve_plain(insts, conv_tol=1e-4, budget=1000, chunk=25, log=print):
 
 def solve_warm(cfgs, insts, builds, sur: BandSurrogate, conv_tol=1e-4, budget=1000,
                max_surrogate_trips=100, surrogate_tol=1e-3, judge_tol=0.2, seed_rel=1e-3,
-               chunk=25, log=print):
+               chunk=25, log=print, ctx=None):
     """Surrogate trips on the band bins, handover, judged plain trips.
     Returns dict(seconds, surrogate_seconds, surrogate_trips[], trips[],
     status[], judge_err[], fallback[])."""
@@ -332,7 +332,7 @@ def solve_warm(cfgs, insts, builds, sur: BandSurrogate, conv_tol=1e-4, budget=10
         Xk = insts[i]["pump"] + sb.to_grid(X[i] - cav.pump[i])
         fields.append(np.fft.ifft(Xk))
     warm = [dict(inst, field0=f) for inst, f in zip(insts, fields)]
-    run = OPOBatch(warm, trip_cap=budget)
+    run = OPOBatch(warm, trip_cap=budget, ctx=ctx)
     # judge: one plain trip vs the surrogate's prediction for the same input
     F_pred = sur(X, pe, pr)
     run.launch(1)
