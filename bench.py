@@ -40,6 +40,21 @@ def workload():
     return cfg, cfg.build()
 
 
+def measured_traffic(key, adaptive_steps_per_launch):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary
+    (profiles/pmc_traffic.json: FETCH_SIZE + WRITE_SIZE of the pass kernel,
+    each divided by what the counter reads for a copy of known size), scaled
+    by this run's adaptive steps per launch; None without a summary."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)[key]
+    except (OSError, ValueError, KeyError):
+        return None
+    return rec["traffic_bytes_per_launch"] * adaptive_steps_per_launch / \
+        rec["adaptive_steps_per_launch"]
+
+
 def batch_workload(n_inst=256, seed=3):
     """256 uneven OPO instances drawn from a seed: poling period 28-32 um,
     crystal length 1-10 mm, grid 2^14-2^16 (window scaled with the grid),
@@ -169,6 +184,59 @@ def main_warmstart(args):
     print(json.dumps(out))
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def mlp_step_flops(batch, n_in_padded, width, n_out):
+    """Algorithmic flops of one training step: forward, weight gradients, and
+    the data gradients of layers 1-3 (the input needs none)."""
+    fwd = 2.0 * batch * (n_in_padded * width + 2 * width * width + width * n_out)
+    dgrad = 2.0 * batch * (2 * width * width + width * n_out)
+    return 2.0 * fwd + dgrad
+
+
+def main_surrogate(args):
+    """Config 4 shape: 2050 -> 2048 x3 -> 2048 MLP, batch 512, Adam, on a
+    synthetic device-resident set of 8192 pairs; steps = Adam steps."""
+    from optifax_amd.surrogate import SurrogateMLP
+    n_in, width, n_out, batch = 2050, 2048, 2048, 512
+    rng = np.random.default_rng(0)
+    X = rng.standard_normal((8192, n_in)).astype(np.float32)
+    T = rng.standard_normal((8192, n_out)).astype(np.float32)
+    net = SurrogateMLP(n_in, width, n_out, max_batch=batch, seed=0)
+    net.upload_dataset(X, T)
+    if args.warmup > 0:
+        net.train(args.warmup, batch=batch)
+    t0 = time.perf_counter()
+    losses, ms = net.train(args.steps, batch=batch)
+    wall = time.perf_counter() - t0
+    flops = mlp_step_flops(batch, (n_in + 63) // 64 * 64, width, n_out)
+    tflops = flops / (ms / args.steps * 1e-3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": args.steps / wall,
+        "unit": "surrogate training steps/s (batch 512)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16 (fp32 accumulate)",
+        "data": "synthetic",
+        "config": {"workload": "surrogate MLP 2050-2048-2048-2048-2048, GELU, batch 512, Adam, "
+                               "relative-L2 loss",
+                   "device_ms_per_step": ms / args.steps,
+                   "gflop_per_step": flops / 1e9},
+        "roofline": {"bound": "mfma", "achieved": tflops, "peak": MFMA_BF16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": tflops / MFMA_BF16_PEAK_TFLOPS,
+                     "traffic": None},
+    }
+    net.close()
+    print(json.dumps(out))
+
+
 def cpu_baseline(b, attempts_per_trip):
     """Oracle (FP64, one core) on the first 1 mm of the 5 mm pass; the time per
     adaptive step is scaled by the steps of one full round trip."""
@@ -207,7 +275,8 @@ def main():
     ap.add_argument("--ws-train-trips", type=int, default=150)
     ap.add_argument("--ws-train-steps", type=int, default=3000)
     ap.add_argument("--ws-out", default="")
-    ap.add_argument("--workload", default="ring", choices=["ring", "batch", "warmstart"],
+    ap.add_argument("--workload", default="ring",
+                    choices=["ring", "batch", "warmstart", "surrogate"],
                     help="ring: one config-2 instance (the headline line); batch: 256 uneven "
                          "instances in one persistent launch")
     ap.add_argument("--tune", default="heuristic", choices=["heuristic", "auto"])
@@ -233,6 +302,8 @@ def main():
         return main_batch(args)
     if args.workload == "warmstart":
         return main_warmstart(args)
+    if args.workload == "surrogate":
+        return main_surrogate(args)
     from optifax_amd.solver import Context, OPOBatch
     cfg, b = workload()
     si, ops = b["solver"], b["ops"]
@@ -289,7 +360,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic("ring_n32768_f32" if args.precision == "single"
+                                        else "ring_n32768_f64", att / args.steps),
         },
     }
     if not args.no_cpu_baseline:
