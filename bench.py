@@ -371,6 +371,21 @@ def main():
     print(json.dumps(out))
 
 
+def _geometry_report(run, meta, tune):
+    rep = {}
+    if tune == "auto":
+        from optifax_amd.autotune import tune_batch
+        for n, best in getattr(tune_batch, "last_report", {}).items():
+            rep[str(n)] = {k: best[k] for k in ("cluster", "n1", "block", "us_per_step",
+                                                 "candidates", "tuning_s")}
+        return rep
+    for i, (n, _) in enumerate(meta):
+        if str(n) not in rep:
+            g = run.geometry(i)
+            rep[str(n)] = {"cluster": g["cluster"], "n1": g["n1"], "block": g["block"]}
+    return rep
+
+
 def main_batch(args):
     """Config 3: all instances in one persistent launch per round trip."""
     from optifax_amd.solver import Context, OPOBatch
@@ -425,6 +440,7 @@ def main_batch(args):
             "adaptive_steps_per_instance_trip": att / inst_trips,
             "tune": args.tune,
             "grid_workgroups": run.geometry(0)["grid"],
+            "geometry_per_grid": _geometry_report(run, meta, args.tune),
         },
         "roofline": {
             "bound": "hbm",

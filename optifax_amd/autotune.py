@@ -175,4 +175,6 @@ def tune_batch(instances, precision="single", steps=1500, ctx=None, verbose=Fals
         blocks[n] = (len(ids), best["block"])
     # one block size per launch: the one chosen for the most instances
     block = max(blocks.values())[1]
+    tune_batch.last_report = {n: tune(instances[ids[0]], batch=len(ids), precision=precision,
+                                      steps=steps, ctx=ctx) for n, ids in groups.items()}
     return out, block
