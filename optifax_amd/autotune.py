@@ -85,8 +85,13 @@ def _truncate(inst, steps):
     k = max(1, min(ns, int(steps)))
     s["nSteps"] = k
     s["G33"] = np.asarray(s["G33"])[:k]
-    # alpha_w = T^(1/NSteps): keep the per-step loss of the full crystal
-    return dict(solver=s, field0=inst["field0"], delay=inst.get("delay", 0.0))
+    # one chunk covering the truncated pass (a longer chunk would leave no
+    # complete chunk to run), no snapshots; alpha_w keeps the full crystal's
+    # per-step loss
+    s["sel"] = float(min(float(s["sel"]), k))
+    s["n_cols"] = 0
+    return dict(solver=s, field0=inst["field0"], delay=inst.get("delay", 0.0),
+                method=inst.get("method", s.get("method", "heun")))
 
 
 def measure(inst, cand, k, precision="single", steps=1500, ctx=None, repeats=2):
@@ -107,8 +112,9 @@ def measure(inst, cand, k, precision="single", steps=1500, ctx=None, repeats=2):
             run.sync(allow_nonfinite=True)
             ms = run.last_ms()
             r = run.read(0)
-            att = max(1, r["attempts"])
-            t = ms * 1e-3 / att
+            if r["attempts"] < 1:
+                return None
+            t = ms * 1e-3 / r["attempts"]
             best = t if best is None else min(best, t)
         return best
     finally:

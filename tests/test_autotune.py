@@ -59,3 +59,17 @@ def test_tuned_uneven_batch_matches_default_geometry(emu_ctx, tmp_path, monkeypa
     assert (ga["cluster"], ga["n1"]) == (tuned[1]["cluster"], tuned[1]["n1"])
     a.close()
     b.close()
+
+
+def test_measurement_pass_takes_steps(emu_ctx):
+    """The truncated pass the tuner times must run (a chunk longer than the
+    truncated crystal would leave nothing to time)."""
+    inst = _inst()
+    proto = autotune._truncate(inst, 12)
+    assert proto["solver"]["nSteps"] == 12 and proto["solver"]["sel"] <= 12
+    from optifax_amd.solver import OPOBatch
+    run = OPOBatch([dict(proto, cluster=2)], precision="double", ctx=emu_ctx)
+    run.launch(0)
+    run.sync()
+    assert run.read(0)["attempts"] >= 3
+    run.close()
