@@ -77,8 +77,8 @@ def batch_workload(n_inst=256, seed=3):
     return insts, meta
 
 
-def warmstart_configs(n_inst=64, seed=5):
-    """Config 5 at a reduced grid: 64 cavity detunings (+50 .. +110 fs from
+def warmstart_configs(n_inst=32, seed=5):
+    """Config 5 at a reduced grid: cavity detunings (+50 .. +110 fs from
     synchronism) of an 8 mm fan-out PPLN (local period 29-31 um, one position
     per instance), 4 W at 1 GHz of 1 ps sech^2 pump in pi (50 um)^2, 16 ps
     window on 2^14 points, Heun with fixed 0.4 um steps (error control off:
@@ -120,15 +120,18 @@ def main_warmstart(args):
     builds = [p[1] for p in pairs]
     log(f"warmstart: {len(insts)} instances, phase {args.ws_phase}")
     cfg_out = {
-        "workload": "config 5 at 2^14 points: 64 detunings +50..+110 fs, 8 mm fan-out PPLN "
+        "workload": f"config 5 at 2^14 points: {len(insts)} detunings +50..+110 fs, 8 mm fan-out PPLN "
                     "29-31 um, 4 W @ 1 GHz 1 ps sech^2 in pi (50 um)^2, 5% OC, fixed 0.4 um "
                     "steps; steady state = change between trips < 1e-4, budget 1000 trips",
         "instances": len(insts),
     }
     plain = warm = None
     if args.ws_phase in ("all", "plain"):
-        plain = ws.solve_plain(insts, conv_tol=1e-4, budget=1000, log=log)
+        plain = ws.solve_plain(insts, conv_tol=1e-4, budget=1000, log=log,
+                               max_seconds=args.ws_max_seconds)
         cfg_out.update(plain_seconds=plain["seconds"],
+                       plain_conv_seconds=[None if not np.isfinite(x) else round(float(x), 2)
+                                           for x in plain["conv_seconds"]],
                        plain_trips_median=float(np.median(plain["trips"])),
                        plain_trips_max=int(np.max(plain["trips"])),
                        converged_plain=int((plain["status"] == 2).sum()))
@@ -142,8 +145,11 @@ def main_warmstart(args):
         net, losses, val, ms = ws.train_surrogate(x, y, args.ws_train_steps, log=log)
         t_train = time.perf_counter() - t0
         sur = ws.BandSurrogate(net, a_ref)
-        warm = ws.solve_warm(cfgs, insts, builds, sur, conv_tol=1e-4, budget=1000, log=log)
+        warm = ws.solve_warm(cfgs, insts, builds, sur, conv_tol=1e-4, budget=1000, log=log,
+                             max_seconds=args.ws_max_seconds)
         cfg_out.update(warm_seconds=warm["seconds"], surrogate_seconds=warm["surrogate_seconds"],
+                       warm_conv_seconds=[None if not np.isfinite(x) else round(float(x), 2)
+                                          for x in warm["conv_seconds"]],
                        warm_trips_median=float(np.median(warm["trips"])),
                        warm_trips_max=int(np.max(warm["trips"])),
                        surrogate_trips_median=float(np.median(warm["surrogate_trips"])),
@@ -284,9 +290,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2, help="untimed round trips")
     ap.add_argument("--precision", default="single", choices=["single", "double"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--ws-instances", type=int, default=64)
-    ap.add_argument("--ws-train-instances", type=int, default=64)
-    ap.add_argument("--ws-train-trips", type=int, default=150)
+    ap.add_argument("--ws-instances", type=int, default=32)
+    ap.add_argument("--ws-train-instances", type=int, default=32)
+    ap.add_argument("--ws-max-seconds", type=float, default=900.0,
+                    help="stop each solve after this many seconds (results so far are reported)")
+    ap.add_argument("--ws-train-trips", type=int, default=250)
     ap.add_argument("--ws-train-steps", type=int, default=3000)
     ap.add_argument("--ws-out", default="")
     ap.add_argument("--ws-phase", default="all", choices=["all", "plain", "warm"])

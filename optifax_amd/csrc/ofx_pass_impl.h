@@ -405,6 +405,8 @@ struct Geo {
   int Cn, lgC, rank, W, lgW, Rr, lgR, HW, LC, sT, sF, c0, k20;
   int coh;  // exchange accesses: 1 device-scope atomics (always used on the GPU)
   cpx<R>*B0, *B1, *B2, *B3, *B4;  // B3, B4: RK23 only
+  const cpx<R>* tw1;  // LDS: exp(-2 pi i j / n1), j < n1
+  const cpx<R>* tw2;  // LDS: exp(-2 pi i j / n2), j < n2
 };
 
 template <typename R>
@@ -424,8 +426,7 @@ template <typename R>
 OFX_DEV void fwd_cols_to_t1(const Geo<R>& g, cpx<R>* src, cpx<R>* other, cpx<R>* t1,
                             const cpx<R>* tw, int nmax) {
   typedef cpx<R> C;
-  C* res = lds_fft<R, -1>(src + g.HW * g.sT, other + g.HW * g.sT, g.lg2, g.W, g.sT, tw,
-                          nmax >> g.lg2);
+  C* res = lds_fft<R, -1>(src + g.HW * g.sT, other + g.HW * g.sT, g.lg2, g.W, g.sT, g.tw2, 1);
   const int tws = nmax >> g.lgn;
   const int tot = g.W * g.n2;
   for (int idx = OFX_TID; idx < tot; idx += OFX_NT) {
@@ -448,7 +449,7 @@ OFX_DEV cpx<R>* t1_to_rows_fwd(const Geo<R>& g, const cpx<R>* t1, cpx<R>* dst, c
     dst[k2l * g.sF + c] = xld(&t1[(g.k20 + k2l) * g.n1 + c], g.coh);
   }
   OFX_SYNC();
-  return lds_fft<R, -1>(dst, other, g.lg1, g.Rr, g.sF, tw, nmax >> g.lg1);
+  return lds_fft<R, -1>(dst, other, g.lg1, g.Rr, g.sF, g.tw1, 1);
 }
 
 // After the first barrier of an attempt: read the cluster's error records
@@ -498,7 +499,7 @@ template <typename R>
 OFX_DEV void rows_inv_to_t2(const Geo<R>& g, cpx<R>* src, cpx<R>* other, cpx<R>* t2,
                             const cpx<R>* tw, int nmax) {
   typedef cpx<R> C;
-  C* res = lds_fft<R, +1>(src, other, g.lg1, g.Rr, g.sF, tw, nmax >> g.lg1);
+  C* res = lds_fft<R, +1>(src, other, g.lg1, g.Rr, g.sF, g.tw1, 1);
   const int tws = nmax >> g.lgn;
   const int tot = g.Rr * g.n1;
   for (int idx = OFX_TID; idx < tot; idx += OFX_NT) {
@@ -523,7 +524,7 @@ OFX_DEV void t2_to_cols_inv(const Geo<R>& g, const cpx<R>* t2, const cpx<R>* tw,
     g.B1[lc * g.sT + k2] = xld(&t2[gc * g.n2 + k2], g.coh);
   }
   OFX_SYNC();
-  C* res = lds_fft<R, +1>(g.B1, g.B2, g.lg2, g.LC, g.sT, tw, nmax >> g.lg2);
+  C* res = lds_fft<R, +1>(g.B1, g.B2, g.lg2, g.LC, g.sT, g.tw2, 1);
   const R inv = R(1) / R(g.n);
   for (int idx = OFX_TID; idx < tot; idx += OFX_NT) {
     const int lc = idx >> g.lg2, rp = idx & (g.n2 - 1);
@@ -705,7 +706,7 @@ OFX_DEV int rk23_pass(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc
       }
       // frequency phase: rows, dispersion (step just taken), snapshot, inverse rows
       const double s_used = stepmod;
-      C* F = lds_fft<R, -1>(g.B1, g.B2, g.lg1, g.Rr, g.sF, tw, nmax >> g.lg1);
+      C* F = lds_fft<R, -1>(g.B1, g.B2, g.lg1, g.Rr, g.sF, g.tw1, 1);
       C* Fo = (F == g.B1) ? g.B2 : g.B1;
       auto disperse = [&](int idx, double beta, R al) {
         const int k2l = idx >> g.lg1, k1i = idx & (n1 - 1);
@@ -787,6 +788,15 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
     g.B2 = lds + 2 * bs;
     g.B3 = lds + 3 * bs;  // RK23 only (the launch sizes LDS for 5 buffers then)
     g.B4 = lds + 4 * bs;
+    // transform twiddles of both lengths after the field buffers (the launch
+    // sizes LDS for nbuf * bs + n1 + n2 elements)
+    C* twl = lds + (d.method == METHOD_RK23 ? 5 : 3) * bs;
+    const C* twg = (const C*)a.twiddle;
+    const int s1 = a.nmax >> d.lg1, s2 = a.nmax >> d.lg2;
+    for (int j = OFX_TID; j < d.n1 + d.n2; j += OFX_NT)
+      twl[j] = j < d.n1 ? twg[j * s1] : twg[(j - d.n1) * s2];
+    g.tw1 = twl;
+    g.tw2 = twl + d.n1;
   }
   const int n = g.n, n1 = g.n1, n2 = g.n2;
   const C* tw = (const C*)a.twiddle;
@@ -933,7 +943,7 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
       int cend = chunk;  // chunks [chunk, cend) end with this step
       while ((double)cend <= nChunks && !(knew < cend * sel && knew <= nsteps)) cend++;
       // frequency phase: rows, dispersion, snapshots, inverse rows
-      C* F = lds_fft<R, -1>(g.B1, g.B2, g.lg1, g.Rr, g.sF, tw, nmax >> g.lg1);
+      C* F = lds_fft<R, -1>(g.B1, g.B2, g.lg1, g.Rr, g.sF, g.tw1, 1);
       C* Fo = (F == g.B1) ? g.B2 : g.B1;
       auto disperse = [&](int idx, double beta, R al) {
         const int k2l = idx >> g.lg1, k1 = idx & (n1 - 1);

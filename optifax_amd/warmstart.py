@@ -264,39 +264,50 @@ def train_surrogate(x, y, steps, batch=512, lr=1e-3, width=2048, seed=0, val_fra
 # ---------------------------------------------------------------------------
 # solves
 # ---------------------------------------------------------------------------
-def _run_to_convergence(run, n_inst, conv_tol, budget, chunk, log, tag):
+def _run_to_convergence(run, n_inst, conv_tol, budget, chunk, log, tag, t_start, max_seconds):
+    """Launch chunks of trips until every instance converged (or diverged),
+    the trip budget or max_seconds is reached.  Returns the wall time (from
+    t_start) at the end of the chunk in which each instance converged (inf if
+    it did not)."""
     done = 0
-    t0 = time.time()
+    conv_at = np.full(n_inst, np.inf)
     while done < budget:
         k = min(chunk, budget - done)
         run.launch(k, conv_tol)
         run.sync(allow_nonfinite=True)
         done += k
-        st = [run.read(i)["status"] for i in range(n_inst)]
-        nconv = sum(1 for x in st if x == 2)
+        now = time.time() - t_start
+        st = np.array([run.read(i)["status"] for i in range(n_inst)])
+        newly = (st == 2) & ~np.isfinite(conv_at)
+        conv_at[newly] = now
         if log:
-            log(f"  {tag}: {done} trips launched, {nconv}/{n_inst} converged, "
-                f"{time.time() - t0:.1f} s")
-        if all(x in (2, 3) for x in st):
+            log(f"  {tag}: {done} trips launched, {int((st == 2).sum())}/{n_inst} converged, "
+                f"{now:.1f} s")
+        if np.all((st == 2) | (st == 3)) or (max_seconds and now > max_seconds):
             break
-    return time.time() - t0
+    return conv_at
 
 
-def solve_plain(insts, conv_tol=1e-4, budget=1000, chunk=25, log=print, ctx=None):
+def solve_plain(insts, conv_tol=1e-4, budget=1000, chunk=10, log=print, ctx=None,
+                max_seconds=None):
     """Plain FP32 round trips from each instance's own initial field until
-    converged (or the budget).  Returns dict(seconds, trips[], status[])."""
+    converged (or the budget).  Returns dict(seconds, conv_seconds[] (time to
+    steady state per instance, inf if not reached), trips[], status[], ...)."""
+    t0 = time.time()
     run = OPOBatch(insts, trip_cap=budget, ctx=ctx)
-    secs = _run_to_convergence(run, len(insts), conv_tol, budget, chunk, log, "plain")
+    conv_at = _run_to_convergence(run, len(insts), conv_tol, budget, chunk, log, "plain", t0,
+                                  max_seconds)
     rs = [run.read(i) for i in range(len(insts))]
     run.close()
-    return dict(seconds=secs, trips=np.array([r["trips_done"] for r in rs]),
+    return dict(seconds=time.time() - t0, conv_seconds=conv_at,
+                trips=np.array([r["trips_done"] for r in rs]),
                 status=np.array([r["status"] for r in rs]),
                 fields=[r["field"] for r in rs], oc=[r["oc_spec"] for r in rs])
 
 
 def solve_warm(cfgs, insts, builds, sur: BandSurrogate, conv_tol=1e-4, budget=1000,
                max_surrogate_trips=100, surrogate_tol=1e-3, judge_tol=0.2, seed_rel=1e-3,
-               chunk=25, log=print, ctx=None):
+               chunk=10, log=print, ctx=None, max_seconds=None):
     """Surrogate trips on the band bins, handover, judged plain trips.
     Returns dict(seconds, surrogate_seconds, surrogate_trips[], trips[],
     status[], judge_err[], fallback[])."""
@@ -347,10 +358,12 @@ def solve_warm(cfgs, insts, builds, sur: BandSurrogate, conv_tol=1e-4, budget=10
     if log:
         log(f"  warm: {t_sur:.2f} s of surrogate trips (max {s_trips.max()}), judge error "
             f"median {np.median(jerr):.3f}, fallbacks {int(fallback.sum())}/{B}")
-    secs = _run_to_convergence(run, B, conv_tol, budget - 1, chunk, log, "warm")
+    conv_at = _run_to_convergence(run, B, conv_tol, budget - 1, chunk, log, "warm", t0,
+                                  max_seconds)
     rs = [run.read(i) for i in range(B)]
     run.close()
-    return dict(seconds=time.time() - t0, surrogate_seconds=t_sur, surrogate_trips=s_trips,
+    return dict(seconds=time.time() - t0, conv_seconds=conv_at, surrogate_seconds=t_sur,
+                surrogate_trips=s_trips,
                 trips=np.array([r["trips_done"] for r in rs]),
                 status=np.array([r["status"] for r in rs]), judge_err=jerr, fallback=fallback,
                 fields=[r["field"] for r in rs], oc=[r["oc_spec"] for r in rs])
