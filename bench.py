@@ -35,8 +35,10 @@ def algorithmic_bytes(n, accepted, rejected, trips):
 
 
 def workload():
+    """Config 2: the output coupler is Optic("R", 0.05) over the whole band
+    (intracavity sqrt(0.95), output sqrt(0.05)), the delay 0.3 ps per trip."""
     from optifax_amd import physics as ph
-    cfg = ph.OPOConfig()
+    cfg = ph.OPOConfig(oc_band=None)
     return cfg, cfg.build()
 
 
@@ -57,14 +59,15 @@ def measured_traffic(key, adaptive_steps_per_launch):
 
 def batch_workload(n_inst=256, seed=3):
     """256 uneven OPO instances drawn from a seed: poling period 28-32 um,
-    crystal length 1-10 mm, grid 2^14-2^16 (window scaled with the grid),
-    detuning +-50 fs around the synchronous delay."""
+    crystal length 1-10 mm, grid 2^14-2^16 at a fixed dt of 12 ps / 2^15
+    (windows 6 / 12 / 24 ps), detuning +-50 fs around the synchronous delay,
+    one progress chunk per pass (sel = NSteps)."""
     from optifax_amd import physics as ph
     rng = np.random.default_rng(seed)
     insts, meta = [], []
     for i in range(n_inst):
         lg = int(rng.integers(14, 17))
-        cfg = ph.OPOConfig(n=2 ** lg, window=12e-12 * 2 ** (lg - 15) if lg < 15 else 12e-12,
+        cfg = ph.OPOConfig(n=2 ** lg, window=12e-12 * 2.0 ** (lg - 15), progress_plots=2,
                            period=float(rng.uniform(28e-6, 32e-6)),
                            length=float(rng.uniform(1e-3, 10e-3)),
                            delay=float(rng.uniform(-50e-15, 50e-15)), delay_mode="sync",
@@ -370,7 +373,8 @@ def main():
         "data": "synthetic (sech^2 pump + seeded one-photon-per-mode noise)",
         "config": {
             "workload": "synchronously pumped ring OPO, N=2^15, 12 ps window, 1040 nm; 5 mm "
-                        "MgO:PPLN 30.5 um; 2 W @ 1 GHz 200 fs sech^2; 5% OC; 0.3 ps detuning",
+                        "MgO:PPLN 30.5 um; 2 W @ 1 GHz 200 fs sech^2; 5% OC (flat R 0.95); "
+                        "0.3 ps delay",
             "instances": 1,
             "adaptive_steps_per_trip": att / args.steps,
             "rejected_per_trip": rej / args.steps,
@@ -458,7 +462,8 @@ def main_batch(args):
         "data": "synthetic (sech^2 pump + seeded one-photon-per-mode noise)",
         "config": {
             "workload": "256 uneven OPO instances in one persistent launch: period 28-32 um, "
-                        "length 1-10 mm, grid 2^14-2^16, detuning +-50 fs (seed 3)",
+                        "length 1-10 mm, grid 2^14-2^16 at dt = 12 ps/2^15, detuning +-50 fs "
+                        "(seed 3)",
             "instances": len(insts),
             "adaptive_steps_per_instance_trip": att / inst_trips,
             "tune": args.tune,

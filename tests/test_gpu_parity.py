@@ -160,12 +160,13 @@ def test_rk23_pass_fp64_exact(gpu_ctx, n, C, window):
 
 def test_rk23_config1_grid_fp64_and_fp32(gpu_ctx):
     """Config-1 grid and pump (2^13 points, 10 ps, 1040 nm, 200 fs sech^2,
-    d_eff 14 pm/V, 5 um^2) over the first 0.2 mm of the 1 mm crystal at 100 pJ
+    d_eff 14 pm/V, 5 um^2) with its RK23 tolerance 1e-6 (maxErr 1e-4 %,
+    minErr 5e-6 %) over the first 0.2 mm of the 1 mm crystal at 100 pJ
     (1 nJ in 5 um^2 drives the FP64 oracle itself to NaN)."""
     sw, si, E, args = crystal_case(n=8192, window=10e-12, L=0.2e-3, energy=1e-10, area=5e-12,
                                    period=30e-6)
     args = list(args)
-    args[8], args[9] = 1e-2, 5e-4
+    args[8], args[9] = 1e-4, 5e-6
     Eo, apo, smo, cnt = oracle.rk23(E, *args, n_cols=4)
     Eg, apg, smg = _run_rk(gpu_ctx, E, args, 8192, "double")
     assert np.array_equal(smg, smo)
