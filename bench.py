@@ -140,10 +140,10 @@ def main_warmstart(args):
                        converged_plain=int((plain["status"] == 2).sum()))
     if args.ws_phase in ("all", "warm"):
         t0 = time.perf_counter()
-        X, F, pe, pr, sb = ws.generate_pairs(training_configs(args.ws_train_instances),
-                                             args.ws_train_trips, log=log)
+        X, F, pe, pr, lin, sb = ws.generate_pairs(training_configs(args.ws_train_instances),
+                                                  args.ws_train_trips, log=log)
         t_data = time.perf_counter() - t0
-        x, y, a_ref = ws.make_dataset(X, F, pe, pr)
+        x, y, a_ref = ws.make_dataset(X, F, pe, pr, lin)
         t0 = time.perf_counter()
         net, losses, val, ms = ws.train_surrogate(x, y, args.ws_train_steps, log=log)
         t_train = time.perf_counter() - t0

@@ -36,20 +36,24 @@ def test_features_round_trip_and_phase_equivariance():
     sb = ws.band_bins(sw)
     rng = np.random.default_rng(0)
     _, Xb = _pulse_bins(sb, sw, rng)
-    a_ref = np.abs(Xb).max()
-    x, th = ws.input_features(Xb, 4e-9, 30.5e-6, a_ref)
+    a_ref = 10 * np.linalg.norm(Xb)
+    x, th, s = ws.input_features(Xb, 4e-9, 30.5e-6, a_ref)
     assert x.shape == (1, ws.N_IN)
     assert x[0, -2] == pytest.approx(1.0) and x[0, -1] == pytest.approx(0.5)
+    assert x[0, -3] == pytest.approx(np.log10(0.1) / 4, rel=1e-6)
+    assert np.linalg.norm(x[0, :-3]) == pytest.approx(1.0, rel=1e-5)   # scale-free field part
     # a global phase rotation changes only the canonical angle
-    x2, th2 = ws.input_features(Xb * np.exp(1.234j), 4e-9, 30.5e-6, a_ref)
+    x2, th2, s2 = ws.input_features(Xb * np.exp(1.234j), 4e-9, 30.5e-6, a_ref)
     assert np.allclose(x2, x, atol=2e-6)
     assert math.remainder(th2[0] - th[0] - 1.234, 2 * math.pi) == pytest.approx(0, abs=1e-9)
-    # output features: exact round trip where the amplitude is significant
-    y = ws.output_features(Xb, th, a_ref)
-    Fb = ws.from_output_features(y, th, a_ref)[0]
-    occ = np.abs(Xb) >= 1e-3 * np.abs(Xb).max()
-    assert np.linalg.norm((Fb - Xb)[occ]) / np.linalg.norm(Xb) < 1e-6
-    assert np.linalg.norm(Fb - Xb) / np.linalg.norm(Xb) < 2e-3
+    # output features: exact round trip through the linear-propagation residual
+    lin = np.exp(1j * rng.uniform(0, 6, sb.nbins)) * rng.uniform(0.5, 1.0, sb.nbins)
+    F = lin * Xb + 0.1 * Xb * np.abs(Xb) / np.abs(Xb).max()
+    y = ws.output_features(F, Xb, lin, th, s)
+    Fb = ws.from_output_features(y, Xb, lin, th, s)[0]
+    assert np.linalg.norm(Fb - F) / np.linalg.norm(F) < 1e-6
+    # the residual of a purely linear pass is zero
+    assert np.abs(ws.output_features(lin * Xb, Xb, lin, th, s)).max() < 1e-6
 
 
 def test_bin_cavity_matches_full_grid_operators():
@@ -109,8 +113,9 @@ def test_anderson_band_solves_a_slow_rotating_contraction():
     class Cav:
         pump = np.zeros((B, n), complex)
         dphase = np.exp(0.7j) * np.ones((B, n))
+        lin = np.ones((B, n), complex)
 
-    def sur(X, pe, pr):
+    def sur(X, pe, pr, lin):
         out = []
         for x, k in zip(X, pe.astype(int)):
             th = ws.canonical_phase(x[None])[0]
