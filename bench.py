@@ -147,7 +147,7 @@ def main_warmstart(args):
         t0 = time.perf_counter()
         net, losses, val, ms = ws.train_surrogate(x, y, args.ws_train_steps, log=log)
         t_train = time.perf_counter() - t0
-        sur = ws.BandSurrogate(net, a_ref)
+        sur = ws.BandSurrogate(net, a_ref, max_norm=float(np.max(np.linalg.norm(X - 0, axis=1))))
         warm = ws.solve_warm(cfgs, insts, builds, sur, conv_tol=1e-4, budget=1000, log=log,
                              max_seconds=args.ws_max_seconds)
         cfg_out.update(warm_seconds=warm["seconds"], surrogate_seconds=warm["surrogate_seconds"],
@@ -298,7 +298,7 @@ def main():
     ap.add_argument("--ws-max-seconds", type=float, default=900.0,
                     help="stop each solve after this many seconds (results so far are reported)")
     ap.add_argument("--ws-train-trips", type=int, default=250)
-    ap.add_argument("--ws-train-steps", type=int, default=3000)
+    ap.add_argument("--ws-train-steps", type=int, default=20000)
     ap.add_argument("--ws-out", default="")
     ap.add_argument("--ws-phase", default="all", choices=["all", "plain", "warm"])
     ap.add_argument("--workload", default="ring",
