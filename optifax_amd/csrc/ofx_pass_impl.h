@@ -140,94 +140,15 @@ OFX_DEV cpx<R> k_term(cpx<R> nl, cpx<R> up, cpx<R> dn, R g, R cc) {
 }
 
 // ------------------------------------------------------------------------
-// Small DFTs in registers: radix-2 decimation in time over RAD = 2..16
-// points, fully unrolled.  w16[m] = exp(-2 pi i m / 16) for m < 8 (the
-// inverse conjugates); factors 1 and -i are applied without multiplies.
+// Batched Stockham FFT in LDS.  nseq sequences of length L = 2^lg, element e
+// of sequence q at buf[q*ss + e]: radix-4 passes (one radix-2 pass for odd
+// lg).  Ping-pongs between `in` and `out`; returns the buffer that holds the
+// result.  SIGN = -1 forward (unnormalised), +1 inverse (unnormalised).
+// tw[m] = exp(-2 pi i m / M), tws = M / L.  (Measured on the ring workload:
+// radix-8 passes with register DFTs were 6 % slower per step.)
 // ------------------------------------------------------------------------
 template <typename R, int SIGN>
-OFX_DEV cpx<R> tw16(cpx<R> v, int m) {
-  // v * exp(SIGN * 2 pi i m / 16), 0 <= m < 8
-  const R c1 = R(0.92387953251128675613), s1 = R(0.38268343236508977173);
-  const R h = R(0.70710678118654752440);
-  R cr, ci;
-  switch (m) {
-    case 0: return v;
-    case 4: return SIGN < 0 ? mkc<R>(v.y, -v.x) : mkc<R>(-v.y, v.x);
-    case 1: cr = c1; ci = s1; break;
-    case 2: cr = h; ci = h; break;
-    case 3: cr = s1; ci = c1; break;
-    case 5: cr = -s1; ci = c1; break;
-    case 6: cr = -h; ci = h; break;
-    default: cr = -c1; ci = s1; break;
-  }
-  // w = cr + i * SIGN * ci
-  const R wi = SIGN < 0 ? -ci : ci;
-  return mkc<R>(v.x * cr - v.y * wi, v.x * wi + v.y * cr);
-}
-
-template <int RAD>
-OFX_DEV constexpr int bitrev_small(int x) {
-  int r = 0;
-  for (int b = 1, c = RAD >> 1; b < RAD; b <<= 1, c >>= 1)
-    if (x & b) r |= c;
-  return r;
-}
-
-template <typename R, int RAD, int SIGN>
-OFX_DEV void dft_reg(cpx<R>* v) {
-  cpx<R> a[RAD];
-#pragma unroll
-  for (int i = 0; i < RAD; i++) a[i] = v[bitrev_small<RAD>(i)];
-#pragma unroll
-  for (int len = 2; len <= RAD; len <<= 1) {
-#pragma unroll
-    for (int i = 0; i < RAD; i += len) {
-#pragma unroll
-      for (int j = 0; j < len / 2; j++) {
-        const cpx<R> u = a[i + j];
-        const cpx<R> t = tw16<R, SIGN>(a[i + j + len / 2], j * (16 / len));
-        a[i + j] = cadd(u, t);
-        a[i + j + len / 2] = csub(u, t);
-      }
-    }
-  }
-#pragma unroll
-  for (int i = 0; i < RAD; i++) v[i] = a[i];
-}
-
-// One radix-RAD Stockham pass over nseq sequences of length L = 2^lg.
-template <typename R, int RAD, int SIGN>
-OFX_DEV void stockham_pass(const cpx<R>* in, cpx<R>* out, int lg, int nseq, int ss,
-                           const cpx<R>* tw, int tws, int Ns) {
-  typedef cpx<R> C;
-  const int L = 1 << lg;
-  const int q1 = L / RAD;
-  const int nb = nseq * q1;
-  const int span = (L / (RAD * Ns)) * tws;
-  for (int t = OFX_TID; t < nb; t += OFX_NT) {
-    const int q = t / q1, j = t - q * q1;
-    const int k = j & (Ns - 1);
-    const C* ib = in + q * ss;
-    C* ob = out + q * ss;
-    C v[RAD];
-#pragma unroll
-    for (int r = 0; r < RAD; r++) v[r] = ib[j + r * q1];
-    if (Ns > 1) {
-      const int m = k * span;
-#pragma unroll
-      for (int r = 1; r < RAD; r++) v[r] = SIGN > 0 ? cmulc(v[r], tw[r * m]) : cmul(v[r], tw[r * m]);
-    }
-    dft_reg<R, RAD, SIGN>(v);
-    const int base = (j - k) * RAD + k;
-#pragma unroll
-    for (int r = 0; r < RAD; r++) ob[base + r * Ns] = v[r];
-  }
-}
-
-// Radix-4 passes (one radix-2 pass for odd lg) with the butterfly written
-// out; the FP64 transforms use it (fewer live registers than the generic pass).
-template <typename R, int SIGN>
-OFX_DEV cpx<R>* lds_fft_r4(cpx<R>* in, cpx<R>* out, int lg, int nseq, int ss, const cpx<R>* tw,
+OFX_DEV cpx<R>* lds_fft(cpx<R>* in, cpx<R>* out, int lg, int nseq, int ss, const cpx<R>* tw,
                         int tws) {
   typedef cpx<R> C;
   const int L = 1 << lg;
@@ -288,41 +209,6 @@ OFX_DEV cpx<R>* lds_fft_r4(cpx<R>* in, cpx<R>* out, int lg, int nseq, int ss, co
       in = out;
       out = tmp;
     }
-  }
-  return in;
-}
-
-// ------------------------------------------------------------------------
-// Batched Stockham FFT in LDS.  nseq sequences of length L = 2^lg, element e
-// of sequence q at buf[q*ss + e].  FP32: passes of radix 8 (bounded by
-// registers), then one of 4 / 2 for the remaining bits; each pass is one
-// LDS read, a register DFT and one LDS write.  Ping-pongs between `in` and
-// `out`; returns the buffer that holds the result.  SIGN = -1 forward
-// (unnormalised), +1 inverse (unnormalised).  tw[m] = exp(-2 pi i m / M),
-// tws = M / L.
-// ------------------------------------------------------------------------
-template <typename R, int SIGN>
-OFX_DEV cpx<R>* lds_fft(cpx<R>* in, cpx<R>* out, int lg, int nseq, int ss, const cpx<R>* tw,
-                        int tws) {
-  if constexpr (sizeof(R) == 8) {
-    return lds_fft_r4<R, SIGN>(in, out, lg, nseq, ss, tw, tws);
-  }
-  constexpr int BIG = 3;  // log2 of the largest radix
-  int Ns = 1, rem = lg;
-  while (rem > 0) {
-    int b = rem >= BIG ? BIG : rem;
-    if (b == 3)
-      stockham_pass<R, 8, SIGN>(in, out, lg, nseq, ss, tw, tws, Ns);
-    else if (b == 2)
-      stockham_pass<R, 4, SIGN>(in, out, lg, nseq, ss, tw, tws, Ns);
-    else
-      stockham_pass<R, 2, SIGN>(in, out, lg, nseq, ss, tw, tws, Ns);
-    OFX_SYNC();
-    cpx<R>* tmp = in;
-    in = out;
-    out = tmp;
-    Ns <<= b;
-    rem -= b;
   }
   return in;
 }
