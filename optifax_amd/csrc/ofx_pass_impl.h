@@ -26,6 +26,7 @@
 #pragma once
 #include "ofx_internal.h"
 #include <math.h>
+#include <float.h>
 
 #ifdef OFX_EMU
 #include <cmath>
@@ -312,29 +313,31 @@ OFX_DEV double xldd(const double* p, int coh) {
 
 OFX_DEV void publish_slots(const ClusterDesc& cd, int par, int rank, int op, double a, double b,
                            double c, int coh) {
+  // op 0 (max) carries one value (the error quotient, +inf for a non-finite
+  // field); op 1 (sums) carries three
   double v[3] = {a, b, c};
+  const int nv = op ? 3 : 1;
 #ifdef OFX_EMU
   const int w = 0;
 #else
-  for (int i = 0; i < 3; i++)
+  for (int i = 0; i < nv; i++)
     for (int o = 32; o > 0; o >>= 1) v[i] = red_op(v[i], __shfl_xor(v[i], o), op);
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) != 0) return;
 #endif
   double* sl = cd.slots + ((size_t)(par * cd.C + rank) * OFX_SLOT_WAVES + w) * 4;
-  xstd(sl + 0, v[0], coh);
-  xstd(sl + 1, v[1], coh);
-  xstd(sl + 2, v[2], coh);
+  for (int i = 0; i < nv; i++) xstd(sl + i, v[i], coh);
 }
 
 OFX_DEV void gather_slots(const ClusterDesc& cd, int par, int op, double* out, double* red,
                           int coh) {
 #ifdef OFX_EMU
   (void)red;
+  const int nv = op ? 3 : 1;
   for (int i = 0; i < 3; i++) out[i] = op ? 0.0 : -1.0;
   for (int q = 0; q < cd.C; q++) {
     const double* sl = cd.slots + ((size_t)(par * cd.C + q) * OFX_SLOT_WAVES) * 4;
-    for (int i = 0; i < 3; i++) out[i] = red_op(out[i], xldd(sl + i, coh), op);
+    for (int i = 0; i < nv; i++) out[i] = red_op(out[i], xldd(sl + i, coh), op);
   }
 #else
   const int nw = (int)(blockDim.x >> 6);
@@ -457,7 +460,8 @@ OFX_DEV cpx<R>* t1_to_rows_fwd(const Geo<R>& g, const cpx<R>* t1, cpx<R>* dst, c
 // After the first barrier of an attempt: read the cluster's error records
 // and this workgroup's t1 rows (into B1; A0 in B0 stays intact for a
 // rejection) with the loads of both in flight together, then reduce the
-// records (max).  out = {max error quotient, non-finite flag, -}.
+// records (max).  out[0] = max error quotient (+inf when any member saw a
+// non-finite field).
 template <typename R>
 OFX_DEV void gather_err_and_rows(const ClusterDesc& cd, int par, const Geo<R>& g,
                                  const cpx<R>* t1, double* out, double* red) {
@@ -471,27 +475,24 @@ OFX_DEV void gather_err_and_rows(const ClusterDesc& cd, int par, const Geo<R>& g
 #else
   const int nw = (int)(blockDim.x >> 6);
   const int tot = cd.C * nw;
-  double v[3] = {-1.0, -1.0, -1.0};
+  double v = -1.0;
   for (int t = threadIdx.x; t < tot; t += blockDim.x) {
     const int q = t / nw, w = t - q * nw;
     const double* sl = cd.slots + ((size_t)(par * cd.C + q) * OFX_SLOT_WAVES + w) * 4;
-    for (int i = 0; i < 3; i++) v[i] = red_op(v[i], xldd(sl + i, g.coh), 0);
+    v = red_op(v, xldd(sl, g.coh), 0);
   }
   for (int idx = threadIdx.x; idx < totR; idx += blockDim.x) {
     const int k2l = idx >> g.lg1, c = idx & (g.n1 - 1);
     g.B1[k2l * g.sF + c] = xld(&t1[(g.k20 + k2l) * g.n1 + c], g.coh);
   }
-  for (int i = 0; i < 3; i++)
-    for (int o = 32; o > 0; o >>= 1) v[i] = red_op(v[i], __shfl_xor(v[i], o), 0);
+  for (int o = 32; o > 0; o >>= 1) v = red_op(v, __shfl_xor(v, o), 0);
   __syncthreads();
-  if ((threadIdx.x & 63) == 0)
-    for (int i = 0; i < 3; i++) red[3 * (threadIdx.x >> 6) + i] = v[i];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
-  for (int i = 0; i < 3; i++) {
-    double r = red[i];
-    for (int w = 1; w < nw; w++) r = red_op(r, red[3 * w + i], 0);
-    out[i] = r;
-  }
+  double r = red[0];
+  for (int w = 1; w < nw; w++) r = red_op(r, red[w], 0);
+  out[0] = r;
+  out[1] = out[2] = 0.0;
 #endif
 }
 
@@ -657,7 +658,7 @@ OFX_DEV int rk23_pass(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc
         OFX_SYNC();
       }
       const int par = (int)(nbar & 1u);
-      publish_slots(cd, par, rank, 0, lmax, (double)bad, 0.0, g.coh);
+      publish_slots(cd, par, rank, 0, bad ? INFINITY : fmin(lmax, DBL_MAX), 0.0, 0.0, g.coh);
       for (int idx = OFX_TID; idx < 6 * n2; idx += OFX_NT) {
         const int sc3 = idx >> g.lg2, r = idx & (n2 - 1);
         const int side = sc3 / 3, col = sc3 - 3 * side;
@@ -671,7 +672,7 @@ OFX_DEV int rk23_pass(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc
       double gr[3];
       gather_err_and_rows<R>(cd, par, g, t1, gr, red);
       att++;
-      if (gr[1] > 0.0) return ST_NONFINITE;
+      if (isinf(gr[0])) return ST_NONFINITE;
       const double pmax = gr[0] < 0.0 ? -1.0 : 4.0 * 100.0 * gr[0];
       const bool flag0 = pmax >= d.max_err;
       const bool flag1 = !flag0 && pmax >= d.min_err;
@@ -911,7 +912,7 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
         corrector(idx, ab[gidx(g, g.HW + (idx >> g.lg2), idx & (n2 - 1))]);
       OFX_SYNC();
       const int par = (int)(nbar & 1u);
-      publish_slots(cd, par, rank, 0, lmax, (double)bad, 0.0, g.coh);
+      publish_slots(cd, par, rank, 0, bad ? INFINITY : fmin(lmax, DBL_MAX), 0.0, 0.0, g.coh);
       // speculative forward column transforms of E
       fwd_cols_to_t1<R>(g, X, K, t1, tw, nmax);
       nbar++;
@@ -921,10 +922,11 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
       }
       double gr[3];
       gather_err_and_rows<R>(cd, par, g, t1, gr, red);
-      const double gmax = gr[0], gbad = gr[1];
+      const double gmax = gr[0];
+      const bool gbad = isinf(gmax);
       attempts++;
       trip_att++;
-      if (gbad > 0.0) {
+      if (gbad) {
         status = ST_NONFINITE;
         break;
       }
