@@ -81,6 +81,9 @@ typedef struct {
   int cluster;    /* workgroups per instance (power of two)  */
   int n1;         /* four-step split N = n1*n2 (power of two) */
   int block;      /* threads per workgroup (64..512)          */
+  int exchange;   /* how a workgroup learns that its cluster's exchange data
+                     are complete: 1 poll the data's own tags, 2 a counter
+                     barrier first; 0 auto (tags from 64 workgroups up) */
 } ofx_tuning;
 
 int ofx_create(int device, ofx_context** out);
@@ -158,6 +161,11 @@ int ofx_opo_geometry(const ofx_opo* run, int inst, int* cluster, int* n1, int* b
                      int* lds_bytes);
 /* hardware XCC id of every launched workgroup of the last launch (diagnostic) */
 int ofx_opo_xcc_ids(ofx_opo* run, int* ids, int cap, int* count);
+/* longest time one workgroup waits for its cluster's exchange data before
+ * the cluster gives up (default 10 s); a launch that gives up leaves the
+ * instance's field unchanged and ofx_opo_sync returns OFX_E_TIMEOUT.  No
+ * reference counterpart (the reference's solver has no cross-block waits). */
+int ofx_opo_set_wait_limit(ofx_opo* run, double seconds);
 
 /* ---- spectral surrogate of one crystal pass (MLP, BF16 MFMA / FP32) ------
  * No reference counterpart: the reference has no learned component.  Network

@@ -47,7 +47,7 @@ EXPORTED = [
     "ofx_crystal_pass_host", "ofx_crystal_pass_host_f64",
     "ofx_opo_create", "ofx_opo_destroy", "ofx_opo_launch", "ofx_opo_sync",
     "ofx_opo_graph_replay", "ofx_opo_last_ms", "ofx_opo_read", "ofx_opo_write_field",
-    "ofx_opo_geometry", "ofx_opo_xcc_ids",
+    "ofx_opo_geometry", "ofx_opo_xcc_ids", "ofx_opo_set_wait_limit",
     "ofx_mlp_create", "ofx_mlp_destroy", "ofx_mlp_last_error", "ofx_mlp_num_params",
     "ofx_mlp_set_params", "ofx_mlp_get_params", "ofx_mlp_forward", "ofx_mlp_gradients",
     "ofx_mlp_step", "ofx_mlp_upload_dataset", "ofx_mlp_train", "ofx_mlp_last_ms",
@@ -68,7 +68,8 @@ class PassParams(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("cluster", ctypes.c_int), ("n1", ctypes.c_int), ("block", ctypes.c_int)]
+    _fields_ = [("cluster", ctypes.c_int), ("n1", ctypes.c_int), ("block", ctypes.c_int),
+                ("exchange", ctypes.c_int)]
 
 
 _DP = ctypes.POINTER(ctypes.c_double)
@@ -119,6 +120,7 @@ def load(path: str | None = None):
     L.ofx_opo_write_field.argtypes = [vp, ctypes.c_int, _DP]
     L.ofx_opo_geometry.argtypes = [vp, ctypes.c_int, _IP, _IP, _IP, _IP, _IP]
     L.ofx_opo_xcc_ids.argtypes = [vp, _IP, ctypes.c_int, _IP]
+    L.ofx_opo_set_wait_limit.argtypes = [vp, ctypes.c_double]
     if hasattr(L, "ofx_mlp_create"):   # the CPU emulation library has no surrogate
         L.ofx_mlp_create.argtypes = [ctypes.c_int] * 5 + [ctypes.POINTER(vp)]
         L.ofx_mlp_destroy.argtypes = [vp]

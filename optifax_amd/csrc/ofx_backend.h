@@ -20,6 +20,7 @@ struct Backend;
 int be_create(int device, Backend** out, DeviceInfo* info);
 void be_destroy(Backend* b);
 const char* be_error(Backend* b);
+// zero-filled device memory (stream-ordered)
 int be_malloc(Backend* b, void** p, size_t bytes);
 void be_free(Backend* b, void* p);
 // stream-ordered copies (the host buffer must stay valid until be_sync)
@@ -27,14 +28,16 @@ int be_h2d(Backend* b, void* dst, const void* src, size_t bytes);
 int be_d2h(Backend* b, void* dst, const void* src, size_t bytes);
 int be_sync(Backend* b);
 // workgroups of the pass kernel that fit on one CU (0 if the config cannot launch)
-int be_blocks_per_cu(Backend* b, int precision, int block, size_t lds_bytes);
-// Enqueue: zero `nzero` words at `zero_ptrs[i]` (graph-safe fill kernel), then the
-// pass kernel.  If replays > 0 the pair is captured once into a graph keyed by
+int be_blocks_per_cu(Backend* b, int precision, int xmode, int block, size_t lds_bytes);
+// Enqueue: zero `nzero` words at `zero_ptrs[i]` and advance the `n_epochs`
+// tag epochs at `epochs` (pairs {base, used}: base += used, used = 0) in one
+// graph-safe fill kernel, then the pass kernel.  If replays > 0 the pair is captured once into a graph keyed by
 // `key` and replayed `replays` times.  Elapsed time of the enqueued span is
 // available from be_last_ms after be_sync.
-int be_launch(Backend* b, const LaunchArgs& a, int precision, int grid, int block, size_t lds,
-              unsigned* const* zero_ptrs, const size_t* zero_words, int nzero, const void* key,
-              int replays);
+// xmode selects the kernel's exchange mode (0 tagged polling, 1 counter barrier).
+int be_launch(Backend* b, const LaunchArgs& a, int precision, int xmode, int grid, int block,
+              size_t lds, unsigned* const* zero_ptrs, const size_t* zero_words, int nzero,
+              unsigned* epochs, int n_epochs, const void* key, int replays);
 int be_last_ms(Backend* b, double* ms);
 void be_forget_graph(Backend* b, const void* key);
 

@@ -14,23 +14,39 @@
 
 namespace ofx {
 
-template <typename R>
+template <typename R, int XM>
 __global__ void __launch_bounds__(512) pass_kernel(LaunchArgs a) {
   extern __shared__ __align__(16) unsigned char ofx_lds_raw[];
   __shared__ double red[64];
   __shared__ int sflag[4];
-  cluster_main<R>(a, (int)blockIdx.x, reinterpret_cast<cpx<R>*>(ofx_lds_raw), red, sflag);
+  cluster_main<R, XM>(a, (int)blockIdx.x, reinterpret_cast<cpx<R>*>(ofx_lds_raw), red, sflag);
+}
+
+// the kernel instantiation for a precision and exchange mode
+static const void* pass_fn(int precision, int xmode) {
+  if (precision == OFX_F64)
+    return xmode == XM_BARRIER ? (const void*)pass_kernel<double, XM_BARRIER>
+                               : (const void*)pass_kernel<double, XM_TAGS>;
+  return xmode == XM_BARRIER ? (const void*)pass_kernel<float, XM_BARRIER>
+                             : (const void*)pass_kernel<float, XM_TAGS>;
 }
 
 // zero scratch words with vector stores (graph-safe replacement for memset)
+// and advance the clusters' tag epochs past the tags of the previous launch
 struct ZeroList {
   unsigned* p[16];
   unsigned w[16];
   int n;
+  unsigned* epochs;  // [n_epochs][2] {base, largest exchange count used}
+  int n_epochs;
 };
 __global__ void zero_kernel(ZeroList z) {
   for (int i = 0; i < z.n; i++)
     for (unsigned j = threadIdx.x; j < z.w[i]; j += blockDim.x) z.p[i][j] = 0u;
+  for (int j = threadIdx.x; j < z.n_epochs; j += blockDim.x) {
+    z.epochs[2 * j] += z.epochs[2 * j + 1];
+    z.epochs[2 * j + 1] = 0u;
+  }
 }
 
 struct Backend {
@@ -92,10 +108,12 @@ int be_create(int device, Backend** out, DeviceInfo* info) {
   // and launch errors report an unusable size later.
   {
     const int dyn = info->lds_max - 1024;
-    if (hipFuncSetAttribute((const void*)pass_kernel<float>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, dyn) != hipSuccess ||
-        hipFuncSetAttribute((const void*)pass_kernel<double>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, dyn) != hipSuccess)
+    bool ok = true;
+    for (int prec : {OFX_F32, OFX_F64})
+      for (int xm : {XM_TAGS, XM_BARRIER})
+        ok = ok && hipFuncSetAttribute(pass_fn(prec, xm), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       dyn) == hipSuccess;
+    if (!ok)
       snprintf(info->name + strlen(info->name), sizeof info->name - strlen(info->name),
                " [dyn-LDS attribute not set]");
     (void)hipGetLastError();  // do not leak this status into later launch checks
@@ -122,6 +140,7 @@ const char* be_error(Backend* b) { return b ? b->err.c_str() : "no backend"; }
 int be_malloc(Backend* b, void** p, size_t bytes) {
   hipSetDevice(b->device);
   HIPCHK(b, hipMalloc(p, bytes));
+  HIPCHK(b, hipMemsetAsync(*p, 0, bytes, b->stream));
   return OFX_OK;
 }
 
@@ -145,42 +164,47 @@ int be_sync(Backend* b) {
   return OFX_OK;
 }
 
-int be_blocks_per_cu(Backend* b, int precision, int block, size_t lds_bytes) {
+int be_blocks_per_cu(Backend* b, int precision, int xmode, int block, size_t lds_bytes) {
   int nb = 0;
-  const void* f = precision == OFX_F64 ? (const void*)pass_kernel<double> : (const void*)pass_kernel<float>;
+  const void* f = pass_fn(precision, xmode);
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, block, lds_bytes) != hipSuccess) return 0;
   (void)b;
   return nb;
 }
 
-static int enqueue(Backend* b, const LaunchArgs& a, int precision, int grid, int block, size_t lds,
-                   unsigned* const* zp, const size_t* zw, int nz) {
+static int enqueue(Backend* b, const LaunchArgs& a, int precision, int xmode, int grid, int block,
+                   size_t lds, unsigned* const* zp, const size_t* zw, int nz, unsigned* ep,
+                   int nep) {
   (void)hipGetLastError();
-  for (int base = 0; base < nz; base += 16) {
+  for (int base = 0; base < nz || base == 0; base += 16) {
     ZeroList z;
     memset(&z, 0, sizeof z);
     z.n = nz - base < 16 ? nz - base : 16;
+    if (z.n < 0) z.n = 0;
     for (int i = 0; i < z.n; i++) {
       z.p[i] = zp[base + i];
       z.w[i] = (unsigned)zw[base + i];
     }
+    if (base == 0) {
+      z.epochs = ep;
+      z.n_epochs = ep ? nep : 0;
+    }
     hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, b->stream, z);
     HIPCHK(b, hipGetLastError());
   }
-  if (precision == OFX_F64)
-    hipLaunchKernelGGL(pass_kernel<double>, dim3(grid), dim3(block), lds, b->stream, a);
-  else
-    hipLaunchKernelGGL(pass_kernel<float>, dim3(grid), dim3(block), lds, b->stream, a);
-  HIPCHK(b, hipGetLastError());
+  void* args[] = {(void*)&a};
+  HIPCHK(b, hipLaunchKernel(pass_fn(precision, xmode), dim3(grid), dim3(block), args, lds,
+                            b->stream));
   return OFX_OK;
 }
 
-int be_launch(Backend* b, const LaunchArgs& a, int precision, int grid, int block, size_t lds,
-              unsigned* const* zp, const size_t* zw, int nz, const void* key, int replays) {
+int be_launch(Backend* b, const LaunchArgs& a, int precision, int xmode, int grid, int block,
+              size_t lds, unsigned* const* zp, const size_t* zw, int nz, unsigned* ep, int nep,
+              const void* key, int replays) {
   hipSetDevice(b->device);
   if (replays <= 0) {
     HIPCHK(b, hipEventRecord(b->ev0, b->stream));
-    int st = enqueue(b, a, precision, grid, block, lds, zp, zw, nz);
+    int st = enqueue(b, a, precision, xmode, grid, block, lds, zp, zw, nz, ep, nep);
     if (st) return st;
     HIPCHK(b, hipEventRecord(b->ev1, b->stream));
     b->timed = true;
@@ -191,7 +215,7 @@ int be_launch(Backend* b, const LaunchArgs& a, int precision, int grid, int bloc
   if (it == b->graphs.end()) {
     hipGraph_t g = nullptr;
     HIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    int st = enqueue(b, a, precision, grid, block, lds, zp, zw, nz);
+    int st = enqueue(b, a, precision, xmode, grid, block, lds, zp, zw, nz, ep, nep);
     hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (st) return st;
     if (e != hipSuccess) return hip_fail(b, e, "hipStreamEndCapture");

@@ -67,13 +67,16 @@ struct ClusterDesc {
   int first_inst;  // offset into the instance-order list
   int n_inst;      // instances processed in sequence
   int halo;        // halo columns each side
-  void* t1;        // N complex transpose buffer (forward), N = largest of the cluster
-  void* t2;        // N complex transpose buffer (inverse)
-  double* slots;   // [2][C][8][4] doubles: per-wave partial results, double-buffered
-  unsigned* bar;   // barrier counter (zeroed before every launch)
-  int* abort_flag; // set when a barrier times out
-  int* xcc;        // C ints: hardware XCC id of each member (placement diagnostic)
-  void* kx;        // RK23 only: [C][2 sides][3 columns][n2] complex, boundary columns of K4
+  // Exchange buffers hold tagged elements: every 32-bit word of a complex
+  // sample as one 64-bit word (word | tag << 32), two parity halves each.
+  void* t1;          // [2][N] tagged complex: forward transpose, N of the cluster
+  void* t2;          // [2][N] tagged complex: inverse transpose
+  void* slots;       // [2][C][8 waves][4] tagged complex-sized records (doubles)
+  unsigned* bar;     // counter barrier (zeroed before every launch)
+  int* abort_flag;   // set when a member gives up waiting
+  int* xcc;          // C ints: hardware XCC id of each member (placement diagnostic)
+  void* kx;          // RK23 only: [2][C][2 sides][3 columns][n2] tagged complex, K4 boundary
+  unsigned* epoch;   // [0] tag base of this launch, [1] largest exchange count used by it
 };
 
 struct LaunchArgs {

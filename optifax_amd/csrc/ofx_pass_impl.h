@@ -29,13 +29,16 @@
 #include <float.h>
 
 #ifdef OFX_EMU
+#include <sched.h>
+
 #include <cmath>
 #define OFX_DEV inline
 #define OFX_TID 0
 #define OFX_NT 1
 #define OFX_SYNC() ((void)0)
 namespace ofx {
-bool emu_cluster_barrier(const ClusterDesc* cd, unsigned target);
+bool emu_cluster_barrier(const ClusterDesc* cd, unsigned target, unsigned long long timeout);
+unsigned long long emu_clock();  // wall clock in the device's ticks
 }
 #else
 #define OFX_DEV __device__ __forceinline__
@@ -215,104 +218,298 @@ OFX_DEV cpx<R>* lds_fft(cpx<R>* in, cpx<R>* out, int lg, int nseq, int ss, const
 }
 
 // ------------------------------------------------------------------------
-// Cluster-wide reductions through the slots.  Every wave reduces its lanes
-// and publishes one record of three doubles (no workgroup reduction before
-// the barrier); after the barrier the C * waves records are read in
-// parallel, one per thread, and reduced.  op 0 = max, op 1 = sum.  The
-// reduction tree is the same in every workgroup, so all members obtain
-// bit-identical results (the step-control decisions depend on it).
-// Slot layout: [parity][rank][wave < 8][4] doubles.
+// Cluster exchange.
+//
+// Workgroups of a cluster exchange data only through tagged buffers: the
+// transposes t1/t2, the per-wave reduction records (slots) and, for RK23, the
+// boundary columns of K4 (kx).  Every 32-bit word of an exchanged value
+// travels in a 64-bit word whose upper half is the exchange's tag; the words
+// are written and read with device-scope relaxed 64-bit atomics (sc1
+// accesses, which bypass the non-coherent caches, so no agent-scope fence and
+// no L2 writeback is involved).  A reader polls the words it needs until each
+// carries the tag it expects: the data is its own completion flag.  An
+// exchange therefore costs the stores' trip to the coherent level plus the
+// reader's poll, with no store-completion wait, counter atomic or flag read
+// in between (the counter barrier this replaces paid all three).
+//
+// Tags: tag = base + x, x = exchanges so far in this launch (every member
+// runs the same sequence of exchanges), base = the cluster's epoch word,
+// moved before each launch past every tag the previous launch used.  Every
+// buffer is double-buffered by the parity of x.  A member rewrites a parity
+// half two exchanges after its previous use, and only after it has read data
+// that every member produced after finishing its reads of that use, so a
+// word still in a buffer always carries an older tag than the one awaited.
+// A poll gives up, and aborts the cluster, after the launch's time limit or
+// when another member has aborted.
+//
+// Cluster-wide reductions go through the slots: every wave reduces its lanes
+// and publishes one record (no workgroup reduction before the exchange); the
+// readers reduce the C * waves records, one per thread.  op 0 = max (one
+// value), op 1 = sum (three values, combined in a fixed order, so all members
+// obtain bit-identical results: the step-control decisions depend on it).
+// Slot layout: [parity][rank][wave < 8][4 values], each value a tagged
+// element the size of one complex sample.
 // ------------------------------------------------------------------------
 #define OFX_SLOT_WAVES 8
+typedef unsigned long long u64;
+
 OFX_DEV double red_op(double a, double b, int op) { return op ? a + b : (a > b ? a : b); }
 
-OFX_DEV void publish_slots(const ClusterDesc& cd, int par, int rank, int op, double a, double b,
-                           double c, int coh);
-OFX_DEV void gather_slots(const ClusterDesc& cd, int par, int op, double* out, double* red,
-                          int coh);
-
-// ------------------------------------------------------------------------
-// Cluster exchange and barrier.
-//
-// Workgroups of a cluster exchange data only through the transpose buffers
-// t1/t2 and the per-rank slots, and synchronise on a monotonic counter
-// (target = C * barriers so far).  No agent-scope fence is used: on gfx950 it
-// writes back and invalidates the XCD's whole L2, and those flushes serialise
-// across the cluster (measured 3-13 us per barrier for 8-32 workgroups).
-// Instead the exchanged words are written and read with device-scope relaxed
-// atomics (sc1 accesses, which bypass the non-coherent caches), each wave
-// waits for its stores to complete before the workgroup barrier, and one
-// thread bumps and polls the counter (about 3.3 us per barrier, flat in C, on
-// one XCD or spread over all eight).  A variant that relied on the shared L2
-// of one XCC (plain accesses plus an L1 invalidate) was faster but returned
-// stale exchange data in some geometries, so it is not used.
-// The barrier returns false when the cluster was aborted (a member timed out).
-// ------------------------------------------------------------------------
-template <typename R>
-OFX_DEV void xst(cpx<R>* p, cpx<R> v, int coh) {
+// (global address space on the device: global_* rather than flat_*
+// instructions, which would also wait on the LDS counter)
+OFX_DEV void tag_store(u64* p, u64 v) {
 #ifdef OFX_EMU
-  (void)coh;
-  *p = v;
+  __atomic_store_n(p, v, __ATOMIC_RELAXED);
 #else
-  if (!coh) {
-    *p = v;
-    return;
+  __hip_atomic_store((__attribute__((address_space(1))) u64*)p, v, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+#endif
+}
+
+OFX_DEV u64 tag_load(const u64* p) {
+#ifdef OFX_EMU
+  return __atomic_load_n(p, __ATOMIC_RELAXED);
+#else
+  return __hip_atomic_load((__attribute__((address_space(1))) u64*)const_cast<u64*>(p),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#endif
+}
+
+OFX_DEV int flag_load(const int* p) {
+#ifdef OFX_EMU
+  return __atomic_load_n(p, __ATOMIC_RELAXED);
+#else
+  return __hip_atomic_load(const_cast<int*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#endif
+}
+
+OFX_DEV void flag_set(int* p) {
+#ifdef OFX_EMU
+  __atomic_store_n(p, 1, __ATOMIC_RELAXED);
+#else
+  __hip_atomic_store(p, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#endif
+}
+
+// A value of type T (a multiple of 4 bytes) as NW tagged 64-bit words.
+template <typename T>
+struct Tagged {
+  static constexpr int NW = (int)(sizeof(T) / 4);
+  struct Words {
+    unsigned w[NW];
+  };
+  // Tagged form: word i of the element at p lives at p + i * ws.  Plain form
+  // (counter-barrier exchanges): the value's own bytes, NW / 2 words at p;
+  // `load` then supplies the expected tag itself.
+  static OFX_DEV void put(u64* dst, size_t ws, int plain, T v, unsigned tag) {
+    const Words x = __builtin_bit_cast(Words, v);
+    if (plain) {
+#pragma unroll
+      for (int i = 0; i < NW / 2; i++)
+        tag_store(dst + i, (u64)x.w[2 * i] | ((u64)x.w[2 * i + 1] << 32));
+      return;
+    }
+    const u64 hi = (u64)tag << 32;
+#pragma unroll
+    for (int i = 0; i < NW; i++) tag_store(dst + i * ws, hi | (u64)x.w[i]);
   }
-  static_assert(sizeof(cpx<R>) % 8 == 0, "complex must be a multiple of 8 bytes");
-  union {
-    cpx<R> c;
-    unsigned long long u[sizeof(cpx<R>) / 8];
-  } x;
-  x.c = v;
-  unsigned long long* q = reinterpret_cast<unsigned long long*>(p);
-  for (int i = 0; i < (int)(sizeof(cpx<R>) / 8); i++)
-    __hip_atomic_store(q + i, x.u[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  static OFX_DEV void load(const u64* src, size_t ws, int plain, unsigned tag, u64* w) {
+    if (plain) {
+      const u64 hi = (u64)tag << 32;
+#pragma unroll
+      for (int i = 0; i < NW / 2; i++) {
+        const u64 q = tag_load(src + i);
+        w[2 * i] = hi | (q & 0xffffffffull);
+        w[2 * i + 1] = hi | (q >> 32);
+      }
+      return;
+    }
+#pragma unroll
+    for (int i = 0; i < NW; i++) w[i] = tag_load(src + i * ws);
+  }
+  static OFX_DEV bool check(const u64* w, unsigned tag) {
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < NW; i++) ok = ok & ((unsigned)(w[i] >> 32) == tag);
+    return ok;
+  }
+  static OFX_DEV T value(const u64* w) {
+    Words x;
+#pragma unroll
+    for (int i = 0; i < NW; i++) x.w[i] = (unsigned)w[i];
+    return __builtin_bit_cast(T, x);
+  }
+};
+
+// Where a tagged element lives: its first word and the distance between its
+// words.  Buffers of complex samples keep word i of all elements in plane i
+// (a plane of P words per parity half), so the stores and loads of one word
+// index cover contiguous memory across the lanes.  In plain form (clusters
+// that meet at a counter barrier before reading) element e is stored
+// untagged at e * NW / 2: half the bytes, which is what bounds the exchange
+// when each workgroup holds a large share of the field.
+struct TagPtr {
+  const u64* p;
+  size_t ws;
+  int plain;
+};
+template <typename T>
+OFX_DEV TagPtr tp_plane(int plain, const u64* half, size_t P, size_t e) {
+  if (plain) return TagPtr{half + e * (Tagged<T>::NW / 2), 1, 1};
+  return TagPtr{half + e, P, 0};
+}
+
+// Poll state of one wait: the limit counts from the first unsuccessful poll.
+struct Spin {
+  int* abort_flag;
+  unsigned long long limit, t0;
+  int n;
+};
+
+OFX_DEV Spin spin_begin(const ClusterDesc& cd, unsigned long long limit) {
+  Spin s;
+  s.abort_flag = cd.abort_flag;
+  s.limit = limit;
+  s.t0 = 0;
+  s.n = 0;
+  return s;
+}
+
+// Pause before the next poll round; false when the cluster is to give up
+// (the clock is read every round, the abort flag every 8th).
+OFX_DEV bool spin_wait(Spin& s) {
+#ifdef OFX_EMU
+  sched_yield();
+  const unsigned long long t = emu_clock();
+  if (s.n++ == 0) s.t0 = t;
+  if (flag_load(s.abort_flag)) return false;
+  if (t - s.t0 > s.limit) {
+    flag_set(s.abort_flag);
+    return false;
+  }
+  return true;
+#else
+  __builtin_amdgcn_s_sleep(1);
+  const unsigned long long t = wall_clock64();
+  if (s.n == 0) s.t0 = t;
+  if ((s.n++ & 7) == 0 && flag_load(s.abort_flag)) return false;
+  if (t - s.t0 > s.limit) {
+    flag_set(s.abort_flag);
+    return false;
+  }
+  return true;
 #endif
+}
+
+// Poll `cnt` tagged elements of type T held by this thread: element j is at
+// src(j) (a TagPtr) and goes to commit(j, value) once all its words carry `tag`.
+// Up to J elements (four words) are in flight at once (more costs registers
+// that the rest of the kernel needs).  False when the cluster gave up.
+template <typename T, typename Src, typename Commit>
+OFX_DEV bool poll_tagged(int cnt, unsigned tag, Src src, Commit commit, Spin& sp) {
+  typedef Tagged<T> TT;
+  constexpr int J = 4 / TT::NW;
+  for (int b = 0; b < cnt; b += J) {
+    u64 w[J][TT::NW];
+    unsigned pend = 0;
+#pragma unroll
+    for (int j = 0; j < J; j++)
+      if (b + j < cnt) {
+        const TagPtr t = src(b + j);
+        TT::load(t.p, t.ws, t.plain, tag, w[j]);
+        pend |= 1u << j;
+      }
+    for (;;) {
+#pragma unroll
+      for (int j = 0; j < J; j++)
+        if (((pend >> j) & 1u) && TT::check(w[j], tag)) {
+          commit(b + j, TT::value(w[j]));
+          pend &= ~(1u << j);
+        }
+      if (!pend) break;
+      if (!spin_wait(sp)) return false;
+#pragma unroll
+      for (int j = 0; j < J; j++)
+        if ((pend >> j) & 1u) {
+          const TagPtr t = src(b + j);
+          TT::load(t.p, t.ws, t.plain, tag, w[j]);
+        }
+    }
+  }
+  return true;
+}
+
+// The same over a workgroup-wide index space [0, tot): this thread takes
+// idx = OFX_TID + j * OFX_NT.  After a counter barrier (`complete`) the data
+// are known to be in place and are read once, without tag checks.
+template <typename T, typename Src, typename Commit>
+OFX_DEV bool read_tagged(int tot, unsigned tag, Src src, Commit commit, Spin& sp, int complete) {
+  if (complete) {
+    typedef Tagged<T> TT;
+    for (int idx = OFX_TID; idx < tot; idx += OFX_NT) {
+      const TagPtr t = src(idx);
+      u64 w[TT::NW];
+      TT::load(t.p, t.ws, t.plain, tag, w);
+      commit(idx, TT::value(w));
+    }
+    return true;
+  }
+  const int cnt = tot > OFX_TID ? (tot - OFX_TID + OFX_NT - 1) / OFX_NT : 0;
+  return poll_tagged<T>(
+      cnt, tag, [&](int j) { return src(OFX_TID + j * OFX_NT); },
+      [&](int j, T v) { commit(OFX_TID + j * OFX_NT, v); }, sp);
+}
+
+// a double carried in a complex-sized tagged element
+template <typename R>
+struct Dbl {
+  double d[sizeof(cpx<R>) / 8];
+};
+template <typename R>
+OFX_DEV cpx<R> pack_d(double v) {
+  Dbl<R> x;
+  for (int i = 0; i < (int)(sizeof(cpx<R>) / 8); i++) x.d[i] = 0.0;
+  x.d[0] = v;
+  return __builtin_bit_cast(cpx<R>, x);
+}
+template <typename R>
+OFX_DEV double unpack_d(cpx<R> c) {
+  return __builtin_bit_cast(Dbl<R>, c).d[0];
+}
+
+OFX_DEV int slot_waves() {
+#ifdef OFX_EMU
+  return 1;
+#else
+  return (int)(blockDim.x >> 6);
+#endif
+}
+
+// Workgroup-wide AND of the per-thread read verdicts.  After a counter
+// barrier the reads cannot fail, and a plain barrier suffices.
+OFX_DEV bool wg_all(bool ok, int complete) {
+#ifdef OFX_EMU
+  (void)complete;
+  return ok;
+#else
+  if (complete) {
+    __syncthreads();
+    return true;
+  }
+  return __syncthreads_and(ok) != 0;
+#endif
+}
+
+// first word of slot value i of (parity, rank, wave)
+template <typename R>
+OFX_DEV u64* slot_at(const ClusterDesc& cd, int par, int rank, int w, int i) {
+  return (u64*)cd.slots +
+         (((size_t)(par * cd.C + rank) * OFX_SLOT_WAVES + w) * 4 + i) * Tagged<cpx<R>>::NW;
 }
 
 template <typename R>
-OFX_DEV cpx<R> xld(const cpx<R>* p, int coh) {
-#ifdef OFX_EMU
-  (void)coh;
-  return *p;
-#else
-  if (!coh) return *p;
-  union {
-    cpx<R> c;
-    unsigned long long u[sizeof(cpx<R>) / 8];
-  } x;
-  unsigned long long* q = const_cast<unsigned long long*>(reinterpret_cast<const unsigned long long*>(p));
-  for (int i = 0; i < (int)(sizeof(cpx<R>) / 8); i++)
-    x.u[i] = __hip_atomic_load(q + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return x.c;
-#endif
-}
-
-OFX_DEV void xstd(double* p, double v, int coh) {
-#ifdef OFX_EMU
-  (void)coh;
-  *p = v;
-#else
-  if (!coh) {
-    *p = v;
-    return;
-  }
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-#endif
-}
-
-OFX_DEV double xldd(const double* p, int coh) {
-#ifdef OFX_EMU
-  (void)coh;
-  return *p;
-#else
-  if (!coh) return *p;
-  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-#endif
-}
-
 OFX_DEV void publish_slots(const ClusterDesc& cd, int par, int rank, int op, double a, double b,
-                           double c, int coh) {
+                           double c, unsigned tag) {
   // op 0 (max) carries one value (the error quotient, +inf for a non-finite
   // field); op 1 (sums) carries three
   double v[3] = {a, b, c};
@@ -325,55 +522,59 @@ OFX_DEV void publish_slots(const ClusterDesc& cd, int par, int rank, int op, dou
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) != 0) return;
 #endif
-  double* sl = cd.slots + ((size_t)(par * cd.C + rank) * OFX_SLOT_WAVES + w) * 4;
-  for (int i = 0; i < nv; i++) xstd(sl + i, v[i], coh);
+  for (int i = 0; i < nv; i++)
+    Tagged<cpx<R>>::put(slot_at<R>(cd, par, rank, w, i), 1, 0, pack_d<R>(v[i]), tag);
 }
 
-OFX_DEV void gather_slots(const ClusterDesc& cd, int par, int op, double* out, double* red,
-                          int coh) {
+// Cluster sums of the three op-1 values.  Each thread adds its records in
+// index order once each is complete; false when the cluster gave up.
+template <typename R>
+OFX_DEV bool gather_sums(const ClusterDesc& cd, int par, unsigned tag, double* out, double* red,
+                         unsigned long long limit) {
+  Spin sp = spin_begin(cd, limit);
+  const int nw = slot_waves();
+  const int tot = cd.C * nw;
+  double v[3] = {0.0, 0.0, 0.0};
+  bool ok = true;
+  for (int t = OFX_TID; t < tot && ok; t += OFX_NT) {
+    const int q = t / nw, w = t - q * nw;
+    double r[3];
+    ok = poll_tagged<cpx<R>>(
+        3, tag, [&](int i) { return TagPtr{slot_at<R>(cd, par, q, w, i), 1, 0}; },
+        [&](int i, cpx<R> x) { r[i] = unpack_d<R>(x); }, sp);
+    if (ok)
+      for (int i = 0; i < 3; i++) v[i] += r[i];
+  }
 #ifdef OFX_EMU
   (void)red;
-  const int nv = op ? 3 : 1;
-  for (int i = 0; i < 3; i++) out[i] = op ? 0.0 : -1.0;
-  for (int q = 0; q < cd.C; q++) {
-    const double* sl = cd.slots + ((size_t)(par * cd.C + q) * OFX_SLOT_WAVES) * 4;
-    for (int i = 0; i < nv; i++) out[i] = red_op(out[i], xldd(sl + i, coh), op);
-  }
+  for (int i = 0; i < 3; i++) out[i] = v[i];
+  return ok;
 #else
-  const int nw = (int)(blockDim.x >> 6);
-  const int tot = cd.C * nw;
-  double v[3];
-  for (int i = 0; i < 3; i++) v[i] = op ? 0.0 : -1.0;
-  for (int t = threadIdx.x; t < tot; t += blockDim.x) {
-    const int q = t / nw, w = t - q * nw;
-    const double* sl = cd.slots + ((size_t)(par * cd.C + q) * OFX_SLOT_WAVES + w) * 4;
-    for (int i = 0; i < 3; i++) v[i] = red_op(v[i], xldd(sl + i, coh), op);
-  }
   for (int i = 0; i < 3; i++)
-    for (int o = 32; o > 0; o >>= 1) v[i] = red_op(v[i], __shfl_xor(v[i], o), op);
+    for (int o = 32; o > 0; o >>= 1) v[i] += __shfl_xor(v[i], o);
   __syncthreads();
   if ((threadIdx.x & 63) == 0)
     for (int i = 0; i < 3; i++) red[3 * (threadIdx.x >> 6) + i] = v[i];
-  __syncthreads();
+  ok = __syncthreads_and(ok) != 0;
   for (int i = 0; i < 3; i++) {
     double r = red[i];
-    for (int w = 1; w < nw; w++) r = red_op(r, red[3 * w + i], op);
+    for (int w = 1; w < nw; w++) r += red[3 * w + i];
     out[i] = r;
   }
+  return ok;
 #endif
 }
 
+// Counter barrier (used once per instance, before the field is stored):
+// every wave's stores have completed, then one thread bumps the cluster's
+// counter and polls it up to `target`.  False when the cluster was aborted.
 OFX_DEV bool cluster_barrier(const ClusterDesc& cd, unsigned target, unsigned long long timeout,
-                             int* sflag, int coh) {
+                             int* sflag) {
 #ifdef OFX_EMU
-  (void)timeout;
   (void)sflag;
-  (void)coh;
-  return emu_cluster_barrier(&cd, target);
+  return emu_cluster_barrier(&cd, target, timeout);
 #else
-  // all of this wave's global stores have completed (reached L2, or memory for
-  // device-scope stores); the memory clobber keeps the compiler from moving
-  // any access across it
+  // the memory clobber keeps the compiler from moving any access across it
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -382,13 +583,13 @@ OFX_DEV bool cluster_barrier(const ClusterDesc& cd, unsigned target, unsigned lo
       __hip_atomic_fetch_add(cd.bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       const unsigned long long t0 = wall_clock64();
       while (__hip_atomic_load(cd.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-        if (__hip_atomic_load(cd.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        if (flag_load(cd.abort_flag)) {
           ok = 0;
           break;
         }
         __builtin_amdgcn_s_sleep(1);
         if (wall_clock64() - t0 > timeout) {
-          __hip_atomic_store(cd.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          flag_set(cd.abort_flag);
           ok = 0;
           break;
         }
@@ -401,6 +602,29 @@ OFX_DEV bool cluster_barrier(const ClusterDesc& cd, unsigned target, unsigned lo
 #endif
 }
 
+// Exchange counters of one cluster member.
+struct SyncState {
+  unsigned base;   // the cluster's tag base for this launch
+  unsigned nx;     // exchanges so far
+  unsigned nreal;  // counter barriers so far
+  int* sflag;      // LDS word for the barrier's verdict
+};
+
+// Exchange modes (a template parameter of the kernel, so that each mode's
+// code is compiled and register-allocated on its own).
+#define XM_TAGS 0     // tagged words, readers poll the tags
+#define XM_BARRIER 1  // plain words, a counter barrier before the reads
+
+// Before the reads of an exchange.  In barrier mode the whole cluster meets
+// first and the reads then find complete data; in tag mode the reads poll
+// straight away.  False when the cluster gave up.
+template <int XM>
+OFX_DEV bool xchg_ready(const ClusterDesc& cd, SyncState& sy, unsigned long long limit) {
+  if (XM == XM_TAGS) return true;
+  sy.nreal++;
+  return cluster_barrier(cd, sy.nreal * (unsigned)cd.C, limit, sy.sflag);
+}
+
 // ------------------------------------------------------------------------
 // Per-instance state and geometry held by one workgroup.
 // ------------------------------------------------------------------------
@@ -408,7 +632,7 @@ template <typename R>
 struct Geo {
   int n, n1, n2, lg1, lg2, lgn;
   int Cn, lgC, rank, W, lgW, Rr, lgR, HW, LC, sT, sF, c0, k20;
-  int coh;  // exchange accesses: 1 device-scope atomics (always used on the GPU)
+  int plain;  // exchange in plain form after a counter barrier (else tagged, polled)
   cpx<R>*B0, *B1, *B2, *B3, *B4;  // B3, B4: RK23 only
   const cpx<R>* tw1;  // LDS: exp(-2 pi i j / n1), j < n1
   const cpx<R>* tw2;  // LDS: exp(-2 pi i j / n2), j < n2
@@ -426,11 +650,13 @@ OFX_DEV double dw_of(int k, int n, double dt) {
 }
 
 // Forward transform of the own time columns held in `src` (layout lc*sT + r,
-// own columns lc in [HW, HW+W)) into t1 (row-major k2*n1 + c, twiddled).
+// own columns lc in [HW, HW+W)) into the tagged buffer t1 (row-major
+// k2*n1 + c, twiddled).
 template <typename R>
-OFX_DEV void fwd_cols_to_t1(const Geo<R>& g, cpx<R>* src, cpx<R>* other, cpx<R>* t1,
-                            const cpx<R>* tw, int nmax) {
+OFX_DEV void fwd_cols_to_t1(const Geo<R>& g, cpx<R>* src, cpx<R>* other, u64* t1,
+                            const cpx<R>* tw, int nmax, unsigned tag) {
   typedef cpx<R> C;
+  typedef Tagged<C> TT;
   C* res = lds_fft<R, -1>(src + g.HW * g.sT, other + g.HW * g.sT, g.lg2, g.W, g.sT, g.tw2, 1);
   const int tws = nmax >> g.lgn;
   const int tot = g.W * g.n2;
@@ -439,69 +665,111 @@ OFX_DEV void fwd_cols_to_t1(const Geo<R>& g, cpx<R>* src, cpx<R>* other, cpx<R>*
     const int c = g.c0 + cl;
     C v = res[cl * g.sT + k2];
     v = cmul(v, tw[(c * k2) * tws]);
-    xst(&t1[k2 * g.n1 + c], v, g.coh);
+    const TagPtr t = tp_plane<C>(g.plain, t1, g.n, (size_t)(k2 * g.n1 + c));
+    TT::put(const_cast<u64*>(t.p), t.ws, t.plain, v, tag);
   }
 }
 
 // Own frequency rows from t1 into `dst` (layout k2l*sF + c), then length-n1
-// forward transforms; returns the buffer holding bins (k2l*sF + k1).
+// forward transforms; returns the buffer holding bins (k2l*sF + k1).  `ok`
+// (uniform over the workgroup) is false when the cluster gave up.
 template <typename R>
-OFX_DEV cpx<R>* t1_to_rows_fwd(const Geo<R>& g, const cpx<R>* t1, cpx<R>* dst, cpx<R>* other,
-                               const cpx<R>* tw, int nmax) {
-  const int tot = g.Rr * g.n1;
-  for (int idx = OFX_TID; idx < tot; idx += OFX_NT) {
-    const int k2l = idx >> g.lg1, c = idx & (g.n1 - 1);
-    dst[k2l * g.sF + c] = xld(&t1[(g.k20 + k2l) * g.n1 + c], g.coh);
-  }
-  OFX_SYNC();
+OFX_DEV cpx<R>* t1_to_rows_fwd(const Geo<R>& g, const u64* t1, unsigned tag, cpx<R>* dst,
+                               cpx<R>* other, Spin& sp, bool& ok) {
+  typedef cpx<R> C;
+  typedef Tagged<C> TT;
+  ok = read_tagged<C>(
+      g.Rr * g.n1, tag,
+      [&](int idx) {
+        return tp_plane<C>(g.plain, t1, g.n, (size_t)((g.k20 + (idx >> g.lg1)) * g.n1 + (idx & (g.n1 - 1))));
+      },
+      [&](int idx, C v) { dst[(idx >> g.lg1) * g.sF + (idx & (g.n1 - 1))] = v; }, sp, g.plain);
+  ok = wg_all(ok, g.plain);
   return lds_fft<R, -1>(dst, other, g.lg1, g.Rr, g.sF, g.tw1, 1);
 }
 
-// After the first barrier of an attempt: read the cluster's error records
-// and this workgroup's t1 rows (into B1; A0 in B0 stays intact for a
-// rejection) with the loads of both in flight together, then reduce the
-// records (max).  out[0] = max error quotient (+inf when any member saw a
-// non-finite field).
+// After the exchange of an attempt: read the cluster's error records, this
+// workgroup's t1 rows (into B1; A0 in B0 stays intact for a rejection) and,
+// for RK23 (kx != null), the neighbours' K4 boundary columns into K1's halo,
+// all polled together, then reduce the records (max).  *emax = max error
+// quotient (+inf when any member saw a non-finite field).  Returns false
+// (uniform over the workgroup) when the cluster gave up.
 template <typename R>
-OFX_DEV void gather_err_and_rows(const ClusterDesc& cd, int par, const Geo<R>& g,
-                                 const cpx<R>* t1, double* out, double* red) {
-  const int totR = g.Rr * g.n1;
-#ifdef OFX_EMU
-  for (int idx = 0; idx < totR; idx++) {
-    const int k2l = idx >> g.lg1, c = idx & (g.n1 - 1);
-    g.B1[k2l * g.sF + c] = xld(&t1[(g.k20 + k2l) * g.n1 + c], g.coh);
-  }
-  gather_slots(cd, par, 0, out, red, g.coh);
-#else
-  const int nw = (int)(blockDim.x >> 6);
-  const int tot = cd.C * nw;
+OFX_DEV bool gather_err_and_rows(const ClusterDesc& cd, int par, const Geo<R>& g, const u64* t1,
+                                 const u64* kx, cpx<R>* K1, unsigned tag, double* emax,
+                                 double* red, Spin& sp) {
+  typedef cpx<R> C;
+  typedef Tagged<C> TT;
+  const int nw = slot_waves();
+  const int ns = cd.C * nw;
+  const int nr = g.Rr * g.n1;
+  const int nk = kx ? 6 * g.n2 : 0;
+  const int n2 = g.n2;
+  const int left = (g.rank + g.Cn - 1) & (g.Cn - 1), right = (g.rank + 1) & (g.Cn - 1);
+  // kx element of halo entry e = side*3n2 + col*n2 + r (row shift across the wrap)
+  auto kx_src = [&](int e) {
+    const int sc3 = e >> g.lg2, r = e & (n2 - 1);
+    const int side = sc3 / 3, col = sc3 - 3 * side;
+    int rr = r, q;
+    if (side == 0) {
+      q = left * 2 + 1;
+      if (g.c0 - 3 + col < 0) rr = (r - 1) & (n2 - 1);
+    } else {
+      q = right * 2 + 0;
+      if (g.c0 + g.W + col >= g.n1) rr = (r + 1) & (n2 - 1);
+    }
+    return tp_plane<C>(g.plain, kx, (size_t)g.Cn * 6 * n2, (size_t)(q * 3 + col) * n2 + rr);
+  };
   double v = -1.0;
-  for (int t = threadIdx.x; t < tot; t += blockDim.x) {
-    const int q = t / nw, w = t - q * nw;
-    const double* sl = cd.slots + ((size_t)(par * cd.C + q) * OFX_SLOT_WAVES + w) * 4;
-    v = red_op(v, xldd(sl, g.coh), 0);
-  }
-  for (int idx = threadIdx.x; idx < totR; idx += blockDim.x) {
-    const int k2l = idx >> g.lg1, c = idx & (g.n1 - 1);
-    g.B1[k2l * g.sF + c] = xld(&t1[(g.k20 + k2l) * g.n1 + c], g.coh);
-  }
+  bool ok = read_tagged<C>(
+      ns + nr + nk, tag,
+      [&](int idx) -> TagPtr {
+        if (idx < ns) return TagPtr{slot_at<R>(cd, par, idx / nw, idx % nw, 0), 1, 0};
+        idx -= ns;
+        if (idx < nr)
+          return tp_plane<C>(g.plain, t1, g.n, (size_t)((g.k20 + (idx >> g.lg1)) * g.n1 + (idx & (g.n1 - 1))));
+        return kx_src(idx - nr);
+      },
+      [&](int idx, C x) {
+        if (idx < ns) {
+          v = red_op(v, unpack_d<R>(x), 0);
+          return;
+        }
+        idx -= ns;
+        if (idx < nr) {
+          g.B1[(idx >> g.lg1) * g.sF + (idx & (g.n1 - 1))] = x;
+          return;
+        }
+        idx -= nr;
+        const int sc3 = idx >> g.lg2, r = idx & (n2 - 1);
+        const int side = sc3 / 3, col = sc3 - 3 * side;
+        const int lc = side == 0 ? 1 + col : g.HW + g.W + col;
+        K1[lc * g.sT + r] = x;
+      },
+      sp, g.plain);
+#ifdef OFX_EMU
+  (void)red;
+  *emax = v;
+  return ok;
+#else
   for (int o = 32; o > 0; o >>= 1) v = red_op(v, __shfl_xor(v, o), 0);
   __syncthreads();
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
+  ok = wg_all(ok, g.plain);
   double r = red[0];
   for (int w = 1; w < nw; w++) r = red_op(r, red[w], 0);
-  out[0] = r;
-  out[1] = out[2] = 0.0;
+  *emax = r;
+  return ok;
 #endif
 }
 
-// Inverse length-n1 transforms of the own rows, conjugate twiddle, write t2
-// (column-major c*n2 + k2).
+// Inverse length-n1 transforms of the own rows, conjugate twiddle, write the
+// tagged buffer t2 (column-major c*n2 + k2).
 template <typename R>
-OFX_DEV void rows_inv_to_t2(const Geo<R>& g, cpx<R>* src, cpx<R>* other, cpx<R>* t2,
-                            const cpx<R>* tw, int nmax) {
+OFX_DEV void rows_inv_to_t2(const Geo<R>& g, cpx<R>* src, cpx<R>* other, u64* t2,
+                            const cpx<R>* tw, int nmax, unsigned tag) {
   typedef cpx<R> C;
+  typedef Tagged<C> TT;
   C* res = lds_fft<R, +1>(src, other, g.lg1, g.Rr, g.sF, g.tw1, 1);
   const int tws = nmax >> g.lgn;
   const int tot = g.Rr * g.n1;
@@ -510,23 +778,29 @@ OFX_DEV void rows_inv_to_t2(const Geo<R>& g, cpx<R>* src, cpx<R>* other, cpx<R>*
     const int k2 = g.k20 + k2l;
     C v = res[k2l * g.sF + c];
     v = cmulc(v, tw[(c * k2) * tws]);
-    xst(&t2[c * g.n2 + k2], v, g.coh);
+    const TagPtr t = tp_plane<C>(g.plain, t2, g.n, (size_t)(c * g.n2 + k2));
+    TT::put(const_cast<u64*>(t.p), t.ws, t.plain, v, tag);
   }
 }
 
 // Own+halo columns from t2, inverse length-n2 transforms, scale 1/N and
 // place into A0 (B0) with the row shift that wrapped halo columns carry.
+// Returns false (uniform over the workgroup) when the cluster gave up.
 template <typename R>
-OFX_DEV void t2_to_cols_inv(const Geo<R>& g, const cpx<R>* t2, const cpx<R>* tw, int nmax) {
+OFX_DEV bool t2_to_cols_inv(const Geo<R>& g, const u64* t2, unsigned tag, Spin& sp) {
   typedef cpx<R> C;
+  typedef Tagged<C> TT;
   const int tot = g.LC * g.n2;
-  for (int idx = OFX_TID; idx < tot; idx += OFX_NT) {
-    const int lc = idx >> g.lg2, k2 = idx & (g.n2 - 1);
-    int gc = g.c0 - g.HW + lc;
-    gc = gc < 0 ? gc + g.n1 : (gc >= g.n1 ? gc - g.n1 : gc);
-    g.B1[lc * g.sT + k2] = xld(&t2[gc * g.n2 + k2], g.coh);
-  }
-  OFX_SYNC();
+  bool ok = read_tagged<C>(
+      tot, tag,
+      [&](int idx) {
+        const int lc = idx >> g.lg2, k2 = idx & (g.n2 - 1);
+        int gc = g.c0 - g.HW + lc;
+        gc = gc < 0 ? gc + g.n1 : (gc >= g.n1 ? gc - g.n1 : gc);
+        return tp_plane<C>(g.plain, t2, g.n, (size_t)(gc * g.n2 + k2));
+      },
+      [&](int idx, C v) { g.B1[(idx >> g.lg2) * g.sT + (idx & (g.n2 - 1))] = v; }, sp, g.plain);
+  ok = wg_all(ok, g.plain);
   C* res = lds_fft<R, +1>(g.B1, g.B2, g.lg2, g.LC, g.sT, g.tw2, 1);
   const R inv = R(1) / R(g.n);
   for (int idx = OFX_TID; idx < tot; idx += OFX_NT) {
@@ -537,6 +811,7 @@ OFX_DEV void t2_to_cols_inv(const Geo<R>& g, const cpx<R>* t2, const cpx<R>* tw,
     g.B0[lc * g.sT + r] = cscale(res[lc * g.sT + rp], inv);
   }
   OFX_SYNC();
+  return ok;
 }
 
 
@@ -551,17 +826,16 @@ OFX_DEV void t2_to_cols_inv(const Geo<R>& g, const cpx<R>* t2, const cpx<R>* tw,
 // cd.kx at the first barrier and fill K1's halo for the next step.
 // Returns ST_RUNNING or a terminal status; counts attempts / accepted steps.
 // ------------------------------------------------------------------------
-template <typename R, int MP>
+template <typename R, int MP, int XM>
 OFX_DEV int rk23_pass(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc& d, Geo<R>& g,
-                      int rank, double* red, int* sflag, unsigned& nbar, const double* breg,
+                      int rank, double* red, SyncState& sy, const double* breg,
                       const R* areg, long long& att, long long& acc) {
   typedef cpx<R> C;
+  typedef Tagged<C> TT;
   const int n = g.n, n1 = g.n1, n2 = g.n2;
   const C* tw = (const C*)a.twiddle;
   const int nmax = a.nmax;
-  C* t1 = (C*)cd.t1;
-  C* t2 = (C*)cd.t2;
-  C* kx = (C*)cd.kx;
+  const size_t kxs = (size_t)g.Cn * 6 * n2 * TT::NW;  // one parity half of kx
   const C* wt = (const C*)d.wt;
   const R* ab = (const R*)d.absorb;
   const R cc = (R)(0.5 / d.dt / d.w0);
@@ -657,23 +931,31 @@ OFX_DEV int rk23_pass(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc
         }
         OFX_SYNC();
       }
-      const int par = (int)(nbar & 1u);
-      publish_slots(cd, par, rank, 0, bad ? INFINITY : fmin(lmax, DBL_MAX), 0.0, 0.0, g.coh);
+      // exchange: error records, K4's boundary columns, forward column FFTs
+      const unsigned x1 = ++sy.nx, tag1 = sy.base + x1;
+      const int par = (int)(x1 & 1u);
+      u64* t1 = (u64*)cd.t1 + (size_t)par * n * TT::NW;
+      u64* kx = (u64*)cd.kx + par * kxs;
+      publish_slots<R>(cd, par, rank, 0, bad ? INFINITY : fmin(lmax, DBL_MAX), 0.0, 0.0, tag1);
       for (int idx = OFX_TID; idx < 6 * n2; idx += OFX_NT) {
         const int sc3 = idx >> g.lg2, r = idx & (n2 - 1);
         const int side = sc3 / 3, col = sc3 - 3 * side;
         const int lc = side == 0 ? HW + col : HW + W - 3 + col;
-        xst(&kx[((size_t)(rank * 2 + side) * 3 + col) * n2 + r], K1[lc * g.sT + r], g.coh);
+        const TagPtr t = tp_plane<C>(g.plain, kx, (size_t)g.Cn * 6 * n2,
+                                     ((size_t)(rank * 2 + side) * 3 + col) * n2 + r);
+        TT::put(const_cast<u64*>(t.p), t.ws, t.plain, K1[lc * g.sT + r], tag1);
       }
-      fwd_cols_to_t1<R>(g, K2, K3, t1, tw, nmax);
-      nbar++;
-      if (!cluster_barrier(cd, nbar * (unsigned)g.Cn, a.timeout_ticks, sflag, g.coh))
-        return ST_BARRIER_TIMEOUT;
-      double gr[3];
-      gather_err_and_rows<R>(cd, par, g, t1, gr, red);
+      fwd_cols_to_t1<R>(g, K2, K3, t1, tw, nmax, tag1);
+      double emax;
+      {
+        Spin sp = spin_begin(cd, a.timeout_ticks);
+        if (!xchg_ready<XM>(cd, sy, a.timeout_ticks) ||
+            !gather_err_and_rows<R>(cd, par, g, t1, kx, K1, tag1, &emax, red, sp))
+          return ST_BARRIER_TIMEOUT;
+      }
       att++;
-      if (isinf(gr[0])) return ST_NONFINITE;
-      const double pmax = gr[0] < 0.0 ? -1.0 : 4.0 * 100.0 * gr[0];
+      if (isinf(emax)) return ST_NONFINITE;
+      const double pmax = emax < 0.0 ? -1.0 : 4.0 * 100.0 * emax;
       const bool flag0 = pmax >= d.max_err;
       const bool flag1 = !flag0 && pmax >= d.min_err;
       if (flag0 && stepmod > 1.0 && (k - stepmod) > 0.0) {
@@ -689,24 +971,8 @@ OFX_DEV int rk23_pass(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc
       // ---------------- accept ----------------
       acc++;
       if (rank == 0 && d.stepmods && (int)k < Ns) d.stepmods[(int)k] = (float)stepmod;
-      // K1 halo <- neighbours' K4 boundary columns (row shift across the wrap)
-      {
-        const int left = (rank + g.Cn - 1) & (g.Cn - 1), right = (rank + 1) & (g.Cn - 1);
-        for (int idx = OFX_TID; idx < 6 * n2; idx += OFX_NT) {
-          const int sc3 = idx >> g.lg2, r = idx & (n2 - 1);
-          const int side = sc3 / 3, col = sc3 - 3 * side;
-          if (side == 0) {
-            const int gc = g.c0 - 3 + col;
-            const int rr = gc < 0 ? ((r - 1) & (n2 - 1)) : r;
-            K1[(1 + col) * g.sT + r] = xld(&kx[((size_t)(left * 2 + 1) * 3 + col) * n2 + rr], g.coh);
-          } else {
-            const int gc = g.c0 + W + col;
-            const int rr = gc >= n1 ? ((r + 1) & (n2 - 1)) : r;
-            K1[(HW + W + col) * g.sT + r] =
-                xld(&kx[((size_t)(right * 2 + 0) * 3 + col) * n2 + rr], g.coh);
-          }
-        }
-      }
+      // (K1's halo already holds the neighbours' K4 boundary columns, read
+      // with the rows)
       // frequency phase: rows, dispersion (step just taken), snapshot, inverse rows
       const double s_used = stepmod;
       C* F = lds_fft<R, -1>(g.B1, g.B2, g.lg1, g.Rr, g.sF, g.tw1, 1);
@@ -731,11 +997,14 @@ OFX_DEV int rk23_pass(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc
         disperse(idx, d.beta[kk], (R)d.alpha[kk]);
       }
       OFX_SYNC();
-      rows_inv_to_t2<R>(g, F, Fo, t2, tw, nmax);
-      nbar++;
-      if (!cluster_barrier(cd, nbar * (unsigned)g.Cn, a.timeout_ticks, sflag, g.coh))
-        return ST_BARRIER_TIMEOUT;
-      t2_to_cols_inv<R>(g, t2, tw, nmax);
+      {
+        const unsigned x2 = ++sy.nx, tag2 = sy.base + x2;
+        u64* t2 = (u64*)cd.t2 + (size_t)(x2 & 1u) * n * TT::NW;
+        rows_inv_to_t2<R>(g, F, Fo, t2, tw, nmax, tag2);
+        Spin sp = spin_begin(cd, a.timeout_ticks);
+        if (!xchg_ready<XM>(cd, sy, a.timeout_ticks) || !t2_to_cols_inv<R>(g, t2, tag2, sp))
+          return ST_BARRIER_TIMEOUT;
+      }
       // step control after an accepted step (OPO_CUDA_ODE23.cu order)
       if (k + stepmod >= cend && k < cend - 1.0) {
         const double stepend = (cend - k) - 1.0;
@@ -754,11 +1023,11 @@ OFX_DEV int rk23_pass(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc
 // ------------------------------------------------------------------------
 // One instance: single pass (trips == 0) or round trips.
 // ------------------------------------------------------------------------
-template <typename R>
+template <typename R, int XM>
 OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc& d,
-                          int rank, cpx<R>* lds, double* red, int* sflag, unsigned& nbar,
-                          int coh) {
+                          int rank, cpx<R>* lds, double* red, int* sflag, SyncState& sy) {
   typedef cpx<R> C;
+  typedef Tagged<C> TT;
   // converged in an earlier launch (and not rewritten since): nothing to do
   if (a.conv_tol > 0.0 && a.trips > 0 && d.status[0] == ST_CONVERGED) return;
   Geo<R> g;
@@ -782,7 +1051,7 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
   g.sF = g.n1 + 1;
   g.c0 = rank * g.W;
   g.k20 = rank * g.Rr;
-  g.coh = coh;
+  g.plain = XM == XM_BARRIER;
   {
     int b1 = g.LC * g.sT, b2 = g.Rr * g.sF;
     int bs = b1 > b2 ? b1 : b2;
@@ -805,8 +1074,9 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
   const C* tw = (const C*)a.twiddle;
   const int nmax = a.nmax;
   C* field = (C*)d.field;
-  C* t1 = (C*)cd.t1;
-  C* t2 = (C*)cd.t2;
+  // parity halves of the tagged transpose buffers
+  auto t1_of = [&](unsigned x) { return (u64*)cd.t1 + (size_t)(x & 1u) * n * TT::NW; };
+  auto t2_of = [&](unsigned x) { return (u64*)cd.t2 + (size_t)(x & 1u) * n * TT::NW; };
   const double dt = d.dt, w0 = d.w0;
   const C* wt = (const C*)d.wt;
   const R* ab = (const R*)d.absorb;
@@ -851,8 +1121,7 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
     double k = 1.0, stepmod = 4.0;
     int chunk = 1;
     if (d.method == METHOD_RK23) {
-      status = rk23_pass<R, MP>(a, cd, d, g, rank, red, sflag, nbar, breg, areg, trip_att,
-                                trip_acc);
+      status = rk23_pass<R, MP, XM>(a, cd, d, g, rank, red, sy, breg, areg, trip_att, trip_acc);
       attempts += trip_att;
       accepted += trip_acc;
       chunk = (int)nChunks + 1;  // skip the Heun loop
@@ -911,18 +1180,21 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
       for (int idx = OFX_TID; idx < g.W * n2; idx += OFX_NT)
         corrector(idx, ab[gidx(g, g.HW + (idx >> g.lg2), idx & (n2 - 1))]);
       OFX_SYNC();
-      const int par = (int)(nbar & 1u);
-      publish_slots(cd, par, rank, 0, bad ? INFINITY : fmin(lmax, DBL_MAX), 0.0, 0.0, g.coh);
-      // speculative forward column transforms of E
-      fwd_cols_to_t1<R>(g, X, K, t1, tw, nmax);
-      nbar++;
-      if (!cluster_barrier(cd, nbar * (unsigned)g.Cn, a.timeout_ticks, sflag, g.coh)) {
-        status = ST_BARRIER_TIMEOUT;
-        break;
+      // exchange: error records and the speculative forward column transforms of E
+      const unsigned x1 = ++sy.nx, tag1 = sy.base + x1;
+      const int par = (int)(x1 & 1u);
+      publish_slots<R>(cd, par, rank, 0, bad ? INFINITY : fmin(lmax, DBL_MAX), 0.0, 0.0, tag1);
+      fwd_cols_to_t1<R>(g, X, K, t1_of(x1), tw, nmax, tag1);
+      double gmax;
+      {
+        Spin sp = spin_begin(cd, a.timeout_ticks);
+        if (!xchg_ready<XM>(cd, sy, a.timeout_ticks) ||
+            !gather_err_and_rows<R>(cd, par, g, t1_of(x1), nullptr, nullptr, tag1, &gmax, red,
+                                    sp)) {
+          status = ST_BARRIER_TIMEOUT;
+          break;
+        }
       }
-      double gr[3];
-      gather_err_and_rows<R>(cd, par, g, t1, gr, red);
-      const double gmax = gr[0];
       const bool gbad = isinf(gmax);
       attempts++;
       trip_att++;
@@ -973,13 +1245,15 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
         disperse(idx, d.beta[kk], (R)d.alpha[kk]);
       }
       OFX_SYNC();
-      rows_inv_to_t2<R>(g, F, Fo, t2, tw, nmax);
-      nbar++;
-      if (!cluster_barrier(cd, nbar * (unsigned)g.Cn, a.timeout_ticks, sflag, g.coh)) {
-        status = ST_BARRIER_TIMEOUT;
-        break;
+      {
+        const unsigned x2 = ++sy.nx, tag2 = sy.base + x2;
+        rows_inv_to_t2<R>(g, F, Fo, t2_of(x2), tw, nmax, tag2);
+        Spin sp = spin_begin(cd, a.timeout_ticks);
+        if (!xchg_ready<XM>(cd, sy, a.timeout_ticks) || !t2_to_cols_inv<R>(g, t2_of(x2), tag2, sp)) {
+          status = ST_BARRIER_TIMEOUT;
+          break;
+        }
       }
-      t2_to_cols_inv<R>(g, t2, tw, nmax);
       k = knew;
       stepmod = ns;
       chunk = cend;
@@ -992,14 +1266,19 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
     }
     if (!cavity) continue;
     // ---------------- cavity: OC split, detuning, pump (one FFT pair) ----------------
-    fwd_cols_to_t1<R>(g, g.B0, g.B1, t1, tw, nmax);
-    nbar++;
-    if (!cluster_barrier(cd, nbar * (unsigned)g.Cn, a.timeout_ticks, sflag, g.coh)) {
-      status = ST_BARRIER_TIMEOUT;
-      break;
-    }
+    const unsigned xc1 = ++sy.nx, tagc1 = sy.base + xc1;
+    fwd_cols_to_t1<R>(g, g.B0, g.B1, t1_of(xc1), tw, nmax, tagc1);
     {
-      C* F = t1_to_rows_fwd<R>(g, t1, g.B0, g.B1, tw, nmax);
+      bool ok = xchg_ready<XM>(cd, sy, a.timeout_ticks);
+      C* F = g.B0;
+      if (ok) {
+        Spin sp = spin_begin(cd, a.timeout_ticks);
+        F = t1_to_rows_fwd<R>(g, t1_of(xc1), tagc1, g.B0, g.B1, sp, ok);
+      }
+      if (!ok) {
+        status = ST_BARRIER_TIMEOUT;
+        break;
+      }
       C* Fo = (F == g.B0) ? g.B1 : g.B0;
       const C* pump = (const C*)d.pump;
       const C* cav = (const C*)d.cav;
@@ -1025,17 +1304,20 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
         F[k2l * g.sF + k1] = cadd(cmul(v, m), pump ? pump[kk] : mkc<R>(R(0), R(0)));
       }
       OFX_SYNC();
-      const int par = (int)(nbar & 1u);
-      publish_slots(cd, par, rank, 1, acc[0], acc[1], acc[2], g.coh);
-      rows_inv_to_t2<R>(g, F, Fo, t2, tw, nmax);
-      nbar++;
-      if (!cluster_barrier(cd, nbar * (unsigned)g.Cn, a.timeout_ticks, sflag, g.coh)) {
+      const unsigned xc2 = ++sy.nx, tagc2 = sy.base + xc2;
+      const int par = (int)(xc2 & 1u);
+      publish_slots<R>(cd, par, rank, 1, acc[0], acc[1], acc[2], tagc2);
+      rows_inv_to_t2<R>(g, F, Fo, t2_of(xc2), tw, nmax, tagc2);
+      double tot[3];
+      ok = xchg_ready<XM>(cd, sy, a.timeout_ticks);
+      if (ok) {
+        Spin sp = spin_begin(cd, a.timeout_ticks);
+        ok = t2_to_cols_inv<R>(g, t2_of(xc2), tagc2, sp);
+      }
+      if (!ok || !gather_sums<R>(cd, par, tagc2, tot, red, a.timeout_ticks)) {
         status = ST_BARRIER_TIMEOUT;
         break;
       }
-      double tot[3];
-      gather_slots(cd, par, 1, tot, red, g.coh);
-      t2_to_cols_inv<R>(g, t2, tw, nmax);
       const double conv = have_prev ? (tot[1] > 0.0 ? sqrt(tot[0] / tot[1]) : 0.0) : INFINITY;
       if (rank == 0 && d.trip_log && trip_base + trip < d.trip_cap) {
         double* tl = d.trip_log + 4 * (size_t)(trip_base + trip);
@@ -1052,8 +1334,8 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
   if (status == ST_RUNNING) status = ST_DONE;
   // store own columns (after a barrier so no member still reads the input halo)
   if (status != ST_BARRIER_TIMEOUT) {
-    nbar++;
-    if (!cluster_barrier(cd, nbar * (unsigned)g.Cn, a.timeout_ticks, sflag, g.coh))
+    sy.nreal++;
+    if (!cluster_barrier(cd, sy.nreal * (unsigned)g.Cn, a.timeout_ticks, sflag))
       status = ST_BARRIER_TIMEOUT;
   }
   if (status != ST_BARRIER_TIMEOUT) {
@@ -1071,14 +1353,21 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
   }
 }
 
-template <typename R>
+template <typename R, int XM>
 OFX_DEV void cluster_main(const LaunchArgs& a, int wg, cpx<R>* lds, double* red, int* sflag) {
   const int ci = a.wg_cluster[wg];
   if (ci < 0) return;
   const ClusterDesc& cd = a.clusters[ci];
   const int rank = a.wg_rank[wg];
-  unsigned nbar = 0;
-  const int coh = 1;
+  SyncState sy;
+#ifdef OFX_EMU
+  sy.base = __atomic_load_n(cd.epoch, __ATOMIC_RELAXED);
+#else
+  sy.base = __hip_atomic_load(cd.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#endif
+  sy.nx = 0;
+  sy.nreal = 0;
+  sy.sflag = sflag;
 #ifndef OFX_EMU
   // placement diagnostic: hardware XCC id of every member
   if (threadIdx.x == 0) {
@@ -1089,11 +1378,20 @@ OFX_DEV void cluster_main(const LaunchArgs& a, int wg, cpx<R>* lds, double* red,
 #endif
   for (int ii = 0; ii < cd.n_inst; ii++) {
     const int id = a.inst_order[cd.first_inst + ii];
-    run_instance<R>(a, cd, a.inst[id], rank, lds, red, sflag, nbar, coh);
+    run_instance<R, XM>(a, cd, a.inst[id], rank, lds, red, sflag, sy);
+    if (flag_load(cd.abort_flag)) break;
+  }
+  // the tags this member used end at base + nx: the next launch starts past
+  // the largest over the cluster
+  if (OFX_TID == 0) {
 #ifdef OFX_EMU
-    if (*cd.abort_flag) break;
+    unsigned cur = __atomic_load_n(cd.epoch + 1, __ATOMIC_RELAXED);
+    while (cur < sy.nx &&
+           !__atomic_compare_exchange_n(cd.epoch + 1, &cur, sy.nx, false, __ATOMIC_RELAXED,
+                                        __ATOMIC_RELAXED)) {
+    }
 #else
-    if (__hip_atomic_load(cd.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+    __hip_atomic_fetch_max(cd.epoch + 1, sy.nx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #endif
   }
 }

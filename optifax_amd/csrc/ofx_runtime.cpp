@@ -70,6 +70,9 @@ struct ofx_opo {
   std::vector<void*> allocs;
   std::vector<unsigned*> zero_ptrs;
   std::vector<size_t> zero_words;
+  unsigned* epoch_d = nullptr;  // [clusters][2] tag epochs (advanced before every launch)
+  double wait_s = 10.0;         // exchange wait limit
+  int xmode = 0;                // exchange mode of the kernel: 0 tagged polling, 1 counter barrier
   int graph_trips = -1;
   double graph_tol = 0.0;
 };
@@ -335,17 +338,29 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
   }
   r->nmax = nmax;
   r->block = (tune && tune->block) ? tune->block : 512;
+  const int exchange = tune ? tune->exchange : 0;
+  if (exchange < 0 || exchange > 2) {
+    delete r;
+    return fail(ctx, OFX_E_ARG, "exchange=%d must be 0 (auto), 1 (tags) or 2 (barrier)", exchange);
+  }
   if (r->block < 64 || r->block > 512 || (r->block & 63)) {
     delete r;
     return fail(ctx, OFX_E_ARG, "block=%d must be a multiple of 64 in [64,512]", r->block);
   }
   size_t lds = 0;
+  int cmin = 1 << 30;
   for (int i = 0; i < n_inst; i++) {
     const int n1 = r->inst_n1[i], n2 = specs[i].pass.n / n1;
     lds = std::max(lds, lds_bytes_for(n1, n2, r->inst_C[i], specs[i].pass.method, r->csz));
+    cmin = std::min(cmin, r->inst_C[i]);
   }
   r->lds = lds;
-  const int bpc = be_blocks_per_cu(ctx->be, precision, r->block, lds);
+  // Exchange mode of the launch: polling the tagged words wins when every
+  // cluster has at least 64 workgroups (a few elements per thread, latency
+  // bound); with larger shares per workgroup the plain words plus a counter
+  // barrier move half the bytes and win (measured on N = 2^15).
+  r->xmode = exchange == 1 ? 0 : exchange == 2 ? 1 : (cmin >= 64 ? 0 : 1);
+  const int bpc = be_blocks_per_cu(ctx->be, precision, r->xmode, r->block, lds);
   if (bpc < 1) {
     delete r;
     return fail(ctx, OFX_E_ARG, "kernel cannot launch with block=%d and %zu bytes of LDS", r->block, lds);
@@ -538,8 +553,10 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
   }
   if (st) { be_sync(ctx->be); ofx_opo_destroy(r); return st; }
 
-  // clusters
+  // clusters; exchange buffers hold tagged elements (twice the complex size)
+  // in two parity halves, zeroed at allocation (tag 0 is never awaited)
   r->cl_h.resize(cl_C.size());
+  st = dev_alloc(r, (void**)&r->epoch_d, sizeof(unsigned) * 2 * cl_C.size());
   for (size_t c = 0; c < cl_C.size() && !st; c++) {
     ClusterDesc& cd = r->cl_h[c];
     memset(&cd, 0, sizeof cd);
@@ -553,15 +570,17 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
       nc = std::max(nc, specs[id].pass.n);
       n2c = std::max(n2c, specs[id].pass.n / r->inst_n1[id]);
     }
-    st = dev_alloc(r, &cd.t1, (size_t)nc * r->csz);
-    if (!st) st = dev_alloc(r, &cd.t2, (size_t)nc * r->csz);
-    if (!st) st = dev_alloc(r, (void**)&cd.slots, sizeof(double) * 2 * 8 * 4 * (size_t)cd.C);
+    const size_t tsz = 2 * r->csz;  // bytes of one tagged complex element
+    st = dev_alloc(r, &cd.t1, 2 * (size_t)nc * tsz);
+    if (!st) st = dev_alloc(r, &cd.t2, 2 * (size_t)nc * tsz);
+    if (!st) st = dev_alloc(r, &cd.slots, 2 * 8 * 4 * (size_t)cd.C * tsz);
     if (!st) st = dev_alloc(r, (void**)&cd.bar, 64);
     if (!st) st = dev_alloc(r, (void**)&cd.abort_flag, 64);
     if (!st) st = dev_alloc(r, (void**)&cd.xcc, sizeof(int) * (size_t)cd.C);
     if (!st && cl_method[c] == OFX_METHOD_RK23)
-      st = dev_alloc(r, &cd.kx, (size_t)cd.C * 6 * (size_t)n2c * r->csz);
+      st = dev_alloc(r, &cd.kx, 2 * (size_t)cd.C * 6 * (size_t)n2c * tsz);
     if (st) break;
+    cd.epoch = r->epoch_d + 2 * c;
     r->zero_ptrs.push_back(cd.bar);
     r->zero_words.push_back(1);
     r->zero_ptrs.push_back((unsigned*)cd.abort_flag);
@@ -603,8 +622,8 @@ static LaunchArgs make_args(ofx_opo* r, int trips, double tol) {
   a.nmax = r->nmax;
   a.trips = trips;
   a.conv_tol = tol;
-  // barrier timeout: 10 s of wall clock
-  a.timeout_ticks = (unsigned long long)r->ctx->info.wallclock_khz * 1000ULL * 10ULL;
+  // exchange wait limit in wall-clock ticks
+  a.timeout_ticks = (unsigned long long)((double)r->ctx->info.wallclock_khz * 1000.0 * r->wait_s);
   a.xcc_ids = r->xcc_d;
   return a;
 }
@@ -613,8 +632,9 @@ int ofx_opo_launch(ofx_opo* r, int trips, double conv_tol) {
   if (!r) return OFX_E_ARG;
   if (trips < 0) return fail(r->ctx, OFX_E_ARG, "trips must be >= 0");
   LaunchArgs a = make_args(r, trips, conv_tol);
-  int st = be_launch(r->ctx->be, a, r->prec, r->grid, r->block, r->lds, r->zero_ptrs.data(),
-                     r->zero_words.data(), (int)r->zero_ptrs.size(), nullptr, 0);
+  int st = be_launch(r->ctx->be, a, r->prec, r->xmode, r->grid, r->block, r->lds,
+                     r->zero_ptrs.data(), r->zero_words.data(), (int)r->zero_ptrs.size(),
+                     r->epoch_d, (int)r->cl_h.size(), nullptr, 0);
   if (st) return fail(r->ctx, OFX_E_HIP, "launch failed: %s", be_error(r->ctx->be));
   return OFX_OK;
 }
@@ -627,9 +647,22 @@ int ofx_opo_graph_replay(ofx_opo* r, int trips, double conv_tol, int replays) {
     r->graph_tol = conv_tol;
   }
   LaunchArgs a = make_args(r, trips, conv_tol);
-  int st = be_launch(r->ctx->be, a, r->prec, r->grid, r->block, r->lds, r->zero_ptrs.data(),
-                     r->zero_words.data(), (int)r->zero_ptrs.size(), r, replays);
+  int st = be_launch(r->ctx->be, a, r->prec, r->xmode, r->grid, r->block, r->lds,
+                     r->zero_ptrs.data(), r->zero_words.data(), (int)r->zero_ptrs.size(),
+                     r->epoch_d, (int)r->cl_h.size(), r, replays);
   if (st) return fail(r->ctx, OFX_E_HIP, "graph replay failed: %s", be_error(r->ctx->be));
+  return OFX_OK;
+}
+
+int ofx_opo_set_wait_limit(ofx_opo* r, double seconds) {
+  if (!r) return OFX_E_ARG;
+  if (!(seconds >= 0.0 && seconds <= 1e6))
+    return fail(r->ctx, OFX_E_ARG, "wait limit must be in [0, 1e6] s");
+  if (seconds != r->wait_s) {
+    be_forget_graph(r->ctx->be, r);  // the captured launch holds the old limit
+    r->graph_trips = -1;
+    r->wait_s = seconds;
+  }
   return OFX_OK;
 }
 
@@ -641,7 +674,7 @@ int ofx_opo_sync(ofx_opo* r) {
     return fail(r->ctx, OFX_E_HIP, "status readback failed: %s", be_error(r->ctx->be));
   for (int i = 0; i < r->n_inst; i++) {
     if (stv[i] == ST_BARRIER_TIMEOUT)
-      return fail(r->ctx, OFX_E_TIMEOUT, "instance %d: cluster barrier timed out", i);
+      return fail(r->ctx, OFX_E_TIMEOUT, "instance %d: a cluster member timed out waiting for exchange data", i);
   }
   for (int i = 0; i < r->n_inst; i++) {
     if (stv[i] == ST_NONFINITE)

@@ -82,7 +82,10 @@ def default_context() -> Context:
 def _tuning(tune):
     if tune is None:
         return None
-    t = Tuning(int(tune.get("cluster", 0)), int(tune.get("n1", 0)), int(tune.get("block", 0)))
+    ex = tune.get("exchange", 0)
+    ex = {"auto": 0, "tags": 1, "barrier": 2}.get(ex, ex)
+    t = Tuning(int(tune.get("cluster", 0)), int(tune.get("n1", 0)), int(tune.get("block", 0)),
+               int(ex))
     return ctypes.byref(t)
 
 
@@ -249,6 +252,11 @@ class OPOBatch:
         st = self.lib.ofx_opo_sync(self.h)
         if st and not (allow_nonfinite and st == _lib.OFX_E_NONFINITE):
             self.ctx.check(st)
+
+    def set_wait_limit(self, seconds):
+        """Longest wait of one workgroup for its cluster's exchange data before
+        the cluster gives up (default 10 s; sync then raises a timeout)."""
+        self.ctx.check(self.lib.ofx_opo_set_wait_limit(self.h, float(seconds)))
 
     def last_ms(self):
         ms = ctypes.c_double()

@@ -11,6 +11,7 @@
 #include <string.h>
 
 #include <atomic>
+#include <chrono>
 #include <condition_variable>
 #include <mutex>
 #include <string>
@@ -26,12 +27,26 @@ namespace ofx {
 static std::mutex g_mu;
 static std::condition_variable g_cv;
 
-bool emu_cluster_barrier(const ClusterDesc* cd, unsigned target) {
+bool emu_cluster_barrier(const ClusterDesc* cd, unsigned target, unsigned long long timeout) {
   std::unique_lock<std::mutex> lk(g_mu);
   (*cd->bar)++;
   g_cv.notify_all();
-  g_cv.wait(lk, [&] { return *cd->bar >= target || *cd->abort_flag; });
-  return *cd->abort_flag == 0;
+  // the abort flag is set without the lock: re-check it every millisecond
+  const unsigned long long t0 = emu_clock();
+  while (!(*cd->bar >= target || __atomic_load_n(cd->abort_flag, __ATOMIC_RELAXED))) {
+    if (emu_clock() - t0 > timeout) {
+      __atomic_store_n(cd->abort_flag, 1, __ATOMIC_RELAXED);
+      break;
+    }
+    g_cv.wait_for(lk, std::chrono::milliseconds(1));
+  }
+  return __atomic_load_n(cd->abort_flag, __ATOMIC_RELAXED) == 0;
+}
+
+// 100 MHz, the wall-clock rate reported by be_create
+unsigned long long emu_clock() {
+  return (unsigned long long)std::chrono::duration_cast<std::chrono::nanoseconds>(
+             std::chrono::steady_clock::now().time_since_epoch()).count() / 10ULL;
 }
 
 struct Backend {
@@ -62,30 +77,36 @@ void be_free(Backend*, void* p) { free(p); }
 int be_h2d(Backend*, void* dst, const void* src, size_t bytes) { memcpy(dst, src, bytes); return OFX_OK; }
 int be_d2h(Backend*, void* dst, const void* src, size_t bytes) { memcpy(dst, src, bytes); return OFX_OK; }
 int be_sync(Backend*) { return OFX_OK; }
-int be_blocks_per_cu(Backend*, int, int, size_t lds) { return lds <= 160 * 1024 ? 1 : 0; }
+int be_blocks_per_cu(Backend*, int, int, int, size_t lds) { return lds <= 160 * 1024 ? 1 : 0; }
 
-template <typename R>
+template <typename R, int XM>
 static void run_wg(const LaunchArgs* a, int wg, size_t lds) {
   std::vector<unsigned char> mem(lds + 64);
   double red[64];
   int sflag[4];
-  cluster_main<R>(*a, wg, reinterpret_cast<cpx<R>*>(mem.data()), red, sflag);
+  cluster_main<R, XM>(*a, wg, reinterpret_cast<cpx<R>*>(mem.data()), red, sflag);
 }
 
-int be_launch(Backend* b, const LaunchArgs& a, int precision, int grid, int block, size_t lds,
-              unsigned* const* zp, const size_t* zw, int nz, const void* key, int replays) {
+int be_launch(Backend* b, const LaunchArgs& a, int precision, int xmode, int grid, int block,
+              size_t lds, unsigned* const* zp, const size_t* zw, int nz, unsigned* ep, int nep,
+              const void* key, int replays) {
   (void)block;
   (void)key;
-  for (int i = 0; i < nz; i++) memset(zp[i], 0, zw[i] * sizeof(unsigned));
   const int reps = replays > 0 ? replays : 1;
   for (int rep = 0; rep < reps; rep++) {
     for (int i = 0; i < nz; i++) memset(zp[i], 0, zw[i] * sizeof(unsigned));
+    for (int j = 0; ep && j < nep; j++) {
+      ep[2 * j] += ep[2 * j + 1];
+      ep[2 * j + 1] = 0u;
+    }
     std::vector<std::thread> th;
     for (int wg = 0; wg < grid; wg++) {
       if (precision == OFX_F64)
-        th.emplace_back(run_wg<double>, &a, wg, lds);
+        th.emplace_back(xmode == XM_BARRIER ? run_wg<double, XM_BARRIER> : run_wg<double, XM_TAGS>,
+                        &a, wg, lds);
       else
-        th.emplace_back(run_wg<float>, &a, wg, lds);
+        th.emplace_back(xmode == XM_BARRIER ? run_wg<float, XM_BARRIER> : run_wg<float, XM_TAGS>,
+                        &a, wg, lds);
     }
     for (auto& t : th) t.join();
   }

@@ -14,18 +14,23 @@ from optifax_amd.solver import OPOBatch, OPOmexBatch
 from cases import crystal_case, opo_case, oracle_next_input, rel_l2
 
 
-def _run(ctx, E, args, n, precision, C, apft_cols=4):
+def _run(ctx, E, args, n, precision, C, apft_cols=4, exchange="auto"):
     ns = args[5]
     return OPOmexBatch(E, *args, np.zeros((n, apft_cols), complex), np.zeros(ns), ctx=ctx,
-                       precision=precision, tune={"cluster": C})
+                       precision=precision, tune={"cluster": C, "exchange": exchange})
 
 
+# both ways a workgroup learns that its cluster's exchange data are complete
+EXCHANGES = ("tags", "barrier")
+
+
+@pytest.mark.parametrize("exchange", EXCHANGES)
 @pytest.mark.parametrize("n,C,window", [(256, 1, 2e-12), (256, 2, 2e-12), (256, 4, 2e-12),
                                         (1024, 2, 4e-12), (1024, 8, 4e-12)])
-def test_pass_fp64_matches_oracle(emu_ctx, n, C, window):
+def test_pass_fp64_matches_oracle(emu_ctx, n, C, window, exchange):
     sw, si, E, args = crystal_case(n=n, window=window)
     Eo, apo, smo, cnt = oracle.nee_heun(E, *args, n_cols=4)
-    Eg, apg, smg = _run(emu_ctx, E, args, n, "double", C)
+    Eg, apg, smg = _run(emu_ctx, E, args, n, "double", C, exchange=exchange)
     assert np.array_equal(smg, smo)                 # identical adaptive step sequence
     assert rel_l2(Eg, Eo) < 1e-10
     assert rel_l2(apg, apo) < 1e-10
@@ -40,7 +45,8 @@ def test_pass_fp32_tracks_oracle(emu_ctx):
     assert rel_l2(Eg, Eo) < 2e-4
 
 
-def test_round_trips_match_literal_opticalsim(emu_ctx):
+@pytest.mark.parametrize("exchange", EXCHANGES)
+def test_round_trips_match_literal_opticalsim(emu_ctx, exchange):
     cfg, b = opo_case()
     si, ops = b["solver"], b["ops"]
     sw = b["simwin"]
@@ -51,7 +57,7 @@ def test_round_trips_match_literal_opticalsim(emu_ctx):
     for C in (1, 2):
         run = OPOBatch([dict(solver=si, field0=ops["field0"], pump=ops["pump"], cav=ops["cav"],
                              ocm=ops["ocm"], delay=cfg.delay)], precision="double", trip_cap=trips,
-                       ctx=emu_ctx, tune={"cluster": C})
+                       ctx=emu_ctx, tune={"cluster": C, "exchange": exchange})
         run.launch(trips)
         run.sync()
         r = run.read(0)
@@ -123,16 +129,17 @@ def test_argument_errors(emu_ctx):
                     None, ctx=emu_ctx)      # N not a power of two
 
 
+@pytest.mark.parametrize("exchange", EXCHANGES)
 @pytest.mark.parametrize("n,C,window", [(256, 1, 2e-12), (256, 2, 2e-12), (1024, 4, 4e-12),
                                         (1024, 8, 4e-12)])
-def test_rk23_pass_fp64_matches_oracle(emu_ctx, n, C, window):
+def test_rk23_pass_fp64_matches_oracle(emu_ctx, n, C, window, exchange):
     """RK23 (ode23 variant): same accept/reject sequence and FP64 fields as orc_rk23,
     including the K4 boundary exchange between workgroups (C > 1)."""
     sw, si, E, args = crystal_case(n=n, window=window)
     Eo, apo, smo, cnt = oracle.rk23(E, *args, n_cols=4)
     assert cnt[0] > cnt[1]                          # the case exercises rejections
     Eg, apg, smg = OPOmexBatch(E, *args, np.zeros((n, 4), complex), np.zeros(args[5]),
-                               ctx=emu_ctx, precision="double", tune={"cluster": C},
+                               ctx=emu_ctx, precision="double", tune={"cluster": C, "exchange": exchange},
                                method="rk23")
     assert np.array_equal(smg, smo)
     assert rel_l2(Eg, Eo) < 1e-10
@@ -193,4 +200,40 @@ def test_converged_instances_are_skipped_until_rewritten(emu_ctx):
     run.sync()
     r3 = run.read(0)
     assert r3["trips_done"] == 4 and np.isinf(r3["trip_log"][2, 0])
+    run.close()
+
+
+@pytest.mark.parametrize("exchange", EXCHANGES)
+def test_wait_limit_gives_up_and_the_next_launch_recovers(emu_ctx, exchange):
+    """A zero wait limit makes the cluster give up at its first wait (every
+    member leaves through its own bounded poll); the field stays as it was,
+    and after the limit is restored the same launch reproduces a fresh run
+    exactly (the tags of the abandoned launch are never mistaken for new
+    data)."""
+    cfg, b = opo_case()
+    si, ops = b["solver"], b["ops"]
+    inst = dict(solver=si, field0=ops["field0"], pump=ops["pump"], cav=ops["cav"],
+                ocm=ops["ocm"], delay=cfg.delay)
+    tune = {"cluster": 4, "exchange": exchange}
+    ref = OPOBatch([inst], precision="double", trip_cap=4, ctx=emu_ctx, tune=tune)
+    ref.launch(2)
+    ref.sync()
+    want = ref.read(0)
+    ref.close()
+    run = OPOBatch([inst], precision="double", trip_cap=8, ctx=emu_ctx, tune=tune)
+    f0 = run.read(0)["field"]
+    run.set_wait_limit(0.0)
+    with pytest.raises(OfxError, match="timed out"):
+        run.launch(2)
+        run.sync()
+    assert np.array_equal(run.read(0)["field"], f0)
+    run.set_wait_limit(10.0)
+    run.write_field(0, f0)
+    run.launch(2)
+    run.sync()
+    got = run.read(0)
+    assert np.array_equal(got["field"], want["field"])
+    assert np.array_equal(got["oc_spec"], want["oc_spec"])
+    with pytest.raises(OfxError):
+        run.set_wait_limit(-1.0)
     run.close()

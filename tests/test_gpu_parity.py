@@ -14,33 +14,42 @@ from cases import crystal_case, opo_case, oracle_next_input, rel_l2
 pytestmark = pytest.mark.gpu
 
 
-def _run(ctx, E, args, n, precision, C=None, cols=4):
+def _run(ctx, E, args, n, precision, C=None, cols=4, exchange="auto"):
+    tune = {"cluster": C, "exchange": exchange} if C else None
     return OPOmexBatch(E, *args, np.zeros((n, cols), complex), np.zeros(args[5]), ctx=ctx,
-                       precision=precision, tune={"cluster": C} if C else None)
+                       precision=precision, tune=tune)
 
 
+# both ways a workgroup learns that its cluster's exchange data are complete
+EXCHANGES = ("tags", "barrier")
+
+
+@pytest.mark.parametrize("exchange", EXCHANGES)
 @pytest.mark.parametrize("n,C,window", [(256, 1, 2e-12), (256, 2, 2e-12), (1024, 2, 4e-12),
                                         (1024, 4, 4e-12), (4096, 8, 8e-12), (4096, 16, 8e-12),
-                                        (8192, 8, 10e-12)])
-def test_pass_fp64_exact(gpu_ctx, n, C, window):
+                                        (8192, 8, 10e-12), (8192, 64, 10e-12)])
+def test_pass_fp64_exact(gpu_ctx, n, C, window, exchange):
     sw, si, E, args = crystal_case(n=n, window=window)
     Eo, apo, smo, cnt = oracle.nee_heun(E, *args, n_cols=4)
-    Eg, apg, smg = _run(gpu_ctx, E, args, n, "double", C)
+    Eg, apg, smg = _run(gpu_ctx, E, args, n, "double", C, exchange=exchange)
     assert np.array_equal(smg, smo)
     assert rel_l2(Eg, Eo) < 1e-9
     assert rel_l2(apg, apo) < 1e-9
 
 
-@pytest.mark.parametrize("n,C,window", [(256, 2, 2e-12), (1024, 4, 4e-12), (4096, 8, 8e-12)])
-def test_pass_fp32(gpu_ctx, n, C, window):
+@pytest.mark.parametrize("exchange", EXCHANGES)
+@pytest.mark.parametrize("n,C,window", [(256, 2, 2e-12), (1024, 4, 4e-12), (4096, 8, 8e-12),
+                                        (8192, 64, 10e-12)])
+def test_pass_fp32(gpu_ctx, n, C, window, exchange):
     sw, si, E, args = crystal_case(n=n, window=window)
     Eo, apo, smo, cnt = oracle.nee_heun(E, *args, n_cols=4)
-    Eg, apg, smg = _run(gpu_ctx, E, args, n, "single", C)
+    Eg, apg, smg = _run(gpu_ctx, E, args, n, "single", C, exchange=exchange)
     assert np.array_equal(smg, smo)
     assert rel_l2(Eg, Eo) < 2e-4
 
 
-def test_round_trips_fp64(gpu_ctx):
+@pytest.mark.parametrize("exchange", EXCHANGES)
+def test_round_trips_fp64(gpu_ctx, exchange):
     cfg, b = opo_case()
     si, ops, sw = b["solver"], b["ops"], b["simwin"]
     trips = 3
@@ -50,7 +59,7 @@ def test_round_trips_fp64(gpu_ctx):
     for C in (1, 4):
         run = OPOBatch([dict(solver=si, field0=ops["field0"], pump=ops["pump"], cav=ops["cav"],
                              ocm=ops["ocm"], delay=cfg.delay)], precision="double",
-                       trip_cap=trips, ctx=gpu_ctx, tune={"cluster": C})
+                       trip_cap=trips, ctx=gpu_ctx, tune={"cluster": C, "exchange": exchange})
         run.launch(trips)
         run.sync()
         r = run.read(0)
@@ -141,18 +150,20 @@ def test_config2_full_size_invariants(gpu_ctx):
 
 
 # ---------------------------------------------------------------- RK23 (ode23 variant)
-def _run_rk(ctx, E, args, n, precision, C=None, cols=4):
+def _run_rk(ctx, E, args, n, precision, C=None, cols=4, exchange="auto"):
+    tune = {"cluster": C, "exchange": exchange} if C else None
     return OPOmexBatch(E, *args, np.zeros((n, cols), complex), np.zeros(args[5]), ctx=ctx,
-                       precision=precision, tune={"cluster": C} if C else None, method="rk23")
+                       precision=precision, tune=tune, method="rk23")
 
 
+@pytest.mark.parametrize("exchange", EXCHANGES)
 @pytest.mark.parametrize("n,C,window", [(256, 1, 2e-12), (256, 2, 2e-12), (1024, 4, 4e-12),
                                         (4096, 8, 8e-12), (8192, 16, 10e-12)])
-def test_rk23_pass_fp64_exact(gpu_ctx, n, C, window):
+def test_rk23_pass_fp64_exact(gpu_ctx, n, C, window, exchange):
     sw, si, E, args = crystal_case(n=n, window=window)
     Eo, apo, smo, cnt = oracle.rk23(E, *args, n_cols=4)
     assert cnt[0] > cnt[1]
-    Eg, apg, smg = _run_rk(gpu_ctx, E, args, n, "double", C)
+    Eg, apg, smg = _run_rk(gpu_ctx, E, args, n, "double", C, exchange=exchange)
     assert np.array_equal(smg, smo)
     assert rel_l2(Eg, Eo) < 1e-9
     assert rel_l2(apg, apo) < 1e-9
@@ -179,7 +190,8 @@ def test_rk23_config1_grid_fp64_and_fp32(gpu_ctx):
     assert rel_l2(Ef, Eo) < 5e-4
 
 
-def test_rk23_round_trips_fp64(gpu_ctx):
+@pytest.mark.parametrize("exchange", EXCHANGES)
+def test_rk23_round_trips_fp64(gpu_ctx, exchange):
     cfg, b = opo_case()
     si, ops, sw = b["solver"], b["ops"], b["simwin"]
     trips = 3
@@ -189,10 +201,45 @@ def test_rk23_round_trips_fp64(gpu_ctx):
     for C in (1, 4):
         run = OPOBatch([dict(solver=si, field0=ops["field0"], pump=ops["pump"], cav=ops["cav"],
                              ocm=ops["ocm"], delay=cfg.delay, method="rk23")],
-                       precision="double", trip_cap=trips, ctx=gpu_ctx, tune={"cluster": C})
+                       precision="double", trip_cap=trips, ctx=gpu_ctx,
+                       tune={"cluster": C, "exchange": exchange})
         run.launch(trips)
         run.sync()
         r = run.read(0)
         assert np.array_equal(r["trip_log"][:, 2:].astype(int), o["steps"])
         assert rel_l2(r["field"], oracle_next_input(b, o["E_cav"])) < 1e-9
         run.close()
+
+
+@pytest.mark.parametrize("exchange", EXCHANGES)
+def test_wait_limit_gives_up_and_the_next_launch_recovers(gpu_ctx, exchange):
+    """A zero wait limit makes every member of the cluster leave through its
+    own bounded poll at the first wait longer than a few rounds (the kernel
+    exits normally with a time-out status); after the limit is restored the
+    same object reproduces a fresh run bit for bit."""
+    from optifax_amd import OfxError
+    cfg, b = opo_case()
+    si, ops = b["solver"], b["ops"]
+    inst = dict(solver=si, field0=ops["field0"], pump=ops["pump"], cav=ops["cav"],
+                ocm=ops["ocm"], delay=cfg.delay)
+    tune = {"cluster": 8, "exchange": exchange}
+    ref = OPOBatch([inst], precision="double", trip_cap=4, ctx=gpu_ctx, tune=tune)
+    ref.launch(2)
+    ref.sync()
+    want = ref.read(0)
+    ref.close()
+    run = OPOBatch([inst], precision="double", trip_cap=8, ctx=gpu_ctx, tune=tune)
+    f0 = run.read(0)["field"]
+    run.set_wait_limit(0.0)
+    with pytest.raises(OfxError, match="timed out"):
+        run.launch(2)
+        run.sync()
+    assert np.array_equal(run.read(0)["field"], f0)
+    run.set_wait_limit(10.0)
+    run.write_field(0, f0)
+    run.launch(2)
+    run.sync()
+    got = run.read(0)
+    assert np.array_equal(got["field"], want["field"])
+    assert np.array_equal(got["oc_spec"], want["oc_spec"])
+    run.close()
