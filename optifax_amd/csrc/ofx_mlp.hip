@@ -20,9 +20,10 @@
 // the producing GEMM's epilogue, 8-byte stores of 4 consecutive rows).
 //
 // Tiles: 64 x 64 outputs per workgroup of 4 waves (each wave one 32 x 32 MFMA
-// block), K in slices of 64 double-buffered in LDS with the next slice's
-// global loads in flight during the current slice's MFMAs.  For the 512-row
-// batch and width 2048, a layer is 8 x 32 = 256 workgroups: one per CU.
+// block), K in slices of 64 double-buffered in LDS, with the global loads of
+// the slices two steps ahead in flight during the current slice's MFMAs.
+// For the 512-row batch and width 2048, a layer is 8 x 32 = 256 workgroups:
+// one per CU.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -53,6 +54,15 @@ __device__ __forceinline__ float gelu_d(float z) {
          z * 0.39894228040143268f * __expf(-0.5f * z * z);
 }
 
+// A pointer that is the same in every lane, marked as such (so that buffer
+// descriptors built from it live in scalar registers)
+__device__ __forceinline__ void* uniform_ptr(const void* p) {
+  const uint64_t v = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return (void*)(((uint64_t)hi << 32) | lo);
+}
+
 struct GemmArgs {
   const bf16* A;  // [M][lda], k-contiguous
   const bf16* B;  // [N][ldb], k-contiguous
@@ -76,32 +86,54 @@ template <int EPI>
 __global__ void __launch_bounds__(512) gemm_nt(GemmArgs g) {
   __shared__ __align__(16) bf16 As[2][2][TM * LDK];  // [half][stage]
   __shared__ __align__(16) bf16 Bs[2][2][TN * LDK];
-  const int tid = threadIdx.x, half = tid >> 8, th = tid & 255;
+  // (the half index is the same across a wave; readfirstlane lets the compiler
+  // keep what depends on it, such as the slice offsets, in scalar registers)
+  const int tid = threadIdx.x, th = tid & 255;
+  const int half = __builtin_amdgcn_readfirstlane(tid >> 8);
   const int lane = th & 63, w = th >> 6;
   const int wr = w >> 1, wc = w & 1;
   const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
   const int r = th >> 3, c = (th & 7) * 8;
-  const bf16* Ap = g.A + (size_t)(m0 + r) * g.lda + c;
-  const bf16* Bp = g.B + (size_t)(n0 + r) * g.ldb + c;
-  const size_t a32 = (size_t)32 * g.lda, b32 = (size_t)32 * g.ldb;
+  // Tile loads through buffer descriptors of the 64-row A and B panels:
+  // loop-invariant per-thread byte offsets plus the slice offset in an
+  // SGPR, so no address registers are rewritten while loads are in flight
+  // (with 64-bit per-thread addresses the compiler reused registers of
+  // in-flight loads and had to wait for them)
+  const __amdgpu_buffer_rsrc_t rA = __builtin_amdgcn_make_buffer_rsrc(
+      uniform_ptr(g.A + (size_t)m0 * g.lda), (short)0,
+      __builtin_amdgcn_readfirstlane(TM * g.lda * 2), 0x00020000);
+  const __amdgpu_buffer_rsrc_t rB = __builtin_amdgcn_make_buffer_rsrc(
+      uniform_ptr(g.B + (size_t)n0 * g.ldb), (short)0,
+      __builtin_amdgcn_readfirstlane(TN * g.ldb * 2), 0x00020000);
+  const int oa0 = 2 * (r * g.lda + c), oa1 = 2 * ((r + 32) * g.lda + c);
+  const int ob0 = 2 * (r * g.ldb + c), ob1 = 2 * ((r + 32) * g.ldb + c);
   const int so0 = r * LDK + c, so1 = (r + 32) * LDK + c;
   const int nk = g.K / TK;
   const int iters = (nk + 1) >> 1;  // both halves run the same number of iterations
-  bf16x8 ra0, ra1, rb0, rb1;
-  {
-    const int kt = half;
-    if (kt < nk) {
-      const size_t ko = (size_t)kt * TK;
-      ra0 = *reinterpret_cast<const bf16x8*>(Ap + ko);
-      ra1 = *reinterpret_cast<const bf16x8*>(Ap + a32 + ko);
-      rb0 = *reinterpret_cast<const bf16x8*>(Bp + ko);
-      rb1 = *reinterpret_cast<const bf16x8*>(Bp + b32 + ko);
-      *reinterpret_cast<bf16x8*>(&As[half][0][so0]) = ra0;
-      *reinterpret_cast<bf16x8*>(&As[half][0][so1]) = ra1;
-      *reinterpret_cast<bf16x8*>(&Bs[half][0][so0]) = rb0;
-      *reinterpret_cast<bf16x8*>(&Bs[half][0][so1]) = rb1;
-    }
-  }
+  // Global loads run two of this half's slices ahead of the MFMAs: slice
+  // kt + 2 waits in one register set while slice kt + 4 is in flight in the
+  // other, so each load has two iterations to arrive.
+  bf16x8 s0[4], s1[4];
+  auto load = [&](int kt, bf16x8 (&r)[4]) {
+    const int so = kt * TK * 2;
+    r[0] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rA, oa0, so, 0));
+    r[1] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rA, oa1, so, 0));
+    r[2] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rB, ob0, so, 0));
+    r[3] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rB, ob1, so, 0));
+  };
+  auto stash = [&](int stage, const bf16x8 (&r)[4]) {
+    *reinterpret_cast<bf16x8*>(&As[half][stage][so0]) = r[0];
+    *reinterpret_cast<bf16x8*>(&As[half][stage][so1]) = r[1];
+    *reinterpret_cast<bf16x8*>(&Bs[half][stage][so0]) = r[2];
+    *reinterpret_cast<bf16x8*>(&Bs[half][stage][so1]) = r[3];
+  };
+  // (slices past the end are clamped to the last one: the loads and LDS
+  // stores stay unconditional, which keeps the compiler's wait counts exact;
+  // their MFMAs are skipped)
+  const int kl = nk - 1;
+  load(min(half, kl), s0);
+  stash(0, s0);
+  load(min(half + 2, kl), s0);
   __syncthreads();
   f32x16 acc;
 #pragma unroll
@@ -109,19 +141,13 @@ __global__ void __launch_bounds__(512) gemm_nt(GemmArgs g) {
   // MFMA operands: lane l supplies row (A) / column (B) l % 32, k = 8 (l / 32) .. + 7
   const int ao = (32 * wr + (lane & 31)) * LDK + 8 * (lane >> 5);
   const int bo = (32 * wc + (lane & 31)) * LDK + 8 * (lane >> 5);
-  for (int it = 0; it < iters; it++) {
+  // iteration it: MFMAs on slice kt = 2 it + half (LDS stage it & 1); `held`
+  // holds slice kt + 2, `next` receives slice kt + 4
+  auto step = [&](int it, bf16x8 (&held)[4], bf16x8 (&next)[4]) {
     const int cur = it & 1;
     const int kt = 2 * it + half;
-    const bool have = kt < nk;
-    const bool more = kt + 2 < nk;
-    if (more) {
-      const size_t ko = (size_t)(kt + 2) * TK;
-      ra0 = *reinterpret_cast<const bf16x8*>(Ap + ko);
-      ra1 = *reinterpret_cast<const bf16x8*>(Ap + a32 + ko);
-      rb0 = *reinterpret_cast<const bf16x8*>(Bp + ko);
-      rb1 = *reinterpret_cast<const bf16x8*>(Bp + b32 + ko);
-    }
-    if (have) {
+    load(min(kt + 4, kl), next);
+    if (kt < nk) {
       const bf16* as = &As[half][cur][ao];
       const bf16* bs = &Bs[half][cur][bo];
 #pragma unroll
@@ -131,13 +157,12 @@ __global__ void __launch_bounds__(512) gemm_nt(GemmArgs g) {
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
       }
     }
-    if (more) {
-      *reinterpret_cast<bf16x8*>(&As[half][cur ^ 1][so0]) = ra0;
-      *reinterpret_cast<bf16x8*>(&As[half][cur ^ 1][so1]) = ra1;
-      *reinterpret_cast<bf16x8*>(&Bs[half][cur ^ 1][so0]) = rb0;
-      *reinterpret_cast<bf16x8*>(&Bs[half][cur ^ 1][so1]) = rb1;
-    }
+    stash(cur ^ 1, held);
     __syncthreads();
+  };
+  for (int it = 0; it < iters; it += 2) {
+    step(it, s0, s1);
+    if (it + 1 < iters) step(it + 1, s1, s0);
   }
   // half 1 -> half 0 through LDS (the A buffers: 256 threads x 16 floats = 16 KB)
   float* xch = reinterpret_cast<float*>(&As[0][0][0]);
