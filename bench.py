@@ -114,7 +114,7 @@ def main_warmstart(args):
     """Time to steady state (trip-to-trip change of the output-coupled
     spectrum < 1e-4, 1000-trip budget): plain FP32 round trips from noise
     (phase "plain") and the surrogate warm start (phase "warm": trajectory
-    data, training, surrogate trips, judged plain trips); "all" runs both."""
+    data, training, surrogate trips, gated plain trips); "all" runs both."""
     from optifax_amd import warmstart as ws
     log = lambda m: print(m, file=sys.stderr, flush=True)
     cfgs = warmstart_configs(args.ws_instances)
@@ -158,7 +158,7 @@ def main_warmstart(args):
                        surrogate_trips_median=float(np.median(warm["surrogate_trips"])),
                        converged_warm=int((warm["status"] == 2).sum()),
                        fallbacks=int(warm["fallback"].sum()),
-                       judge_err_median=float(np.median(warm["judge_err"])),
+                       gate_err_median=float(np.median(warm["gate_err"])),
                        train_pairs=int(x.shape[0]), data_seconds=t_data, train_seconds=t_train,
                        train_loss_last=float(np.mean(losses[-50:])), val_loss=val,
                        mlp_train_device_ms=ms)
@@ -202,7 +202,7 @@ def main_warmstart(args):
                 arrs[f"{tag}_status"] = r["status"]
                 arrs[f"{tag}_oc"] = np.stack(r["oc"])
         if warm is not None:
-            arrs.update(sur_trips=warm["surrogate_trips"], judge_err=warm["judge_err"])
+            arrs.update(sur_trips=warm["surrogate_trips"], gate_err=warm["gate_err"])
         np.savez(args.ws_out, **arrs)
     print(json.dumps(out))
 
@@ -320,8 +320,8 @@ def main():
         print(json.dumps({"copy_bytes": x.numel() * 4, "copies": 3}))
         return
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(instances shard across GPUs by design, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: one GPU only; --gpus {args.gpus} is not supported (independent "
+              "instances would shard across GPUs, one process per GPU)", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "batch":

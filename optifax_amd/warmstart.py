@@ -20,8 +20,8 @@ energy and poling period, and replaces it for the first trips of a solve:
        "anderson"; measured, it converged to the trivial zero-signal root);
     2. handover: the bins become a full-grid field (band-limited
        interpolation) plus the exact pump;
-    3. judge: the first plain trip's band output is compared with the
-       surrogate's prediction for the same input; above `judge_tol` the
+    3. gate: the first plain trip's band output is compared with the
+       surrogate's prediction for the same input; above `gate_tol` the
        surrogate state is discarded and the instance restarts plainly from its
        noise seed (fallback);
     4. plain FP32 round trips until the trip-to-trip change of the
@@ -410,12 +410,12 @@ def _anderson_band(sur, cav, X, pe, pr, max_trips, tol, depth=5):
 
 
 def solve_warm(cfgs, insts, builds, sur: BandSurrogate, conv_tol=1e-4, budget=1000,
-               max_surrogate_trips=100, surrogate_tol=1e-5, judge_tol=0.3, seed_rel=1e-3,
+               max_surrogate_trips=100, surrogate_tol=1e-5, gate_tol=0.3, seed_rel=1e-3,
                chunk=10, log=print, ctx=None, max_seconds=None, accel="growth",
                growth_stop=1.02):
-    """Surrogate trips on the band bins, handover, judged plain trips.
+    """Surrogate trips on the band bins, handover, gated plain trips.
     Returns dict(seconds, surrogate_seconds, surrogate_trips[], trips[],
-    status[], judge_err[], fallback[])."""
+    status[], gate_err[], fallback[])."""
     t0 = time.time()
     sw = builds[0]["simwin"]
     sb = band_bins(sw)
@@ -455,7 +455,7 @@ def solve_warm(cfgs, insts, builds, sur: BandSurrogate, conv_tol=1e-4, budget=10
         fields.append(np.fft.ifft(Xk))
     warm = [dict(inst, field0=f) for inst, f in zip(insts, fields)]
     run = OPOBatch(warm, trip_cap=budget, ctx=ctx)
-    # judge: one plain trip vs the surrogate's prediction for the same input
+    # gate: one plain trip vs the surrogate's prediction for the same input
     F_pred = sur(X, pe, pr, cav.lin)
     run.launch(1)
     run.sync(allow_nonfinite=True)
@@ -463,11 +463,11 @@ def solve_warm(cfgs, insts, builds, sur: BandSurrogate, conv_tol=1e-4, budget=10
     F_plain = cav.band_out(nxt)
     jerr = np.linalg.norm(F_pred - F_plain, axis=1) / np.maximum(np.linalg.norm(F_plain, axis=1),
                                                                  1e-300)
-    fallback = jerr > judge_tol
+    fallback = jerr > gate_tol
     for i in np.where(fallback)[0]:
         run.write_field(i, insts[i]["field0"])      # restart plainly from the noise seed
     if log:
-        log(f"  warm: {t_sur:.2f} s of surrogate trips (max {s_trips.max()}), judge error "
+        log(f"  warm: {t_sur:.2f} s of surrogate trips (max {s_trips.max()}), gate error "
             f"median {np.median(jerr):.3f}, fallbacks {int(fallback.sum())}/{B}")
     conv_at = _run_to_convergence(run, B, conv_tol, budget - 1, chunk, log, "warm", t0,
                                   max_seconds)
@@ -476,5 +476,5 @@ def solve_warm(cfgs, insts, builds, sur: BandSurrogate, conv_tol=1e-4, budget=10
     return dict(seconds=time.time() - t0, conv_seconds=conv_at, surrogate_seconds=t_sur,
                 surrogate_trips=s_trips,
                 trips=np.array([r["trips_done"] for r in rs]),
-                status=np.array([r["status"] for r in rs]), judge_err=jerr, fallback=fallback,
+                status=np.array([r["status"] for r in rs]), gate_err=jerr, fallback=fallback,
                 fields=[r["field"] for r in rs], oc=[r["oc_spec"] for r in rs])

@@ -1,7 +1,7 @@
 """Parity of the HIP path (libofx_hip.so on the GPU, through the C ABI)
 against the FP64 oracle.  FP64 device mode must reproduce the oracle's
 adaptive step sequence exactly and its fields to rounding; FP32 mode follows
-the same step sequence with the error documented in DESIGN.md."""
+the same step sequence, with the FP32 rounding error bounded per test."""
 import numpy as np
 import pytest
 

@@ -1,5 +1,5 @@
 """Surrogate warm start on the device: what the trained network predicts
-better than linear propagation alone, and the judge's fallback."""
+better than linear propagation alone, and the gate's fallback."""
 import math
 
 import numpy as np
@@ -50,8 +50,8 @@ def test_trained_surrogate_beats_linear_propagation(gpu_ctx):
     net.close()
 
 
-def test_judge_falls_back_to_the_plain_solve(gpu_ctx):
-    """An untrained network is rejected by the judge: every instance restarts
+def test_gate_falls_back_to_the_plain_solve(gpu_ctx):
+    """An untrained network is rejected by the gate: every instance restarts
     from its own noise field and then follows the plain solve trip for trip."""
     from optifax_amd.surrogate import SurrogateMLP
     cfgs = _cfgs(2, 3)
@@ -67,7 +67,7 @@ def test_judge_falls_back_to_the_plain_solve(gpu_ctx):
     plain.launch(5, 1e-4)
     plain.sync()
     for i in range(2):
-        # the warm batch ran 1 judged trip, then 5 from the noise field
+        # the warm batch ran 1 gated trip, then 5 from the noise field
         assert np.array_equal(warm["fields"][i], plain.read(i)["field"])
     plain.close()
     net.close()
