@@ -636,7 +636,16 @@ struct Geo {
   cpx<R>*B0, *B1, *B2, *B3, *B4;  // B3, B4: RK23 only
   const cpx<R>* tw1;  // LDS: exp(-2 pi i j / n1), j < n1
   const cpx<R>* tw2;  // LDS: exp(-2 pi i j / n2), j < n2
+  const cpx<R>* twf;  // LDS: exp(-2 pi i j / n), j < n1
 };
+
+// Four-step twiddle exp(-2 pi i m / n), m = q n1 + r < n, as the product of
+// the LDS entries exp(-2 pi i q / n2) and exp(-2 pi i r / n) (no access to the
+// n-entry table in global memory).
+template <typename R>
+OFX_DEV cpx<R> fourstep_tw(const Geo<R>& g, int m) {
+  return cmul(g.tw2[m >> g.lg1], g.twf[m & (g.n1 - 1)]);
+}
 
 template <typename R>
 OFX_DEV int gidx(const Geo<R>& g, int lc, int r) {
@@ -654,17 +663,16 @@ OFX_DEV double dw_of(int k, int n, double dt) {
 // k2*n1 + c, twiddled).
 template <typename R>
 OFX_DEV void fwd_cols_to_t1(const Geo<R>& g, cpx<R>* src, cpx<R>* other, u64* t1,
-                            const cpx<R>* tw, int nmax, unsigned tag) {
+                            unsigned tag) {
   typedef cpx<R> C;
   typedef Tagged<C> TT;
   C* res = lds_fft<R, -1>(src + g.HW * g.sT, other + g.HW * g.sT, g.lg2, g.W, g.sT, g.tw2, 1);
-  const int tws = nmax >> g.lgn;
   const int tot = g.W * g.n2;
   for (int idx = OFX_TID; idx < tot; idx += OFX_NT) {
     const int cl = idx & (g.W - 1), k2 = idx >> g.lgW;
     const int c = g.c0 + cl;
     C v = res[cl * g.sT + k2];
-    v = cmul(v, tw[(c * k2) * tws]);
+    v = cmul(v, fourstep_tw(g, c * k2));
     const TagPtr t = tp_plane<C>(g.plain, t1, g.n, (size_t)(k2 * g.n1 + c));
     TT::put(const_cast<u64*>(t.p), t.ws, t.plain, v, tag);
   }
@@ -677,7 +685,6 @@ template <typename R>
 OFX_DEV cpx<R>* t1_to_rows_fwd(const Geo<R>& g, const u64* t1, unsigned tag, cpx<R>* dst,
                                cpx<R>* other, Spin& sp, bool& ok) {
   typedef cpx<R> C;
-  typedef Tagged<C> TT;
   ok = read_tagged<C>(
       g.Rr * g.n1, tag,
       [&](int idx) {
@@ -699,7 +706,6 @@ OFX_DEV bool gather_err_and_rows(const ClusterDesc& cd, int par, const Geo<R>& g
                                  const u64* kx, cpx<R>* K1, unsigned tag, double* emax,
                                  double* red, Spin& sp) {
   typedef cpx<R> C;
-  typedef Tagged<C> TT;
   const int nw = slot_waves();
   const int ns = cd.C * nw;
   const int nr = g.Rr * g.n1;
@@ -767,17 +773,16 @@ OFX_DEV bool gather_err_and_rows(const ClusterDesc& cd, int par, const Geo<R>& g
 // tagged buffer t2 (column-major c*n2 + k2).
 template <typename R>
 OFX_DEV void rows_inv_to_t2(const Geo<R>& g, cpx<R>* src, cpx<R>* other, u64* t2,
-                            const cpx<R>* tw, int nmax, unsigned tag) {
+                            unsigned tag) {
   typedef cpx<R> C;
   typedef Tagged<C> TT;
   C* res = lds_fft<R, +1>(src, other, g.lg1, g.Rr, g.sF, g.tw1, 1);
-  const int tws = nmax >> g.lgn;
   const int tot = g.Rr * g.n1;
   for (int idx = OFX_TID; idx < tot; idx += OFX_NT) {
     const int k2l = idx & (g.Rr - 1), c = idx >> g.lgR;
     const int k2 = g.k20 + k2l;
     C v = res[k2l * g.sF + c];
-    v = cmulc(v, tw[(c * k2) * tws]);
+    v = cmulc(v, fourstep_tw(g, c * k2));
     const TagPtr t = tp_plane<C>(g.plain, t2, g.n, (size_t)(c * g.n2 + k2));
     TT::put(const_cast<u64*>(t.p), t.ws, t.plain, v, tag);
   }
@@ -789,7 +794,6 @@ OFX_DEV void rows_inv_to_t2(const Geo<R>& g, cpx<R>* src, cpx<R>* other, u64* t2
 template <typename R>
 OFX_DEV bool t2_to_cols_inv(const Geo<R>& g, const u64* t2, unsigned tag, Spin& sp) {
   typedef cpx<R> C;
-  typedef Tagged<C> TT;
   const int tot = g.LC * g.n2;
   bool ok = read_tagged<C>(
       tot, tag,
@@ -833,8 +837,6 @@ OFX_DEV int rk23_pass(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc
   typedef cpx<R> C;
   typedef Tagged<C> TT;
   const int n = g.n, n1 = g.n1, n2 = g.n2;
-  const C* tw = (const C*)a.twiddle;
-  const int nmax = a.nmax;
   const size_t kxs = (size_t)g.Cn * 6 * n2 * TT::NW;  // one parity half of kx
   const C* wt = (const C*)d.wt;
   const R* ab = (const R*)d.absorb;
@@ -945,7 +947,7 @@ OFX_DEV int rk23_pass(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc
                                      ((size_t)(rank * 2 + side) * 3 + col) * n2 + r);
         TT::put(const_cast<u64*>(t.p), t.ws, t.plain, K1[lc * g.sT + r], tag1);
       }
-      fwd_cols_to_t1<R>(g, K2, K3, t1, tw, nmax, tag1);
+      fwd_cols_to_t1<R>(g, K2, K3, t1, tag1);
       double emax;
       {
         Spin sp = spin_begin(cd, a.timeout_ticks);
@@ -1000,7 +1002,7 @@ OFX_DEV int rk23_pass(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc
       {
         const unsigned x2 = ++sy.nx, tag2 = sy.base + x2;
         u64* t2 = (u64*)cd.t2 + (size_t)(x2 & 1u) * n * TT::NW;
-        rows_inv_to_t2<R>(g, F, Fo, t2, tw, nmax, tag2);
+        rows_inv_to_t2<R>(g, F, Fo, t2, tag2);
         Spin sp = spin_begin(cd, a.timeout_ticks);
         if (!xchg_ready<XM>(cd, sy, a.timeout_ticks) || !t2_to_cols_inv<R>(g, t2, tag2, sp))
           return ST_BARRIER_TIMEOUT;
@@ -1060,19 +1062,20 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
     g.B2 = lds + 2 * bs;
     g.B3 = lds + 3 * bs;  // RK23 only (the launch sizes LDS for 5 buffers then)
     g.B4 = lds + 4 * bs;
-    // transform twiddles of both lengths after the field buffers (the launch
-    // sizes LDS for nbuf * bs + n1 + n2 elements)
+    // twiddles after the field buffers (the launch sizes LDS for
+    // nbuf * bs + 2 n1 + n2 elements): the transforms of both lengths, and
+    // the fine part exp(-2 pi i r / n), r < n1, of the four-step twiddles
     C* twl = lds + (d.method == METHOD_RK23 ? 5 : 3) * bs;
     const C* twg = (const C*)a.twiddle;
-    const int s1 = a.nmax >> d.lg1, s2 = a.nmax >> d.lg2;
-    for (int j = OFX_TID; j < d.n1 + d.n2; j += OFX_NT)
-      twl[j] = j < d.n1 ? twg[j * s1] : twg[(j - d.n1) * s2];
+    const int s1 = a.nmax >> d.lg1, s2 = a.nmax >> d.lg2, sn = a.nmax >> (d.lg1 + d.lg2);
+    for (int j = OFX_TID; j < 2 * d.n1 + d.n2; j += OFX_NT)
+      twl[j] = j < d.n1 ? twg[j * s1]
+                        : (j < d.n1 + d.n2 ? twg[(j - d.n1) * s2] : twg[(j - d.n1 - d.n2) * sn]);
     g.tw1 = twl;
     g.tw2 = twl + d.n1;
+    g.twf = twl + d.n1 + d.n2;
   }
   const int n = g.n, n1 = g.n1, n2 = g.n2;
-  const C* tw = (const C*)a.twiddle;
-  const int nmax = a.nmax;
   C* field = (C*)d.field;
   // parity halves of the tagged transpose buffers
   auto t1_of = [&](unsigned x) { return (u64*)cd.t1 + (size_t)(x & 1u) * n * TT::NW; };
@@ -1184,7 +1187,7 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
       const unsigned x1 = ++sy.nx, tag1 = sy.base + x1;
       const int par = (int)(x1 & 1u);
       publish_slots<R>(cd, par, rank, 0, bad ? INFINITY : fmin(lmax, DBL_MAX), 0.0, 0.0, tag1);
-      fwd_cols_to_t1<R>(g, X, K, t1_of(x1), tw, nmax, tag1);
+      fwd_cols_to_t1<R>(g, X, K, t1_of(x1), tag1);
       double gmax;
       {
         Spin sp = spin_begin(cd, a.timeout_ticks);
@@ -1247,7 +1250,7 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
       OFX_SYNC();
       {
         const unsigned x2 = ++sy.nx, tag2 = sy.base + x2;
-        rows_inv_to_t2<R>(g, F, Fo, t2_of(x2), tw, nmax, tag2);
+        rows_inv_to_t2<R>(g, F, Fo, t2_of(x2), tag2);
         Spin sp = spin_begin(cd, a.timeout_ticks);
         if (!xchg_ready<XM>(cd, sy, a.timeout_ticks) || !t2_to_cols_inv<R>(g, t2_of(x2), tag2, sp)) {
           status = ST_BARRIER_TIMEOUT;
@@ -1267,7 +1270,7 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
     if (!cavity) continue;
     // ---------------- cavity: OC split, detuning, pump (one FFT pair) ----------------
     const unsigned xc1 = ++sy.nx, tagc1 = sy.base + xc1;
-    fwd_cols_to_t1<R>(g, g.B0, g.B1, t1_of(xc1), tw, nmax, tagc1);
+    fwd_cols_to_t1<R>(g, g.B0, g.B1, t1_of(xc1), tagc1);
     {
       bool ok = xchg_ready<XM>(cd, sy, a.timeout_ticks);
       C* F = g.B0;
@@ -1307,7 +1310,7 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
       const unsigned xc2 = ++sy.nx, tagc2 = sy.base + xc2;
       const int par = (int)(xc2 & 1u);
       publish_slots<R>(cd, par, rank, 1, acc[0], acc[1], acc[2], tagc2);
-      rows_inv_to_t2<R>(g, F, Fo, t2_of(xc2), tw, nmax, tagc2);
+      rows_inv_to_t2<R>(g, F, Fo, t2_of(xc2), tagc2);
       double tot[3];
       ok = xchg_ready<XM>(cd, sy, a.timeout_ticks);
       if (ok) {
