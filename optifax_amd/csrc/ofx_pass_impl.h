@@ -637,6 +637,8 @@ struct Geo {
   const cpx<R>* tw1;  // LDS: exp(-2 pi i j / n1), j < n1
   const cpx<R>* tw2;  // LDS: exp(-2 pi i j / n2), j < n2
   const cpx<R>* twf;  // LDS: exp(-2 pi i j / n), j < n1
+  const cpx<R>* wl;   // LDS: exp(i w0 t_l), l < n1
+  const cpx<R>* wh;   // LDS: exp(i w0 t_{h n1}), h < n2
 };
 
 // Four-step twiddle exp(-2 pi i m / n), m = q n1 + r < n, as the product of
@@ -650,6 +652,13 @@ OFX_DEV cpx<R> fourstep_tw(const Geo<R>& g, int m) {
 template <typename R>
 OFX_DEV int gidx(const Geo<R>& g, int lc, int r) {
   return ((g.c0 - g.HW + lc) + g.n1 * r) & (g.n - 1);
+}
+
+// exp(i w0 t_j) at local column lc, row r, from the two LDS factors
+template <typename R>
+OFX_DEV cpx<R> wt_at(const Geo<R>& g, int lc, int r) {
+  const int j = gidx(g, lc, r);
+  return cmul(g.wh[j >> g.lg1], g.wl[j & (g.n1 - 1)]);
 }
 
 // relative angular frequency of FFT-order bin k
@@ -838,7 +847,6 @@ OFX_DEV int rk23_pass(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc
   typedef Tagged<C> TT;
   const int n = g.n, n1 = g.n1, n2 = g.n2;
   const size_t kxs = (size_t)g.Cn * 6 * n2 * TT::NW;  // one parity half of kx
-  const C* wt = (const C*)d.wt;
   const R* ab = (const R*)d.absorb;
   const R cc = (R)(0.5 / d.dt / d.w0);
   C* A0 = g.B0;
@@ -866,9 +874,9 @@ OFX_DEV int rk23_pass(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc
         su = cadd(su, cscale(kin[p + g.sT], cin));
         sd = cadd(sd, cscale(kin[p - g.sT], cin));
       }
-      const C nc = nl_term_e<R>(sc, cmul(wt[gidx(g, lc, r)], eb));
-      const C nu = nl_term_e<R>(su, cmul(wt[gidx(g, lc + 1, r)], eb));
-      const C nd = nl_term_e<R>(sd, cmul(wt[gidx(g, lc - 1, r)], eb));
+      const C nc = nl_term_e<R>(sc, cmul(wt_at(g, lc, r), eb));
+      const C nu = nl_term_e<R>(su, cmul(wt_at(g, lc + 1, r), eb));
+      const C nd = nl_term_e<R>(sd, cmul(wt_at(g, lc - 1, r), eb));
       out[p] = k_term<R>(nc, nu, nd, gk, cc);
     }
     OFX_SYNC();
@@ -893,9 +901,9 @@ OFX_DEV int rk23_pass(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc
           const C sc = cadd(A0[p], cscale(K2[p], a2));
           const C su = cadd(A0[p + g.sT], cscale(K2[p + g.sT], a2));
           const C sd = cadd(A0[p - g.sT], cscale(K2[p - g.sT], a2));
-          const C nc = nl_term_e<R>(sc, cmul(wt[gidx(g, lc, r)], eb));
-          const C nu = nl_term_e<R>(su, cmul(wt[gidx(g, lc + 1, r)], eb));
-          const C nd = nl_term_e<R>(sd, cmul(wt[gidx(g, lc - 1, r)], eb));
+          const C nc = nl_term_e<R>(sc, cmul(wt_at(g, lc, r), eb));
+          const C nu = nl_term_e<R>(su, cmul(wt_at(g, lc + 1, r), eb));
+          const C nd = nl_term_e<R>(sd, cmul(wt_at(g, lc - 1, r), eb));
           const C k3 = k_term<R>(nc, nu, nd, gk, cc);
           K3[p] = k3;
           Eb[p] = cadd(cadd(A0[p], cadd(cscale(k3, a3), cscale(K2[p], a4))), cscale(K1[p], a5));
@@ -912,9 +920,9 @@ OFX_DEV int rk23_pass(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc
           const int lc = HW + (idx >> g.lg2), r = idx & (n2 - 1);
           const int p = lc * g.sT + r;
           const C e = Eb[p];
-          const C nc = nl_term_e<R>(e, cmul(wt[gidx(g, lc, r)], eb));
-          const C nu = nl_term_e<R>(Eb[p + g.sT], cmul(wt[gidx(g, lc + 1, r)], eb));
-          const C nd = nl_term_e<R>(Eb[p - g.sT], cmul(wt[gidx(g, lc - 1, r)], eb));
+          const C nc = nl_term_e<R>(e, cmul(wt_at(g, lc, r), eb));
+          const C nu = nl_term_e<R>(Eb[p + g.sT], cmul(wt_at(g, lc + 1, r), eb));
+          const C nd = nl_term_e<R>(Eb[p - g.sT], cmul(wt_at(g, lc - 1, r), eb));
           const C k4 = k_term<R>(nc, nu, nd, gk, cc);
           const C k1 = K1[p], k2 = K2[p], k3 = K3[p];
           const C a1s = cadd(cadd(cadd(A0[p], cadd(cscale(k4, b1), cscale(k3, b2))), cscale(k2, b3)),
@@ -1062,18 +1070,36 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
     g.B2 = lds + 2 * bs;
     g.B3 = lds + 3 * bs;  // RK23 only (the launch sizes LDS for 5 buffers then)
     g.B4 = lds + 4 * bs;
-    // twiddles after the field buffers (the launch sizes LDS for
-    // nbuf * bs + 2 n1 + n2 elements): the transforms of both lengths, and
-    // the fine part exp(-2 pi i r / n), r < n1, of the four-step twiddles
+    // tables after the field buffers (the launch sizes LDS for
+    // nbuf * bs + 3 n1 + 2 n2 elements): the transforms' twiddles of both
+    // lengths, the fine part exp(-2 pi i r / n), r < n1, of the four-step
+    // twiddles, and the two factors of exp(i w0 t_j) (j = h n1 + l:
+    // wt[j] = wt[h n1] wt[l], since t_j is linear in j within each half of
+    // the fftshifted axis and n/2 is a multiple of n1)
     C* twl = lds + (d.method == METHOD_RK23 ? 5 : 3) * bs;
     const C* twg = (const C*)a.twiddle;
+    const C* wtg = (const C*)d.wt;
     const int s1 = a.nmax >> d.lg1, s2 = a.nmax >> d.lg2, sn = a.nmax >> (d.lg1 + d.lg2);
-    for (int j = OFX_TID; j < 2 * d.n1 + d.n2; j += OFX_NT)
-      twl[j] = j < d.n1 ? twg[j * s1]
-                        : (j < d.n1 + d.n2 ? twg[(j - d.n1) * s2] : twg[(j - d.n1 - d.n2) * sn]);
+    const int n1 = d.n1, n2 = d.n2;
+    for (int j = OFX_TID; j < 3 * n1 + 2 * n2; j += OFX_NT) {
+      C v;
+      if (j < n1)
+        v = twg[j * s1];
+      else if (j < n1 + n2)
+        v = twg[(j - n1) * s2];
+      else if (j < 2 * n1 + n2)
+        v = twg[(j - n1 - n2) * sn];
+      else if (j < 3 * n1 + n2)
+        v = wtg[j - 2 * n1 - n2];
+      else
+        v = wtg[(j - 3 * n1 - n2) * n1];
+      twl[j] = v;
+    }
     g.tw1 = twl;
-    g.tw2 = twl + d.n1;
-    g.twf = twl + d.n1 + d.n2;
+    g.tw2 = twl + n1;
+    g.twf = twl + n1 + n2;
+    g.wl = twl + 2 * n1 + n2;
+    g.wh = twl + 3 * n1 + n2;
   }
   const int n = g.n, n1 = g.n1, n2 = g.n2;
   C* field = (C*)d.field;
@@ -1081,7 +1107,6 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
   auto t1_of = [&](unsigned x) { return (u64*)cd.t1 + (size_t)(x & 1u) * n * TT::NW; };
   auto t2_of = [&](unsigned x) { return (u64*)cd.t2 + (size_t)(x & 1u) * n * TT::NW; };
   const double dt = d.dt, w0 = d.w0;
-  const C* wt = (const C*)d.wt;
   const R* ab = (const R*)d.absorb;
   const R cc = (R)(0.5 / dt / w0);
   const int nsteps = d.nsteps;
@@ -1152,9 +1177,9 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
         const C v = A0[p];
         if (lc >= g.HW && lc < g.HW + g.W && !(isfinite((double)v.x) && isfinite((double)v.y)))
           bad = 1;
-        const C nc = nl_term_e<R>(v, cmul(wt[gidx(g, lc, r)], eb0));
-        const C nu = nl_term_e<R>(A0[p + g.sT], cmul(wt[gidx(g, lc + 1, r)], eb0));
-        const C nd = nl_term_e<R>(A0[p - g.sT], cmul(wt[gidx(g, lc - 1, r)], eb0));
+        const C nc = nl_term_e<R>(v, cmul(wt_at(g, lc, r), eb0));
+        const C nu = nl_term_e<R>(A0[p + g.sT], cmul(wt_at(g, lc + 1, r), eb0));
+        const C nd = nl_term_e<R>(A0[p - g.sT], cmul(wt_at(g, lc - 1, r), eb0));
         K[p] = k_term<R>(nc, nu, nd, gk, cc);
       }
       OFX_SYNC();
@@ -1168,9 +1193,9 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
         const C k1 = K[p];
         const C a0 = A0[p];
         const C a1 = cadd(a0, k1);
-        const C nc = nl_term_e<R>(a1, cmul(wt[gidx(g, lc, r)], eb1));
-        const C nu = nl_term_e<R>(cadd(A0[p + g.sT], K[p + g.sT]), cmul(wt[gidx(g, lc + 1, r)], eb1));
-        const C nd = nl_term_e<R>(cadd(A0[p - g.sT], K[p - g.sT]), cmul(wt[gidx(g, lc - 1, r)], eb1));
+        const C nc = nl_term_e<R>(a1, cmul(wt_at(g, lc, r), eb1));
+        const C nu = nl_term_e<R>(cadd(A0[p + g.sT], K[p + g.sT]), cmul(wt_at(g, lc + 1, r), eb1));
+        const C nd = nl_term_e<R>(cadd(A0[p - g.sT], K[p - g.sT]), cmul(wt_at(g, lc - 1, r), eb1));
         const C k2 = k_term<R>(nc, nu, nd, gk, cc);
         C e = cadd(cadd(a0, cscale(k1, R(0.5))), cscale(k2, R(0.5)));
         const double ex = (double)e.x - (double)a1.x, ey = (double)e.y - (double)a1.y;
