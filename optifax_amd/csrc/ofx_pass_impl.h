@@ -639,6 +639,7 @@ struct Geo {
   const cpx<R>* twf;  // LDS: exp(-2 pi i j / n), j < n1
   const cpx<R>* wl;   // LDS: exp(i w0 t_l), l < n1
   const cpx<R>* wh;   // LDS: exp(i w0 t_{h n1}), h < n2
+  const R* ab;        // LDS: absorber factor of own column lc, row r at (lc - HW) n2 + r
 };
 
 // Four-step twiddle exp(-2 pi i m / n), m = q n1 + r < n, as the product of
@@ -847,7 +848,6 @@ OFX_DEV int rk23_pass(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc
   typedef Tagged<C> TT;
   const int n = g.n, n1 = g.n1, n2 = g.n2;
   const size_t kxs = (size_t)g.Cn * 6 * n2 * TT::NW;  // one parity half of kx
-  const R* ab = (const R*)d.absorb;
   const R cc = (R)(0.5 / d.dt / d.w0);
   C* A0 = g.B0;
   C* Eb = g.B1;
@@ -936,7 +936,7 @@ OFX_DEV int rk23_pass(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc
           }
           if (isnan(ex) || isnan(ey)) bad = 1;
           K1[p] = k4;
-          const R f = ab[gidx(g, lc, r)];
+          const R f = g.ab[(lc - HW) * n2 + r];
           K2[p] = f != R(1) ? cscale(e, f) : e;
         }
         OFX_SYNC();
@@ -1100,6 +1100,13 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
     g.twf = twl + n1 + n2;
     g.wl = twl + 2 * n1 + n2;
     g.wh = twl + 3 * n1 + n2;
+    // absorber factors of the own columns, (lc - HW) * n2 + r (W n2 reals
+    // after the tables)
+    R* abl = reinterpret_cast<R*>(twl + 3 * n1 + 2 * n2);
+    const R* abg = (const R*)d.absorb;
+    for (int i = OFX_TID; i < g.W * n2; i += OFX_NT)
+      abl[i] = abg[gidx(g, g.HW + (i >> g.lg2), i & (n2 - 1))];
+    g.ab = abl;
   }
   const int n = g.n, n1 = g.n1, n2 = g.n2;
   C* field = (C*)d.field;
@@ -1107,7 +1114,6 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
   auto t1_of = [&](unsigned x) { return (u64*)cd.t1 + (size_t)(x & 1u) * n * TT::NW; };
   auto t2_of = [&](unsigned x) { return (u64*)cd.t2 + (size_t)(x & 1u) * n * TT::NW; };
   const double dt = d.dt, w0 = d.w0;
-  const R* ab = (const R*)d.absorb;
   const R cc = (R)(0.5 / dt / w0);
   const int nsteps = d.nsteps;
   const double sel = d.sel;
@@ -1206,7 +1212,7 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
         X[p] = e;
       };
       for (int idx = OFX_TID; idx < g.W * n2; idx += OFX_NT)
-        corrector(idx, ab[gidx(g, g.HW + (idx >> g.lg2), idx & (n2 - 1))]);
+        corrector(idx, g.ab[idx]);
       OFX_SYNC();
       // exchange: error records and the speculative forward column transforms of E
       const unsigned x1 = ++sy.nx, tag1 = sy.base + x1;
