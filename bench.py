@@ -260,18 +260,19 @@ def main_surrogate(args):
     print(json.dumps(out))
 
 
-def cpu_baseline(b, attempts_per_trip):
+def cpu_baseline(b, attempts_per_trip, method="heun"):
     """Oracle (FP64, one core) on the first 1 mm of the 5 mm pass; the time per
     adaptive step is scaled by the steps of one full round trip."""
     import oracle
     si = b["solver"]
     sample_steps = si["nSteps"] // 5
     sel = float(sample_steps)
+    solve = oracle.nee_heun if method == "heun" else oracle.rk23
     t0 = time.perf_counter()
-    _, _, _, cnt = oracle.nee_heun(b["ops"]["field0"], si["alpha_w"], si["G33"][:sample_steps],
-                                   si["w0"], si["bdiffw0"], si["h"], sample_steps, si["dt"],
-                                   si["beta_op"], si["maxErr"], si["minErr"], sel, n_cols=0,
-                                   want_stepmods=False)
+    _, _, _, cnt = solve(b["ops"]["field0"], si["alpha_w"], si["G33"][:sample_steps],
+                         si["w0"], si["bdiffw0"], si["h"], sample_steps, si["dt"],
+                         si["beta_op"], si["maxErr"], si["minErr"], sel, n_cols=0,
+                         want_stepmods=False)
     dt = time.perf_counter() - t0
     per_step = dt / max(int(cnt[0]), 1)
     trip_s = per_step * attempts_per_trip
@@ -292,6 +293,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20, help="timed round trips")
     ap.add_argument("--warmup", type=int, default=2, help="untimed round trips")
     ap.add_argument("--precision", default="single", choices=["single", "double"])
+    ap.add_argument("--method", default="heun", choices=["heun", "rk23"],
+                    help="crystal solver of the ring workload (rk23: the ode23 variant)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ws-instances", type=int, default=32)
     ap.add_argument("--ws-train-instances", type=int, default=32)
@@ -337,8 +340,8 @@ def main():
     ctx = Context(0)
     trip_cap = args.warmup + args.steps + 1
     run = OPOBatch([dict(solver=si, field0=ops["field0"], pump=ops["pump"], cav=ops["cav"],
-                         ocm=ops["ocm"], delay=b["delay"])], precision=args.precision,
-                   trip_cap=trip_cap, ctx=ctx)
+                         ocm=ops["ocm"], delay=b["delay"], method=args.method)],
+                   precision=args.precision, trip_cap=trip_cap, ctx=ctx)
     geo = run.geometry(0)
     # warm-up trips (also builds the one-trip graph)
     if args.warmup > 0:
@@ -374,7 +377,8 @@ def main():
         "config": {
             "workload": "synchronously pumped ring OPO, N=2^15, 12 ps window, 1040 nm; 5 mm "
                         "MgO:PPLN 30.5 um; 2 W @ 1 GHz 200 fs sech^2; 5% OC (flat R 0.95); "
-                        "0.3 ps delay",
+                        "0.3 ps delay" + ("" if args.method == "heun" else
+                                          "; RK23 (ode23 variant) crystal solver"),
             "instances": 1,
             "adaptive_steps_per_trip": att / args.steps,
             "rejected_per_trip": rej / args.steps,
@@ -387,12 +391,14 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic("ring_n32768_f32" if args.precision == "single"
-                                        else "ring_n32768_f64", att / args.steps),
+            "traffic": measured_traffic(("ring_n32768_f32" if args.precision == "single"
+                                         else "ring_n32768_f64") +
+                                        ("" if args.method == "heun" else "_rk23"),
+                                        att / args.steps),
         },
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(b, att / args.steps)
+        out["cpu_baseline"] = cpu_baseline(b, att / args.steps, args.method)
     run.close()
     ctx.close()
     print(json.dumps(out))
