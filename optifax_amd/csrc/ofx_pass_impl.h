@@ -279,7 +279,8 @@ OFX_DEV int flag_load(const int* p) {
 #ifdef OFX_EMU
   return __atomic_load_n(p, __ATOMIC_RELAXED);
 #else
-  return __hip_atomic_load(const_cast<int*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load((__attribute__((address_space(1))) int*)const_cast<int*>(p),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #endif
 }
 
@@ -376,7 +377,9 @@ OFX_DEV Spin spin_begin(const ClusterDesc& cd, unsigned long long limit) {
 }
 
 // Pause before the next poll round; false when the cluster is to give up
-// (the clock is read every round, the abort flag every 8th).
+// (the clock is read every round, the abort flag every 8th round from the
+// 8th on: most waits end within a few rounds, and a flag read in the first
+// one would add its own round trip to every wait).
 OFX_DEV bool spin_wait(Spin& s) {
 #ifdef OFX_EMU
   sched_yield();
@@ -392,7 +395,7 @@ OFX_DEV bool spin_wait(Spin& s) {
   __builtin_amdgcn_s_sleep(1);
   const unsigned long long t = wall_clock64();
   if (s.n == 0) s.t0 = t;
-  if ((s.n++ & 7) == 0 && flag_load(s.abort_flag)) return false;
+  if ((++s.n & 7) == 0 && flag_load(s.abort_flag)) return false;
   if (t - s.t0 > s.limit) {
     flag_set(s.abort_flag);
     return false;
@@ -582,8 +585,11 @@ OFX_DEV bool cluster_barrier(const ClusterDesc& cd, unsigned target, unsigned lo
     if (cd.C > 1) {
       __hip_atomic_fetch_add(cd.bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       const unsigned long long t0 = wall_clock64();
+      unsigned polls = 0;
+      // (the abort flag is read every 8th poll: a read per poll would double
+      // the poll period)
       while (__hip_atomic_load(cd.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-        if (flag_load(cd.abort_flag)) {
+        if ((++polls & 7) == 0 && flag_load(cd.abort_flag)) {
           ok = 0;
           break;
         }
