@@ -230,7 +230,11 @@ OFX_DEV cpx<R>* lds_fft(cpx<R>* in, cpx<R>* out, int lg, int nseq, int ss, const
 // carries the tag it expects: the data is its own completion flag.  An
 // exchange therefore costs the stores' trip to the coherent level plus the
 // reader's poll, with no store-completion wait, counter atomic or flag read
-// in between (the counter barrier this replaces paid all three).
+// in between.  That wins when each thread reads one or two samples per
+// exchange (clusters of 64 workgroups and more).  With larger shares the
+// doubled bytes and the repeated polls cost more, and the kernel's other
+// instantiation (XM_BARRIER) stores the samples untagged (plain form, half
+// the bytes) and meets at a counter barrier before reading them once.
 //
 // Tags: tag = base + x, x = exchanges so far in this launch (every member
 // runs the same sequence of exchanges), base = the cluster's epoch word,
