@@ -218,6 +218,53 @@ def mlp_step_flops(batch, n_in_padded, width, n_out):
     return 2.0 * fwd + dgrad
 
 
+def main_steady(args):
+    """Accuracy of the FP32 steady state: eight config-5 instances run to
+    steady state (change between trips < 1e-4) with FP32 fields and with
+    FP64 fields (the FP64 device mode reproduces the FP64 oracle's step
+    sequence exactly and its fields to rounding); value = median relative L2
+    distance of their output-coupled power spectra."""
+    from optifax_amd import warmstart as ws
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    pick = [3, 30, 7, 18, 24, 28, 23, 21]          # the eight fastest to settle in FP32
+    cfgs = [warmstart_configs(32)[i] for i in pick]
+    insts = [ws.opo_instance(c)[0] for c in cfgs]
+    res = {}
+    for prec in ("single", "double"):
+        res[prec] = ws.solve_plain(insts, conv_tol=1e-4, budget=args.steady_budget, chunk=10,
+                                   log=log, precision=prec)
+    s32 = [np.abs(o) ** 2 for o in res["single"]["oc"]]
+    s64 = [np.abs(o) ** 2 for o in res["double"]["oc"]]
+    both = np.isfinite(res["single"]["conv_seconds"]) & np.isfinite(res["double"]["conv_seconds"])
+    rel = np.array([np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(s32, s64)])
+    out = {
+        "metric": METRIC,
+        "value": float(np.median(rel[both])) if both.any() else None,
+        "unit": "relative L2, FP32 vs FP64 steady-state output power spectrum (median)",
+        "n_gpus": 1,
+        "steps": args.steady_budget,
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 vs f64",
+        "data": "synthetic (sech^2 pump + seeded one-photon-per-mode noise)",
+        "config": {
+            "workload": "config 5 at 2^14 points, 8 of the 32 detunings (+50..+110 fs), each run "
+                        "to a change between trips < 1e-4 in FP32 and in FP64",
+            "instances": len(insts),
+            "converged_both": int(both.sum()),
+            "rel_l2_per_instance": [float(x) for x in rel],
+            "trips_f32": res["single"]["trips"].tolist(),
+            "trips_f64": res["double"]["trips"].tolist(),
+            "seconds_f32": res["single"]["seconds"],
+            "seconds_f64": res["double"]["seconds"],
+        },
+    }
+    print(json.dumps(out))
+
+
 def main_surrogate(args):
     """Config 4 shape: 2050 -> 2048 x3 -> 2048 MLP, batch 512, Adam, on a
     synthetic device-resident set of 8192 pairs; steps = Adam steps."""
@@ -303,9 +350,11 @@ def main():
     ap.add_argument("--ws-train-trips", type=int, default=250)
     ap.add_argument("--ws-train-steps", type=int, default=20000)
     ap.add_argument("--ws-out", default="")
+    ap.add_argument("--steady-budget", type=int, default=600,
+                    help="trip budget of each run of the steady workload")
     ap.add_argument("--ws-phase", default="all", choices=["all", "plain", "warm"])
     ap.add_argument("--workload", default="ring",
-                    choices=["ring", "batch", "warmstart", "surrogate"],
+                    choices=["ring", "batch", "warmstart", "surrogate", "steady"],
                     help="ring: one config-2 instance (the headline line); batch: 256 uneven "
                          "instances in one persistent launch")
     ap.add_argument("--tune", default="heuristic", choices=["heuristic", "auto"])
@@ -333,6 +382,8 @@ def main():
         return main_warmstart(args)
     if args.workload == "surrogate":
         return main_surrogate(args)
+    if args.workload == "steady":
+        return main_steady(args)
     from optifax_amd.solver import Context, OPOBatch
     cfg, b = workload()
     si, ops = b["solver"], b["ops"]

@@ -646,18 +646,21 @@ struct Geo {
   cpx<R>*B0, *B1, *B2, *B3, *B4;  // B3, B4: RK23 only
   const cpx<R>* tw1;  // LDS: exp(-2 pi i j / n1), j < n1
   const cpx<R>* tw2;  // LDS: exp(-2 pi i j / n2), j < n2
-  const cpx<R>* twf;  // LDS: exp(-2 pi i j / n), j < n1
+  const cpx<double>* twq;  // LDS, FP64: exp(-2 pi i q / n2), q < n2
+  const cpx<double>* twr;  // LDS, FP64: exp(-2 pi i r / n), r < n1
   const cpx<R>* wl;   // LDS: exp(i w0 t_l), l < n1
   const cpx<R>* wh;   // LDS: exp(i w0 t_{h n1}), h < n2
   const R* ab;        // LDS: absorber factor of own column lc, row r at (lc - HW) n2 + r
 };
 
-// Four-step twiddle exp(-2 pi i m / n), m = q n1 + r < n, as the product of
-// the LDS entries exp(-2 pi i q / n2) and exp(-2 pi i r / n) (no access to the
-// n-entry table in global memory).
+// Four-step twiddle exp(-2 pi i m / n), m = q n1 + r < n, as the FP64
+// product of the LDS entries exp(-2 pi i q / n2) and exp(-2 pi i r / n),
+// rounded once to R (no access to an n-entry table in global memory; a
+// product of two FP32 entries raised the FP32 pass error by half).
 template <typename R>
 OFX_DEV cpx<R> fourstep_tw(const Geo<R>& g, int m) {
-  return cmul(g.tw2[m >> g.lg1], g.twf[m & (g.n1 - 1)]);
+  const cpx<double> a = g.twq[m >> g.lg1], b = g.twr[m & (g.n1 - 1)];
+  return mkc<R>((R)(a.x * b.x - a.y * b.y), (R)(a.x * b.y + a.y * b.x));
 }
 
 template <typename R>
@@ -1080,39 +1083,52 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
     g.B2 = lds + 2 * bs;
     g.B3 = lds + 3 * bs;  // RK23 only (the launch sizes LDS for 5 buffers then)
     g.B4 = lds + 4 * bs;
-    // tables after the field buffers (the launch sizes LDS for
-    // nbuf * bs + 3 n1 + 2 n2 elements): the transforms' twiddles of both
-    // lengths, the fine part exp(-2 pi i r / n), r < n1, of the four-step
-    // twiddles, and the two factors of exp(i w0 t_j) (j = h n1 + l:
-    // wt[j] = wt[h n1] wt[l], since t_j is linear in j within each half of
-    // the fftshifted axis and n/2 is a multiple of n1)
+    // tables after the field buffers (the launch sizes LDS for them): the
+    // transforms' twiddles of both lengths and the two factors of
+    // exp(i w0 t_j) (j = h n1 + l: wt[j] = wt[h n1] wt[l], since t_j is
+    // linear in j within each half of the fftshifted axis and n/2 is a
+    // multiple of n1), 2 n1 + 2 n2 complex; then, 16-byte aligned and in
+    // FP64, the two factors of the four-step twiddle (exp(-2 pi i q / n2),
+    // q < n2, and exp(-2 pi i r / n), r < n1; their product is rounded once,
+    // as a table entry would be); then the own columns' absorber factors
+    // ((lc - HW) * n2 + r, W n2 reals)
     C* twl = lds + (d.method == METHOD_RK23 ? 5 : 3) * bs;
     const C* twg = (const C*)a.twiddle;
     const C* wtg = (const C*)d.wt;
-    const int s1 = a.nmax >> d.lg1, s2 = a.nmax >> d.lg2, sn = a.nmax >> (d.lg1 + d.lg2);
+    const int s1 = a.nmax >> d.lg1, s2 = a.nmax >> d.lg2;
     const int n1 = d.n1, n2 = d.n2;
-    for (int j = OFX_TID; j < 3 * n1 + 2 * n2; j += OFX_NT) {
+    for (int j = OFX_TID; j < 2 * n1 + 2 * n2; j += OFX_NT) {
       C v;
       if (j < n1)
         v = twg[j * s1];
       else if (j < n1 + n2)
         v = twg[(j - n1) * s2];
       else if (j < 2 * n1 + n2)
-        v = twg[(j - n1 - n2) * sn];
-      else if (j < 3 * n1 + n2)
-        v = wtg[j - 2 * n1 - n2];
+        v = wtg[j - n1 - n2];
       else
-        v = wtg[(j - 3 * n1 - n2) * n1];
+        v = wtg[(j - 2 * n1 - n2) * n1];
       twl[j] = v;
     }
     g.tw1 = twl;
     g.tw2 = twl + n1;
-    g.twf = twl + n1 + n2;
-    g.wl = twl + 2 * n1 + n2;
-    g.wh = twl + 3 * n1 + n2;
-    // absorber factors of the own columns, (lc - HW) * n2 + r (W n2 reals
-    // after the tables)
-    R* abl = reinterpret_cast<R*>(twl + 3 * n1 + 2 * n2);
+    g.wl = twl + n1 + n2;
+    g.wh = twl + 2 * n1 + n2;
+    // (the LDS base is 16-byte aligned; pointer arithmetic from it keeps the
+    // accesses in the LDS address space)
+    unsigned char* lbase = reinterpret_cast<unsigned char*>(lds);
+    const size_t toff = (size_t)(reinterpret_cast<unsigned char*>(twl + 2 * n1 + 2 * n2) - lbase);
+    cpx<double>* twd = reinterpret_cast<cpx<double>*>(lbase + ((toff + 15) & ~(size_t)15));
+    {
+      const double TWO_PI = 6.283185307179586232;
+      for (int j = OFX_TID; j < n2 + n1; j += OFX_NT) {
+        const double ang = j < n2 ? -TWO_PI * (double)j / (double)n2
+                                  : -TWO_PI * (double)(j - n2) / (double)(n1 * n2);
+        twd[j] = mkc<double>(cos(ang), sin(ang));
+      }
+    }
+    g.twq = twd;
+    g.twr = twd + n2;
+    R* abl = reinterpret_cast<R*>(twd + n2 + n1);
     const R* abg = (const R*)d.absorb;
     for (int i = OFX_TID; i < g.W * n2; i += OFX_NT)
       abl[i] = abg[gidx(g, g.HW + (i >> g.lg2), i & (n2 - 1))];

@@ -104,10 +104,11 @@ int nbuf_of(int method) { return method == OFX_METHOD_RK23 ? 5 : 3; }
 size_t lds_bytes_for(int n1, int n2, int C, int method, size_t csz) {
   const int W = n1 / C, Rr = n2 / C, LC = W + 2 * halo_of(method);
   const size_t b1 = (size_t)LC * (size_t)(n2 + 1), b2 = (size_t)Rr * (size_t)(n1 + 1);
-  // + the tables (3 n1 + 2 n2 complex) + the own columns' absorber factors
-  // (W n2 reals, half a complex each)
-  return ((size_t)nbuf_of(method) * std::max(b1, b2) + (size_t)(3 * n1 + 2 * n2) +
-          ((size_t)W * (size_t)n2 + 1) / 2) * csz;
+  // + the tables (2 n1 + 2 n2 complex), 16 bytes of alignment, the FP64
+  // four-step factors (n1 + n2 complex doubles) and the own columns' absorber
+  // factors (W n2 reals, half a complex each)
+  return ((size_t)nbuf_of(method) * std::max(b1, b2) + (size_t)(2 * n1 + 2 * n2) +
+          ((size_t)W * (size_t)n2 + 1) / 2) * csz + 16 + (size_t)(n1 + n2) * 16;
 }
 
 // largest cluster size the method allows (RK23 exchanges 3 boundary columns

@@ -304,12 +304,13 @@ def _run_to_convergence(run, n_inst, conv_tol, budget, chunk, log, tag, t_start,
 
 
 def solve_plain(insts, conv_tol=1e-4, budget=1000, chunk=10, log=print, ctx=None,
-                max_seconds=None):
-    """Plain FP32 round trips from each instance's own initial field until
-    converged (or the budget).  Returns dict(seconds, conv_seconds[] (time to
-    steady state per instance, inf if not reached), trips[], status[], ...)."""
+                max_seconds=None, precision="single"):
+    """Plain round trips (FP32 fields unless precision="double") from each
+    instance's own initial field until converged (or the budget).  Returns
+    dict(seconds, conv_seconds[] (time to steady state per instance, inf if
+    not reached), trips[], status[], ...)."""
     t0 = time.time()
-    run = OPOBatch(insts, trip_cap=budget, ctx=ctx)
+    run = OPOBatch(insts, trip_cap=budget, ctx=ctx, precision=precision)
     conv_at = _run_to_convergence(run, len(insts), conv_tol, budget, chunk, log, "plain", t0,
                                   max_seconds)
     rs = [run.read(i) for i in range(len(insts))]
