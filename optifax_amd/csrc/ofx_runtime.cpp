@@ -330,18 +330,24 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
       }
     } else {
       // Spread a small batch over more workgroups: a step's latency is set by
-      // the per-workgroup share until it falls to ~512 points (measured on
-      // N = 2^15: 33 / 27 / 24 us per step at 16 / 32 / 64 workgroups).
+      // the per-workgroup share until it falls to ~256 points (measured on
+      // N = 2^15, FP32 / FP64 us per adaptive step: 20.1 / 24.2 at 32
+      // workgroups, 16.6 / 21.1 at 64, 15.5 / 19.6 at 128 with 256-thread
+      // workgroups).
       const int slots_est = ctx->info.cus;
       while (C * 2 <= cmax && (long long)n_inst * C * 2 <= slots_est &&
-             C * 2 <= 64 && n / (C * 2) >= 512)
+             C * 2 <= 128 && n / (C * 2) >= 256)
         C *= 2;
     }
     r->inst_C[i] = C;
     r->inst_n1[i] = n1;
   }
   r->nmax = nmax;
-  r->block = (tune && tune->block) ? tune->block : 512;
+  int cmin0 = 1 << 30;
+  for (int i = 0; i < n_inst; i++) cmin0 = std::min(cmin0, r->inst_C[i]);
+  // 256-thread workgroups once every cluster has 128 members or more (a
+  // share of <= 256 points per workgroup), 512 below
+  r->block = (tune && tune->block) ? tune->block : (cmin0 >= 128 ? 256 : 512);
   const int exchange = tune ? tune->exchange : 0;
   if (exchange < 0 || exchange > 2) {
     delete r;
