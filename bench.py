@@ -265,6 +265,76 @@ def main_steady(args):
     print(json.dumps(out))
 
 
+def main_esd(args):
+    """Config 2 as specified: 300 round trips from noise with FP32 fields and
+    with FP64 fields (the FP64 device mode reproduces the FP64 oracle's step
+    sequence exactly and its fields to rounding; the oracle itself needs ~55 s
+    per trip here).  value = relative L2 distance of the run-averaged
+    output-coupled energy spectral densities, mean over all trips of
+    |E_oc(w)|^2 (OpticalSim.averageSimPulses, OpticalSim.m:302-307: the sum of
+    each trip's output ESD divided by the number of trips), accumulated on
+    the device."""
+    from optifax_amd.solver import Context, OPOBatch
+    cfg, b = workload()
+    si, ops = b["solver"], b["ops"]
+    ctx = Context(0)
+    trips = args.esd_trips
+    res = {}
+    for prec in ("single", "double"):
+        run = OPOBatch([dict(solver=si, field0=ops["field0"], pump=ops["pump"], cav=ops["cav"],
+                             ocm=ops["ocm"], delay=b["delay"])],
+                       precision=prec, trip_cap=trips + 1, ctx=ctx)
+        t0 = time.perf_counter()
+        done = 0
+        while done < trips:
+            k = min(25, trips - done)
+            run.launch(k)
+            run.sync(allow_nonfinite=True)
+            done += k
+            print(f"esd: {prec} {done}/{trips} trips {time.perf_counter() - t0:.1f} s",
+                  file=sys.stderr, flush=True)
+        wall = time.perf_counter() - t0
+        r = run.read(0)
+        avg, n_avg = run.read_average(0)          # summed on the device as each trip ends
+        assert n_avg == trips
+        res[prec] = dict(seconds=wall, last=np.abs(r["oc_spec"]) ** 2, avg=avg,
+                         energy=r["trip_log"][:trips, 1].copy())
+        run.close()
+    ctx.close()
+    rl2 = lambda x, y: float(np.linalg.norm(x - y) / np.linalg.norm(y))
+    s32, s64 = res["single"], res["double"]
+    en32, en64 = s32["energy"], s64["energy"]
+    marks = [t for t in (1, 10, 25, 50, 100, 150, 200, 250, 300) if t <= trips]
+    out = {
+        "metric": METRIC,
+        "value": rl2(s32["avg"], s64["avg"]),
+        "unit": f"relative L2, FP32 vs FP64 run-averaged output-coupled ESD over {trips} trips",
+        "n_gpus": 1,
+        "steps": trips,
+        "warmup": 0,
+        "ms_per_step": 1e3 * s32["seconds"] / trips,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 vs f64",
+        "data": "synthetic (sech^2 pump + seeded one-photon-per-mode noise)",
+        "config": {
+            "workload": "synchronously pumped ring OPO, N=2^15, 12 ps window, 1040 nm; 5 mm "
+                        "MgO:PPLN 30.5 um; 2 W @ 1 GHz 200 fs sech^2; 5% OC; 0.3 ps delay; "
+                        f"{trips} trips from noise in FP32 and in FP64",
+            "last_trip_esd_rel_l2": rl2(s32["last"], s64["last"]),
+            "avg_power_rel_diff": float(abs(en32.mean() - en64.mean()) / en64.mean()),
+            "seconds_f32": s32["seconds"],
+            "seconds_f64": s64["seconds"],
+            "oc_energy_f64": {str(t): float(en64[t - 1]) for t in marks},
+            "oc_energy_rel_diff": {str(t): float(abs(en32[t - 1] - en64[t - 1]) /
+                                                 max(abs(en64[t - 1]), 1e-300))
+                                   for t in marks},
+        },
+    }
+    print(json.dumps(out))
+
+
 def main_surrogate(args):
     """Config 4 shape: 2050 -> 2048 x3 -> 2048 MLP, batch 512, Adam, on a
     synthetic device-resident set of 8192 pairs; steps = Adam steps."""
@@ -334,6 +404,72 @@ def cpu_baseline(b, attempts_per_trip, method="heun"):
     }
 
 
+def cpu_baseline_batch(insts, meta, attempts_per_trip, sample_steps=3000):
+    """Oracle (FP64) on the host cores for the uneven batch: the instances
+    are independent, so the CPU path runs one instance per core.  Sample: the
+    first `sample_steps` minimal steps of one instance per core (spread over
+    the batch and its grid sizes), all cores at once, giving a cost per
+    adaptive step for each grid size under full load; the batch's round trip
+    is then the sum over instances of (GPU-measured adaptive steps) x (that
+    cost), divided by the cores.  The single-core figure repeats the sample
+    with one instance of each grid size on an otherwise idle host."""
+    import concurrent.futures as cf
+    import oracle
+    oracle.lib()
+    try:
+        cores = min(16, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        cores = min(16, os.cpu_count() or 1)
+    sizes = sorted({n for n, _ in meta})
+    order = sorted(range(len(insts)), key=lambda i: (sizes.index(meta[i][0]), i))
+    pick = [order[(k * len(order)) // cores] for k in range(cores)]
+    for n in sizes:                                   # every grid size at least once
+        if not any(meta[i][0] == n for i in pick):
+            pick[-1 - sizes.index(n)] = next(i for i in order if meta[i][0] == n)
+
+    def one(i):
+        si = insts[i]["solver"]
+        ns = min(sample_steps, si["nSteps"])
+        t0 = time.perf_counter()
+        _, _, _, cnt = oracle.nee_heun(insts[i]["field0"], si["alpha_w"], si["G33"][:ns],
+                                       si["w0"], si["bdiffw0"], si["h"], ns, si["dt"],
+                                       si["beta_op"], si["maxErr"], si["minErr"], float(ns),
+                                       n_cols=0, want_stepmods=False)
+        return meta[i][0], time.perf_counter() - t0, int(cnt[0])
+
+    def per_size(rows):
+        out = {}
+        for n in sizes:
+            t = sum(r[1] for r in rows if r[0] == n)
+            c = sum(r[2] for r in rows if r[0] == n)
+            out[n] = t / max(c, 1)
+        return out
+
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(cores) as ex:
+        loaded = list(ex.map(one, pick))
+    t_par = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    alone = [one(next(i for i in pick if meta[i][0] == n)) for n in sizes]
+    t_one = time.perf_counter() - t0
+    c_par, c_one = per_size(loaded), per_size(alone)
+    trip_par = sum(a * c_par[n] for a, (n, _) in zip(attempts_per_trip, meta)) / cores
+    trip_one = sum(a * c_one[n] for a, (n, _) in zip(attempts_per_trip, meta))
+    return {
+        "value": len(insts) / trip_par,
+        "unit": "instance round trips/s",
+        "cores": cores,
+        "kind": "port",
+        "single_core_value": len(insts) / trip_one,
+        "sample": (f"first {sample_steps} minimal steps of {cores} instances at once "
+                   f"({sum(r[2] for r in loaded)} adaptive steps, {t_par:.1f} s) and of one "
+                   f"instance per grid size alone ({t_one:.1f} s); seconds per adaptive step "
+                   + ", ".join(f"2^{int(np.log2(n))}: {c_par[n] * 1e3:.2f} ms loaded / "
+                               f"{c_one[n] * 1e3:.2f} ms alone" for n in sizes)
+                   + "; scaled by each instance's GPU-measured adaptive steps per trip"),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -353,8 +489,10 @@ def main():
     ap.add_argument("--steady-budget", type=int, default=600,
                     help="trip budget of each run of the steady workload")
     ap.add_argument("--ws-phase", default="all", choices=["all", "plain", "warm"])
+    ap.add_argument("--esd-trips", type=int, default=300,
+                    help="round trips of each run of the esd workload")
     ap.add_argument("--workload", default="ring",
-                    choices=["ring", "batch", "warmstart", "surrogate", "steady"],
+                    choices=["ring", "batch", "warmstart", "surrogate", "steady", "esd"],
                     help="ring: one config-2 instance (the headline line); batch: 256 uneven "
                          "instances in one persistent launch")
     ap.add_argument("--tune", default="heuristic", choices=["heuristic", "auto"])
@@ -384,6 +522,8 @@ def main():
         return main_surrogate(args)
     if args.workload == "steady":
         return main_steady(args)
+    if args.workload == "esd":
+        return main_esd(args)
     from optifax_amd.solver import Context, OPOBatch
     cfg, b = workload()
     si, ops = b["solver"], b["ops"]
@@ -496,11 +636,13 @@ def main_batch(args):
     wall = time.perf_counter() - t0
     att = acc = 0
     bytes_total = 0.0
+    att_inst = []
     for i, (n, L) in enumerate(meta):
         log = run.read(i)["trip_log"][args.warmup:args.warmup + args.steps]
         a_i, c_i = int(log[:, 2].sum()), int(log[:, 3].sum())
         att += a_i
         acc += c_i
+        att_inst.append(a_i / args.steps)
         bytes_total += algorithmic_bytes(n, c_i, a_i - c_i, args.steps)
     inst_trips = len(insts) * args.steps
     achieved = bytes_total / wall / 1e9
@@ -538,6 +680,8 @@ def main_batch(args):
     }
     run.close()
     ctx.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_batch(insts, meta, att_inst)
     print(json.dumps(out))
 
 

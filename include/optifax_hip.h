@@ -153,6 +153,16 @@ int ofx_opo_last_ms(ofx_opo* run, double* ms);
 int ofx_opo_read(ofx_opo* run, int inst, double* field, double* oc_spec, double* trip_log,
                  int* status, int* trips_done, long long* counters, double* apft,
                  double* stepmods);
+/* Run-averaged output-coupled energy spectral density of one cavity
+ * instance: esd [n] (may be NULL) = mean over the trips since creation (or
+ * the last ofx_opo_reset_average) of |E_oc(w_j)|^2 in FFT order, summed on
+ * the device in FP64 as each trip ends; *trips = the trips averaged (esd is
+ * all zeros for 0).  Replaces the ESDOutAverage accumulation of
+ * OpticalSim.averageSimPulses (OpticalSim.m:302-307), without its per-trip
+ * gather to the host. */
+int ofx_opo_read_average(ofx_opo* run, int inst, double* esd, int* trips);
+/* clear that average (the next trip starts a new one) */
+int ofx_opo_reset_average(ofx_opo* run, int inst);
 /* overwrite the crystal-input field of one instance (warm start / checkpoint
  * restore); resets its status to running */
 int ofx_opo_write_field(ofx_opo* run, int inst, const double* field);

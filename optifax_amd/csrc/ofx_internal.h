@@ -52,6 +52,7 @@ struct InstDesc {
   const void* cav;      // N complex: crystal output -> next crystal input transfer
   const void* ocm;      // N complex: crystal output -> output-coupled spectrum
   void* oc_spec;        // N complex: last output-coupled spectrum (in/out)
+  double* esd_sum;      // N: sum over trips of |output-coupled spectrum|^2 (FFT order)
   double* trip_log;     // [trip_cap][4]: conv metric, oc energy, attempts, accepted
   int trip_cap;
   int pad1;

@@ -1344,6 +1344,7 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
       const C* cav = (const C*)d.cav;
       const C* ocm = (const C*)d.ocm;
       C* ocs = (C*)d.oc_spec;
+      double* esd = d.esd_sum;
       const bool have_prev = (trip_base + trip) > 0 && !(fresh && trip == 0);
       double acc[3] = {0.0, 0.0, 0.0};  // sum (S-Sp)^2, sum S^2, sum S
       for (int idx = OFX_TID; idx < g.Rr * n1; idx += OFX_NT) {
@@ -1360,6 +1361,7 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
         acc[1] += S * S;
         acc[2] += S;
         ocs[kk] = o;
+        esd[kk] += S;  // each kk belongs to one thread of one member
         C m = cmul(cav[kk], expi(-d.delay * dw_of(kk, n, dt), R(0)));
         F[k2l * g.sF + k1] = cadd(cmul(v, m), pump ? pump[kk] : mkc<R>(R(0), R(0)));
       }

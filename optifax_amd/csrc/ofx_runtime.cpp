@@ -38,6 +38,8 @@ struct InstBufs {
   void* cav = nullptr;
   void* ocm = nullptr;
   void* oc_spec = nullptr;
+  void* esd_sum = nullptr;
+  int esd_base = 0;  // trips_done when the run average was last cleared
   void* trip_log = nullptr;
 };
 
@@ -491,6 +493,7 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
       st = dev_alloc(r, &b.cav, n * r->csz);
       if (!st) st = dev_alloc(r, &b.ocm, n * r->csz);
       if (!st) st = dev_alloc(r, &b.oc_spec, n * r->csz);
+      if (!st) st = dev_alloc(r, &b.esd_sum, n * 8);
       if (!st && s.pump) st = dev_alloc(r, &b.pump, n * r->csz);
     }
     if (!st && trip_cap > 0) st = dev_alloc(r, &b.trip_log, (size_t)trip_cap * 4 * 8);
@@ -518,6 +521,8 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
       upload(r, b.ocm, convert(s.ocm, 2 * n, precision, keep), n * r->csz);
       keep.emplace_back(n * r->csz, 0);
       upload(r, b.oc_spec, keep.back().data(), n * r->csz);
+      keep.emplace_back(n * 8, 0);
+      upload(r, b.esd_sum, keep.back().data(), n * 8);
       if (b.pump) upload(r, b.pump, convert(s.pump, 2 * n, precision, keep), n * r->csz);
     }
     if (b.trip_log) {
@@ -554,6 +559,7 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
     d.cav = b.cav;
     d.ocm = b.ocm;
     d.oc_spec = b.oc_spec;
+    d.esd_sum = (double*)b.esd_sum;
     d.trip_log = (double*)b.trip_log;
     d.trip_cap = trip_cap;
     d.status = r->status_d + i;
@@ -749,6 +755,41 @@ int ofx_opo_read(ofx_opo* r, int inst, double* field, double* oc_spec, double* t
       st = fail(r->ctx, OFX_E_HIP, "readback failed");
     for (size_t j = 0; !st && j < sm.size(); j++) stepmods[j] = sm[j];
   }
+  return st;
+}
+
+int ofx_opo_read_average(ofx_opo* r, int inst, double* esd, int* trips) {
+  if (!r || inst < 0 || inst >= r->n_inst) return OFX_E_ARG;
+  const InstBufs& b = r->bufs[inst];
+  if (!b.esd_sum) return fail(r->ctx, OFX_E_ARG, "instance %d has no cavity", inst);
+  int done = 0;
+  if (be_sync(r->ctx->be) || be_d2h(r->ctx->be, &done, r->trips_d + inst, sizeof(int)) ||
+      be_sync(r->ctx->be))
+    return fail(r->ctx, OFX_E_HIP, "readback failed: %s", be_error(r->ctx->be));
+  const int k = done - b.esd_base;
+  if (trips) *trips = k;
+  if (esd) {
+    const size_t n = (size_t)r->pass[inst].n;
+    if (be_d2h(r->ctx->be, esd, b.esd_sum, n * 8) || be_sync(r->ctx->be))
+      return fail(r->ctx, OFX_E_HIP, "readback failed: %s", be_error(r->ctx->be));
+    for (size_t j = 0; j < n; j++) esd[j] = k > 0 ? esd[j] / k : 0.0;
+  }
+  return OFX_OK;
+}
+
+int ofx_opo_reset_average(ofx_opo* r, int inst) {
+  if (!r || inst < 0 || inst >= r->n_inst) return OFX_E_ARG;
+  InstBufs& b = r->bufs[inst];
+  if (!b.esd_sum) return fail(r->ctx, OFX_E_ARG, "instance %d has no cavity", inst);
+  int done = 0;
+  if (be_sync(r->ctx->be) || be_d2h(r->ctx->be, &done, r->trips_d + inst, sizeof(int)) ||
+      be_sync(r->ctx->be))
+    return fail(r->ctx, OFX_E_HIP, "readback failed: %s", be_error(r->ctx->be));
+  const size_t n = (size_t)r->pass[inst].n;
+  std::vector<double> z(n, 0.0);
+  int st = upload(r, b.esd_sum, z.data(), n * 8);
+  if (!st && be_sync(r->ctx->be)) st = fail(r->ctx, OFX_E_HIP, "upload failed");
+  if (!st) b.esd_base = done;
   return st;
 }
 

@@ -283,6 +283,20 @@ class OPOBatch:
                     trips_done=trips.value, attempts=int(cnt[0]), accepted=int(cnt[1]),
                     apft=None if ap is None else ap.T.copy(), stepmods=sm)
 
+    def read_average(self, inst):
+        """(esd, trips): mean of |E_oc(w)|^2 (FFT order) over the trips since
+        creation or the last reset_average (OpticalSim.m:302-307)."""
+        n = self.meta[inst]["n"]
+        esd = np.zeros(n)
+        trips = ctypes.c_int()
+        self.ctx.check(self.lib.ofx_opo_read_average(
+            self.h, inst, esd.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+            ctypes.byref(trips)))
+        return esd, trips.value
+
+    def reset_average(self, inst):
+        self.ctx.check(self.lib.ofx_opo_reset_average(self.h, inst))
+
     def write_field(self, inst, field):
         a = np.ascontiguousarray(np.asarray(field, dtype=np.complex128).reshape(-1))
         self.ctx.check(self.lib.ofx_opo_write_field(

@@ -11,7 +11,7 @@ import oracle
 from optifax_amd import OfxError
 from optifax_amd.solver import OPOBatch, OPOmexBatch
 
-from cases import crystal_case, opo_case, oracle_next_input, rel_l2
+from cases import crystal_case, opo_case, oracle_next_input, oracle_oc_esds, rel_l2
 
 
 def _run(ctx, E, args, n, precision, C, apft_cols=4, exchange="auto"):
@@ -236,4 +236,30 @@ def test_wait_limit_gives_up_and_the_next_launch_recovers(emu_ctx, exchange):
     assert np.array_equal(got["oc_spec"], want["oc_spec"])
     with pytest.raises(OfxError):
         run.set_wait_limit(-1.0)
+    run.close()
+
+
+@pytest.mark.parametrize("exchange", EXCHANGES)
+def test_run_averaged_output_esd(emu_ctx, exchange):
+    """The device's run-averaged output ESD (ESDOutAverage of
+    OpticalSim.averageSimPulses, OpticalSim.m:302-307) equals the mean of the
+    oracle's per-trip output-coupled |E(w)|^2, and restarts after a reset."""
+    cfg, b = opo_case()
+    si, ops = b["solver"], b["ops"]
+    esd = oracle_oc_esds(b, cfg.delay, 5)
+    run = OPOBatch([dict(solver=si, field0=ops["field0"], pump=ops["pump"], cav=ops["cav"],
+                         ocm=ops["ocm"], delay=cfg.delay)], precision="double", trip_cap=6,
+                   ctx=emu_ctx, tune={"cluster": 2, "exchange": exchange})
+    e, k = run.read_average(0)
+    assert k == 0 and not e.any()
+    run.launch(3)
+    run.sync()
+    e, k = run.read_average(0)
+    assert k == 3 and rel_l2(e, np.mean(esd[:3], axis=0)) < 1e-10
+    run.reset_average(0)
+    run.launch(1)
+    run.launch(1)
+    run.sync()
+    e, k = run.read_average(0)
+    assert k == 2 and rel_l2(e, np.mean(esd[3:5], axis=0)) < 1e-10
     run.close()

@@ -9,7 +9,7 @@ import oracle
 from optifax_amd.physics import SimWindow
 from optifax_amd.solver import OPOBatch, OPOmexBatch
 
-from cases import crystal_case, opo_case, oracle_next_input, rel_l2
+from cases import crystal_case, opo_case, oracle_next_input, oracle_oc_esds, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -286,3 +286,29 @@ def test_config5_saturated_trip_fp64_matches_oracle(gpu_ctx):
     f32.close()
     print(f"config-5 saturated trip: FP32 rel L2 vs oracle {e32:.2e}")
     assert e32 < 5e-3
+
+
+@pytest.mark.parametrize("exchange", EXCHANGES)
+def test_run_averaged_output_esd(gpu_ctx, exchange):
+    """The device's run-averaged output ESD (ESDOutAverage of
+    OpticalSim.averageSimPulses, OpticalSim.m:302-307) equals the mean of the
+    oracle's per-trip output-coupled |E(w)|^2, and restarts after a reset."""
+    cfg, b = opo_case()
+    si, ops = b["solver"], b["ops"]
+    esd = oracle_oc_esds(b, cfg.delay, 5)
+    run = OPOBatch([dict(solver=si, field0=ops["field0"], pump=ops["pump"], cav=ops["cav"],
+                         ocm=ops["ocm"], delay=cfg.delay)], precision="double", trip_cap=6,
+                   ctx=gpu_ctx, tune={"cluster": 2, "exchange": exchange})
+    e, k = run.read_average(0)
+    assert k == 0 and not e.any()
+    run.launch(3)
+    run.sync()
+    e, k = run.read_average(0)
+    assert k == 3 and rel_l2(e, np.mean(esd[:3], axis=0)) < 1e-9
+    run.reset_average(0)
+    run.launch(1)
+    run.launch(1)
+    run.sync()
+    e, k = run.read_average(0)
+    assert k == 2 and rel_l2(e, np.mean(esd[3:5], axis=0)) < 1e-9
+    run.close()
