@@ -295,7 +295,7 @@ def main_esd(args):
                   file=sys.stderr, flush=True)
         wall = time.perf_counter() - t0
         r = run.read(0)
-        avg, n_avg = run.read_average(0)          # summed on the device as each trip ends
+        avg, _, n_avg = run.read_average(0)       # summed on the device as each trip ends
         assert n_avg == trips
         res[prec] = dict(seconds=wall, last=np.abs(r["oc_spec"]) ** 2, avg=avg,
                          energy=r["trip_log"][:trips, 1].copy())

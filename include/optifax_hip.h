@@ -153,16 +153,26 @@ int ofx_opo_last_ms(ofx_opo* run, double* ms);
 int ofx_opo_read(ofx_opo* run, int inst, double* field, double* oc_spec, double* trip_log,
                  int* status, int* trips_done, long long* counters, double* apft,
                  double* stepmods);
-/* Run-averaged output-coupled energy spectral density of one cavity
- * instance: esd [n] (may be NULL) = mean over the trips since creation (or
- * the last ofx_opo_reset_average) of |E_oc(w_j)|^2 in FFT order, summed on
- * the device in FP64 as each trip ends; *trips = the trips averaged (esd is
- * all zeros for 0).  Replaces the ESDOutAverage accumulation of
- * OpticalSim.averageSimPulses (OpticalSim.m:302-307), without its per-trip
- * gather to the host. */
-int ofx_opo_read_average(ofx_opo* run, int inst, double* esd, int* trips);
-/* clear that average (the next trip starts a new one) */
+/* Run averages of one cavity instance, summed on the device in FP64 as each
+ * trip ends (either pointer may be NULL): oc_esd [n] = mean of the
+ * output-coupled |E_oc(w_j)|^2 and xout_esd [n] = mean of the crystal-output
+ * |E_x(w_j)|^2, FFT order, over the trips since creation or the last
+ * ofx_opo_reset_average; *trips = the trips averaged (all zeros for 0).
+ * Replaces the ESDOutAverage, ESDPumpDepAverage and ESDIdlerICAverage sums of
+ * OpticalSim.averageSimPulses (OpticalSim.m:302-313; the last two are the
+ * crystal-output ESD behind a wavelength mask, applied by the caller), without
+ * their per-trip gather to the host. */
+int ofx_opo_read_average(ofx_opo* run, int inst, double* oc_esd, double* xout_esd, int* trips);
+/* clear both averages (the next trip starts new ones) */
 int ofx_opo_reset_average(ofx_opo* run, int inst);
+/* Keep the crystal-output spectrum (FFT order, inside the crystal) of the
+ * last `trips` round trips of every cavity instance in device memory, from
+ * the next trip on (trips x n complex per instance; once per run).  Replaces
+ * StoredPulses (OpticalSim.m:173-180,268-272), which gathers every trip's
+ * crystal output to the host. */
+int ofx_opo_keep_history(ofx_opo* run, int trips);
+/* spectrum [2n] of cumulative 0-based trip `trip` of one instance, if held */
+int ofx_opo_read_history(ofx_opo* run, int inst, int trip, double* spectrum);
 /* overwrite the crystal-input field of one instance (warm start / checkpoint
  * restore); resets its status to running */
 int ofx_opo_write_field(ofx_opo* run, int inst, const double* field);

@@ -48,7 +48,8 @@ EXPORTED = [
     "ofx_opo_create", "ofx_opo_destroy", "ofx_opo_launch", "ofx_opo_sync",
     "ofx_opo_graph_replay", "ofx_opo_last_ms", "ofx_opo_read", "ofx_opo_write_field",
     "ofx_opo_geometry", "ofx_opo_xcc_ids", "ofx_opo_set_wait_limit",
-    "ofx_opo_read_average", "ofx_opo_reset_average",
+    "ofx_opo_read_average", "ofx_opo_reset_average", "ofx_opo_keep_history",
+    "ofx_opo_read_history",
     "ofx_mlp_create", "ofx_mlp_destroy", "ofx_mlp_last_error", "ofx_mlp_num_params",
     "ofx_mlp_set_params", "ofx_mlp_get_params", "ofx_mlp_forward", "ofx_mlp_gradients",
     "ofx_mlp_step", "ofx_mlp_upload_dataset", "ofx_mlp_train", "ofx_mlp_last_ms",
@@ -122,8 +123,10 @@ def load(path: str | None = None):
     L.ofx_opo_geometry.argtypes = [vp, ctypes.c_int, _IP, _IP, _IP, _IP, _IP]
     L.ofx_opo_xcc_ids.argtypes = [vp, _IP, ctypes.c_int, _IP]
     L.ofx_opo_set_wait_limit.argtypes = [vp, ctypes.c_double]
-    L.ofx_opo_read_average.argtypes = [vp, ctypes.c_int, _DP, _IP]
+    L.ofx_opo_read_average.argtypes = [vp, ctypes.c_int, _DP, _DP, _IP]
     L.ofx_opo_reset_average.argtypes = [vp, ctypes.c_int]
+    L.ofx_opo_keep_history.argtypes = [vp, ctypes.c_int]
+    L.ofx_opo_read_history.argtypes = [vp, ctypes.c_int, ctypes.c_int, _DP]
     if hasattr(L, "ofx_mlp_create"):   # the CPU emulation library has no surrogate
         L.ofx_mlp_create.argtypes = [ctypes.c_int] * 5 + [ctypes.POINTER(vp)]
         L.ofx_mlp_destroy.argtypes = [vp]

@@ -1345,12 +1345,16 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
       const C* ocm = (const C*)d.ocm;
       C* ocs = (C*)d.oc_spec;
       double* esd = d.esd_sum;
+      double* xesd = d.xesd_sum;
+      C* hist = d.hist ? (C*)d.hist + (size_t)((trip_base + trip) % d.hist_cap) * (size_t)n : nullptr;
       const bool have_prev = (trip_base + trip) > 0 && !(fresh && trip == 0);
       double acc[3] = {0.0, 0.0, 0.0};  // sum (S-Sp)^2, sum S^2, sum S
       for (int idx = OFX_TID; idx < g.Rr * n1; idx += OFX_NT) {
         const int k2l = idx >> g.lg1, k1 = idx & (n1 - 1);
         const int kk = (g.k20 + k2l) + n2 * k1;
         C v = F[k2l * g.sF + k1];
+        xesd[kk] += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+        if (hist) hist[kk] = v;
         C o = cmul(v, ocm[kk]);
         const double S = (double)o.x * (double)o.x + (double)o.y * (double)o.y;
         if (have_prev) {

@@ -39,7 +39,11 @@ struct InstBufs {
   void* ocm = nullptr;
   void* oc_spec = nullptr;
   void* esd_sum = nullptr;
-  int esd_base = 0;  // trips_done when the run average was last cleared
+  void* xesd_sum = nullptr;
+  int esd_base = 0;  // trips_done when the run averages were last cleared
+  void* hist = nullptr;
+  int hist_cap = 0;
+  int hist_base = 0;  // trips_done when the history started
   void* trip_log = nullptr;
 };
 
@@ -494,6 +498,7 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
       if (!st) st = dev_alloc(r, &b.ocm, n * r->csz);
       if (!st) st = dev_alloc(r, &b.oc_spec, n * r->csz);
       if (!st) st = dev_alloc(r, &b.esd_sum, n * 8);
+      if (!st) st = dev_alloc(r, &b.xesd_sum, n * 8);
       if (!st && s.pump) st = dev_alloc(r, &b.pump, n * r->csz);
     }
     if (!st && trip_cap > 0) st = dev_alloc(r, &b.trip_log, (size_t)trip_cap * 4 * 8);
@@ -523,6 +528,7 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
       upload(r, b.oc_spec, keep.back().data(), n * r->csz);
       keep.emplace_back(n * 8, 0);
       upload(r, b.esd_sum, keep.back().data(), n * 8);
+      upload(r, b.xesd_sum, keep.back().data(), n * 8);
       if (b.pump) upload(r, b.pump, convert(s.pump, 2 * n, precision, keep), n * r->csz);
     }
     if (b.trip_log) {
@@ -560,6 +566,7 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
     d.ocm = b.ocm;
     d.oc_spec = b.oc_spec;
     d.esd_sum = (double*)b.esd_sum;
+    d.xesd_sum = (double*)b.xesd_sum;
     d.trip_log = (double*)b.trip_log;
     d.trip_cap = trip_cap;
     d.status = r->status_d + i;
@@ -758,21 +765,30 @@ int ofx_opo_read(ofx_opo* r, int inst, double* field, double* oc_spec, double* t
   return st;
 }
 
-int ofx_opo_read_average(ofx_opo* r, int inst, double* esd, int* trips) {
+static int trips_done_of(ofx_opo* r, int inst, int* done) {
+  if (be_sync(r->ctx->be) || be_d2h(r->ctx->be, done, r->trips_d + inst, sizeof(int)) ||
+      be_sync(r->ctx->be))
+    return fail(r->ctx, OFX_E_HIP, "readback failed: %s", be_error(r->ctx->be));
+  return OFX_OK;
+}
+
+int ofx_opo_read_average(ofx_opo* r, int inst, double* oc_esd, double* xout_esd, int* trips) {
   if (!r || inst < 0 || inst >= r->n_inst) return OFX_E_ARG;
   const InstBufs& b = r->bufs[inst];
   if (!b.esd_sum) return fail(r->ctx, OFX_E_ARG, "instance %d has no cavity", inst);
   int done = 0;
-  if (be_sync(r->ctx->be) || be_d2h(r->ctx->be, &done, r->trips_d + inst, sizeof(int)) ||
-      be_sync(r->ctx->be))
-    return fail(r->ctx, OFX_E_HIP, "readback failed: %s", be_error(r->ctx->be));
+  int st = trips_done_of(r, inst, &done);
+  if (st) return st;
   const int k = done - b.esd_base;
   if (trips) *trips = k;
-  if (esd) {
-    const size_t n = (size_t)r->pass[inst].n;
-    if (be_d2h(r->ctx->be, esd, b.esd_sum, n * 8) || be_sync(r->ctx->be))
+  const size_t n = (size_t)r->pass[inst].n;
+  void* src[2] = {b.esd_sum, b.xesd_sum};
+  double* dst[2] = {oc_esd, xout_esd};
+  for (int q = 0; q < 2; q++) {
+    if (!dst[q]) continue;
+    if (be_d2h(r->ctx->be, dst[q], src[q], n * 8) || be_sync(r->ctx->be))
       return fail(r->ctx, OFX_E_HIP, "readback failed: %s", be_error(r->ctx->be));
-    for (size_t j = 0; j < n; j++) esd[j] = k > 0 ? esd[j] / k : 0.0;
+    for (size_t j = 0; j < n; j++) dst[q][j] = k > 0 ? dst[q][j] / k : 0.0;
   }
   return OFX_OK;
 }
@@ -782,15 +798,51 @@ int ofx_opo_reset_average(ofx_opo* r, int inst) {
   InstBufs& b = r->bufs[inst];
   if (!b.esd_sum) return fail(r->ctx, OFX_E_ARG, "instance %d has no cavity", inst);
   int done = 0;
-  if (be_sync(r->ctx->be) || be_d2h(r->ctx->be, &done, r->trips_d + inst, sizeof(int)) ||
-      be_sync(r->ctx->be))
-    return fail(r->ctx, OFX_E_HIP, "readback failed: %s", be_error(r->ctx->be));
+  int st = trips_done_of(r, inst, &done);
+  if (st) return st;
   const size_t n = (size_t)r->pass[inst].n;
   std::vector<double> z(n, 0.0);
-  int st = upload(r, b.esd_sum, z.data(), n * 8);
+  st = upload(r, b.esd_sum, z.data(), n * 8);
+  if (!st) st = upload(r, b.xesd_sum, z.data(), n * 8);
   if (!st && be_sync(r->ctx->be)) st = fail(r->ctx, OFX_E_HIP, "upload failed");
   if (!st) b.esd_base = done;
   return st;
+}
+
+int ofx_opo_keep_history(ofx_opo* r, int trips) {
+  if (!r || trips < 1) return OFX_E_ARG;
+  int st = OFX_OK;
+  for (int i = 0; i < r->n_inst && !st; i++) {
+    InstBufs& b = r->bufs[i];
+    if (!b.esd_sum) continue;
+    if (b.hist) return fail(r->ctx, OFX_E_ARG, "instance %d already keeps a history", i);
+    const size_t n = (size_t)r->pass[i].n;
+    st = trips_done_of(r, i, &b.hist_base);
+    if (!st) st = dev_alloc(r, &b.hist, (size_t)trips * n * r->csz);
+    if (!st) {
+      b.hist_cap = trips;
+      r->inst_h[i].hist = b.hist;
+      r->inst_h[i].hist_cap = trips;
+    }
+  }
+  if (!st) st = upload(r, r->inst_d, r->inst_h.data(), sizeof(InstDesc) * r->n_inst);
+  if (!st && be_sync(r->ctx->be)) st = fail(r->ctx, OFX_E_HIP, "upload failed");
+  return st;
+}
+
+int ofx_opo_read_history(ofx_opo* r, int inst, int trip, double* spectrum) {
+  if (!r || !spectrum || inst < 0 || inst >= r->n_inst) return OFX_E_ARG;
+  const InstBufs& b = r->bufs[inst];
+  if (!b.hist) return fail(r->ctx, OFX_E_ARG, "instance %d keeps no history", inst);
+  int done = 0;
+  int st = trips_done_of(r, inst, &done);
+  if (st) return st;
+  const int lo = std::max(b.hist_base, done - b.hist_cap);
+  if (trip < lo || trip >= done)
+    return fail(r->ctx, OFX_E_ARG, "trip %d of instance %d is not held (held: %d..%d)", trip, inst,
+                lo, done - 1);
+  const size_t n = (size_t)r->pass[inst].n;
+  return read_conv(r, (char*)b.hist + (size_t)(trip % b.hist_cap) * n * r->csz, spectrum, 2 * n);
 }
 
 int ofx_opo_write_field(ofx_opo* r, int inst, const double* field) {

@@ -284,18 +284,33 @@ class OPOBatch:
                     apft=None if ap is None else ap.T.copy(), stepmods=sm)
 
     def read_average(self, inst):
-        """(esd, trips): mean of |E_oc(w)|^2 (FFT order) over the trips since
-        creation or the last reset_average (OpticalSim.m:302-307)."""
+        """(oc_esd, xout_esd, trips): means of the output-coupled and the
+        crystal-output |E(w)|^2 (FFT order) over the trips since creation or
+        the last reset_average (OpticalSim.m:302-313)."""
         n = self.meta[inst]["n"]
-        esd = np.zeros(n)
+        oc, xo = np.zeros(n), np.zeros(n)
         trips = ctypes.c_int()
-        self.ctx.check(self.lib.ofx_opo_read_average(
-            self.h, inst, esd.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-            ctypes.byref(trips)))
-        return esd, trips.value
+        dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        self.ctx.check(self.lib.ofx_opo_read_average(self.h, inst, dp(oc), dp(xo),
+                                                     ctypes.byref(trips)))
+        return oc, xo, trips.value
 
     def reset_average(self, inst):
         self.ctx.check(self.lib.ofx_opo_reset_average(self.h, inst))
+
+    def keep_history(self, trips):
+        """Hold the crystal-output spectra of the last `trips` trips of every
+        cavity instance on the device (StoredPulses, OpticalSim.m:268-272)."""
+        self.ctx.check(self.lib.ofx_opo_keep_history(self.h, int(trips)))
+
+    def read_history(self, inst, trip):
+        """Crystal-output spectrum (FFT order) of cumulative 0-based trip `trip`."""
+        n = self.meta[inst]["n"]
+        out = np.zeros(n, np.complex128)
+        self.ctx.check(self.lib.ofx_opo_read_history(
+            self.h, inst, int(trip), out.view(np.float64).ctypes.data_as(
+                ctypes.POINTER(ctypes.c_double))))
+        return out
 
     def write_field(self, inst, field):
         a = np.ascontiguousarray(np.asarray(field, dtype=np.complex128).reshape(-1))

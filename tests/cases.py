@@ -36,9 +36,9 @@ def oracle_next_input(b, E_cav):
     return np.fft.fftshift(X)
 
 
-def oracle_oc_esds(b, delay, trips):
-    """|E_oc(w)|^2 in FFT order after each of trips 1..trips, from separate
-    oracle runs of the literal OpticalSim.run order."""
+def oracle_oc_spectra(b, delay, trips):
+    """Output-coupled spectrum (FFT order) after each of trips 1..trips, from
+    separate oracle runs of the literal OpticalSim.run order."""
     import oracle
     from optifax_amd.physics import SimWindow
     si, sw = b["solver"], b["simwin"]
@@ -47,5 +47,5 @@ def oracle_oc_esds(b, delay, trips):
     for t in range(1, trips + 1):
         o = oracle.opo_run(t, b["noise"], b["pump"], np.ones(sw.n), si["n_xtal"], R, T, phi,
                            delay, sw.rel_omegas, si)
-        out.append(np.abs(SimWindow.fft_order(SimWindow.t2k(o["E_oc"]))) ** 2)
+        out.append(SimWindow.fft_order(SimWindow.t2k(o["E_oc"])))
     return out

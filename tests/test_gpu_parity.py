@@ -6,10 +6,11 @@ import numpy as np
 import pytest
 
 import oracle
+from optifax_amd import OfxError
 from optifax_amd.physics import SimWindow
 from optifax_amd.solver import OPOBatch, OPOmexBatch
 
-from cases import crystal_case, opo_case, oracle_next_input, oracle_oc_esds, rel_l2
+from cases import crystal_case, opo_case, oracle_next_input, oracle_oc_spectra, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -217,7 +218,6 @@ def test_wait_limit_gives_up_and_the_next_launch_recovers(gpu_ctx, exchange):
     own bounded poll at the first wait longer than a few rounds (the kernel
     exits normally with a time-out status); after the limit is restored the
     same object reproduces a fresh run bit for bit."""
-    from optifax_amd import OfxError
     cfg, b = opo_case()
     si, ops = b["solver"], b["ops"]
     inst = dict(solver=si, field0=ops["field0"], pump=ops["pump"], cav=ops["cav"],
@@ -289,26 +289,42 @@ def test_config5_saturated_trip_fp64_matches_oracle(gpu_ctx):
 
 
 @pytest.mark.parametrize("exchange", EXCHANGES)
-def test_run_averaged_output_esd(gpu_ctx, exchange):
-    """The device's run-averaged output ESD (ESDOutAverage of
-    OpticalSim.averageSimPulses, OpticalSim.m:302-307) equals the mean of the
-    oracle's per-trip output-coupled |E(w)|^2, and restarts after a reset."""
+def test_run_averages_and_trip_history(gpu_ctx, exchange):
+    """Device-side run averages (ESDOutAverage and the crystal-output ESD
+    behind ESDPumpDepAverage / ESDIdlerICAverage, OpticalSim.m:302-313) and
+    the trip history (StoredPulses, OpticalSim.m:268-272) against the
+    oracle's output-coupled spectrum of each trip: the held crystal output
+    times the output-coupling transfer must give that spectrum."""
     cfg, b = opo_case()
     si, ops = b["solver"], b["ops"]
-    esd = oracle_oc_esds(b, cfg.delay, 5)
+    oc = oracle_oc_spectra(b, cfg.delay, 6)
     run = OPOBatch([dict(solver=si, field0=ops["field0"], pump=ops["pump"], cav=ops["cav"],
-                         ocm=ops["ocm"], delay=cfg.delay)], precision="double", trip_cap=6,
+                         ocm=ops["ocm"], delay=cfg.delay)], precision="double", trip_cap=8,
                    ctx=gpu_ctx, tune={"cluster": 2, "exchange": exchange})
-    e, k = run.read_average(0)
-    assert k == 0 and not e.any()
-    run.launch(3)
+    e, x, k = run.read_average(0)
+    assert k == 0 and not e.any() and not x.any()
+    run.launch(1)
+    run.keep_history(3)                        # held from the second trip (index 1) on
+    run.launch(2)
     run.sync()
-    e, k = run.read_average(0)
-    assert k == 3 and rel_l2(e, np.mean(esd[:3], axis=0)) < 1e-9
+    e, x, k = run.read_average(0)
+    assert k == 3 and rel_l2(e, np.mean([abs(s) ** 2 for s in oc[:3]], axis=0)) < 1e-9
+    hs = [run.read_history(0, t) for t in (1, 2)]
+    for t, h in zip((1, 2), hs):
+        assert rel_l2(h * ops["ocm"], oc[t]) < 1e-9
+    with pytest.raises(OfxError):
+        run.read_history(0, 0)                 # before the history started
+    with pytest.raises(OfxError):
+        run.keep_history(2)                    # once per run
     run.reset_average(0)
-    run.launch(1)
-    run.launch(1)
+    run.launch(3)                              # trips 3..5; the ring of 3 now holds 3..5
     run.sync()
-    e, k = run.read_average(0)
-    assert k == 2 and rel_l2(e, np.mean(esd[3:5], axis=0)) < 1e-9
+    e, x, k = run.read_average(0)
+    assert k == 3 and rel_l2(e, np.mean([abs(s) ** 2 for s in oc[3:6]], axis=0)) < 1e-9
+    hs = [run.read_history(0, t) for t in (3, 4, 5)]
+    for t, h in zip((3, 4, 5), hs):
+        assert rel_l2(h * ops["ocm"], oc[t]) < 1e-9
+    assert rel_l2(x, np.mean([abs(h) ** 2 for h in hs], axis=0)) < 1e-12
+    with pytest.raises(OfxError):
+        run.read_history(0, 2)                 # overwritten by trip 5
     run.close()
