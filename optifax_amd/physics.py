@@ -92,8 +92,55 @@ def gayer_n(lam_um, T=20.0, col=1):
     return np.sqrt(x.astype(np.complex128)).real
 
 
-def phi_calc(L, T, simwin: SimWindow):
-    """phi_calc.m:5-45 for PPLN: (phi_rel, phi_abs).
+def _re_sqrt(x):
+    """Real part of MATLAB's sqrt (complex for a negative argument)."""
+    return np.sqrt(np.asarray(x, dtype=np.float64).astype(np.complex128)).real
+
+
+def opgap_wei_n(lam_um, T=20.0):
+    """OP-GaP index, Wei et al., Opt. Mater. Express 8, 485 (2018), with the
+    temperature in kelvin inside (n_opgap_wei.m:1-15)."""
+    Tk = T + 273.15
+    l2 = np.asarray(lam_um, dtype=np.float64) ** 2
+    A = 10.926 + 7.0787e-4 * Tk + 1.8594e-7 * Tk ** 2
+    B = 0.53718 + 5.8035e-5 * Tk + 1.9819e-7 * Tk ** 2
+    D = 1504.0 + 0.25935 * Tk - 0.00023326 * Tk ** 2
+    return _re_sqrt(A + B / (l2 - 0.0911014) + D / (l2 - 758.048))
+
+
+def ktp_z_n(lam_um, T=20.0):
+    """KTP n_z: the Kato & Takaoka (2002) Sellmeier form at 20 C plus the
+    Emanueli & Arie (2003) thermal terms in (T - 20) (n_ktp_zhao.m:1-38)."""
+    lam = np.asarray(lam_um, dtype=np.float64)
+    l2 = lam * lam
+    n0 = _re_sqrt(2.12725 + 1.18431 / (1 - 5.14852e-2 / l2) + 0.6603 / (1 - 100.00507 / l2)
+                  - 9.68956e-3 * l2)
+    dT = T - 20.0
+    n1 = -6.1537e-6 + 64.505e-6 / lam - 56.447e-6 / l2 + 17.169e-6 / (l2 * lam)
+    n2 = -0.96751e-8 + 13.192e-8 / lam - 11.78e-8 / l2 + 3.6292e-8 / (l2 * lam)
+    return n0 + n1 * dT + n2 * dT ** 2
+
+
+def sellmeier_n(lam_um, material="PPLN", T=20.0):
+    """Crystal materials of sellmeier.m:3-21 (real part).  The glass table
+    (Schott / CDGM forms of Materials.csv) serves cavity optics, not crystals."""
+    if material in ("PPLN", "LN_e"):
+        return gayer_n(lam_um, T, 1)
+    if material == "LN_o":
+        return gayer_n(lam_um, T, 2)
+    if material == "PPLN_undoped":
+        return gayer_n(lam_um, T, 4)
+    if material in ("OP-GaP", "OPGaP"):
+        return opgap_wei_n(lam_um, T)
+    if material in ("KTP", "PPKTP"):
+        return ktp_z_n(lam_um, T)
+    raise ValueError(f"unknown crystal material {material!r}")
+
+
+def phi_calc(L, T, simwin: SimWindow, material="PPLN"):
+    """phi_calc.m:5-45: (phi_rel, phi_abs).  The string "PPLN" takes the
+    Gayer index at T; every other material goes through sellmeier() at its
+    default 20 C (phi_calc.m:17-20 passes no temperature there).
 
     phi_rel is defined on the 0.2-20 um knot band (NaN elsewhere): at the
     knots the reference's spline (phi_calc.m:40) returns the cumtrapz values
@@ -101,7 +148,8 @@ def phi_calc(L, T, simwin: SimWindow):
     transmission is zero, so the extrapolated values never reach the field."""
     w_abs, w_rel = simwin.omegas, simwin.rel_omegas
     lam = 2 * np.pi * C0 / w_abs
-    n_mat = gayer_n(lam * 1e6, T, 1)
+    n_mat = gayer_n(lam * 1e6, T, 1) if material == "PPLN" else \
+        sellmeier_n(lam * 1e6, material, 20.0)
     beta_abs = w_abs * n_mat / C0
     phi_abs = L * beta_abs
     gd_abs = np.concatenate([[0.0], np.diff(phi_abs) / np.diff(w_abs)])
@@ -127,20 +175,16 @@ def phi2gd(phi, dw):
     return np.concatenate([[gd[0]], gd])
 
 
-def qpm_poling(nsteps, L, period, duty_off=0.0):
-    """QPMcrystal.m:1-57 with zero domain-width uncertainty: +-1 per step."""
-    n_dom = int(math.floor(2.0 * L / period))
-    poled = period * (1.0 - 2.0 * duty_off) / 2.0
-    widths = np.full(n_dom, poled)
-    for i in range(1, n_dom - 1, 2):          # domain_widths(2:2:nDs-1), QPMcrystal.m:30
-        widths[i] = period - 0.5 * (widths[i - 1] + widths[i + 1])
-    walls = np.concatenate([[0.0], np.cumsum(widths)])
+def poling_from_widths(nsteps, L, widths):
+    """QPMcrystal.m:32-51 for given domain widths: walls at their cumulative
+    sums, the last domain extended or cut to L, then +-1 per step of
+    z = linspace(0, L, nsteps) by the parity of its domain."""
+    walls = np.concatenate([[0.0], np.cumsum(np.asarray(widths, dtype=np.float64))])
     if walls[-1] < L:
         walls = np.concatenate([walls, [L]])
     else:
         walls[-1] = L
-    n1 = nsteps - 1
-    z = (np.arange(nsteps, dtype=np.float64) * L) / n1   # linspace(0, L, nsteps)
+    z = (np.arange(nsteps, dtype=np.float64) * L) / (nsteps - 1)   # linspace(0, L, nsteps)
     z[-1] = L
     bins = np.searchsorted(walls, z, side="right")        # discretize: edges(k) <= z < edges(k+1)
     bins = np.minimum(bins, walls.size - 1)               # last bin includes the right edge
@@ -148,34 +192,107 @@ def qpm_poling(nsteps, L, period, duty_off=0.0):
     return -(2.0 * q - 1.0)
 
 
+def qpm_poling(nsteps, L, period, duty_off=0.0, uncertainty=0.0, seed=0):
+    """QPMcrystal.m:1-57 for one period: +-1 per step.  With a domain-width
+    uncertainty the odd domains are drawn from a normal of sigma
+    uncertainty/2 around the poled width (QPMcrystal.m:27-29) from a seeded
+    PCG64 stream; MATLAB's rng('default') + normrnd stream is not reproduced."""
+    n_dom = int(math.floor(2.0 * L / period))
+    poled = period * (1.0 - 2.0 * duty_off) / 2.0
+    if uncertainty > 0:
+        widths = np.random.Generator(np.random.PCG64(seed)).normal(poled, uncertainty / 2.0,
+                                                                     n_dom)
+    else:
+        widths = np.full(n_dom, poled)
+    for i in range(1, n_dom - 1, 2):          # domain_widths(2:2:nDs-1), QPMcrystal.m:30
+        widths[i] = period - 0.5 * (widths[i - 1] + widths[i + 1])
+    return poling_from_widths(nsteps, L, widths)
+
+
+def _mround(x):
+    """MATLAB round: half away from zero."""
+    return np.sign(x) * np.floor(np.abs(x) + 0.5)
+
+
+def chirped_grating(z, P1=6.3e-6, P2=2.2e-6, a=0.48, tol=0.1e-6):
+    """chirpedgrating.m:1-50: domain widths of a grating whose period runs
+    from P1 (+ tol) to P2 as (z / max z)^a, quantised to tol/2 steps; each
+    domain's width is read at the position reached by stepping whole
+    domains, until the next domain would pass max(z)."""
+    z = np.asarray(z, dtype=np.float64)
+    zmax = z.max()
+    p1 = P1 + tol
+    g = p1 + (P2 - p1) * (z / zmax) ** a - (P2 + tol / 2.0)
+    dom = _mround(g / tol) * tol / 2.0 + P2 / 2.0
+    dz = z[1] - z[0]
+    n_dom = int(math.ceil(np.sum(dz / dom)))
+    dsteps = _mround(dom / dz).astype(np.int64)
+    out, sid, pos = [], 0, 0.0
+    for _ in range(n_dom):
+        out.append(dom[sid])
+        sid = min(sid + int(dsteps[sid]), z.size - 1)
+        pos += out[-1]
+        if pos + dom[sid] > zmax:
+            break
+    return np.array(out)
+
+
 @dataclass
 class NonlinearCrystal:
-    """MgO:PPLN crystal (NonlinearCrystal.m, Dielectric.m) with an ideal grating."""
-    period: float                  # grating period [m]
+    """Poled crystal (NonlinearCrystal.m, Dielectric.m).
+
+    period, as NonlinearCrystal.pole (NonlinearCrystal.m:83-110): a number
+    (one period), two numbers (fan-out: P1 + (P2 - P1) * y / height), more
+    numbers (stepped: grating number ceil(y), 1-based), or a callable of the
+    step positions returning domain widths (e.g. a chirped_grating)."""
+    period: object                 # [m], see above
     length: float                  # [m]
     chi2: float = 2 * 27e-12       # NonlinearCrystal.m:53-54 default for PPLN
     temperature: float = 20.0      # Optic.m:56 default
     duty_off: float = 0.0
+    material: str = "PPLN"
+    uncertainty: float = 0.0       # domain-width uncertainty [m]
+    height: float = 1.0            # fan-out / stepped crystal height (units of y)
+    vertical_position: float = 0.0
+    seed: int = 0
+
+    def local_period(self):
+        """The single period the grating uses, or None for a domain-width callable."""
+        if callable(self.period):
+            return None
+        p = np.atleast_1d(np.asarray(self.period, dtype=np.float64))
+        if p.size == 1:
+            return float(p[0])
+        if p.size == 2:                                   # fanout, NonlinearCrystal.m:131-139
+            return float(p[0] + (p[1] - p[0]) / self.height * self.vertical_position)
+        return float(p[int(math.ceil(self.vertical_position)) - 1])
+
+    def poling(self, nsteps):
+        P = self.local_period()
+        if P is None:                                     # QPMcrystal.m:14-15
+            z = np.linspace(0.0, self.length, nsteps)
+            return poling_from_widths(nsteps, self.length, self.period(z))
+        return qpm_poling(nsteps, self.length, P, self.duty_off, self.uncertainty, self.seed)
 
     def nsteps(self, step):
         return int(math.floor(self.length / step))      # NonlinearCrystal.m:79
 
     def refractive_index(self, simwin: SimWindow):
         """Dielectric.simulate (Dielectric.m:39-46): real part, clamped at 1."""
-        nr = gayer_n(simwin.wavelengths * 1e6, self.temperature, 1)
+        nr = sellmeier_n(simwin.wavelengths * 1e6, self.material, self.temperature)
         return np.where(nr < 1, 1.0, nr)
 
     def solver_inputs(self, simwin: SimWindow, step=1e-7):
         """Arguments of the solver call (OpticalSim.m:213-226), FFT order."""
         ns = self.nsteps(step)
-        P = qpm_poling(ns, self.length, self.period, self.duty_off)
+        P = self.poling(ns)
         nr = self.refractive_index(simwin)
         ref = simwin.reference_index
         n0 = nr[ref]
         w0 = simwin.reference_omega
         G33 = P * self.chi2 * w0 / n0 / 4 / C0
         beta0_abs = 2 * np.pi * nr / simwin.wavelengths
-        phi_rel, phi_abs = phi_calc(self.length, self.temperature, simwin)
+        phi_rel, phi_abs = phi_calc(self.length, self.temperature, simwin, self.material)
         gd = phi2gd(phi_abs, simwin.d_omega)
         beta1_w0 = gd[ref] / self.length
         bdiffw0 = beta0_abs[ref] - beta1_w0 * w0

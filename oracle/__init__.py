@@ -66,6 +66,14 @@ def lib():
                                ctypes.c_int, _D, _LL]
         L.orc_gayer_n.restype = ctypes.c_double
         L.orc_gayer_n.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        L.orc_sellmeier_n.restype = ctypes.c_double
+        L.orc_sellmeier_n.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_double]
+        L.orc_phi_calc_mat.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_int, _D, _D, ctypes.c_int, _D, _D, _D]
+        L.orc_chirped_grating.argtypes = [_D, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_double, _D, ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_int)]
+        L.orc_qpm_poling_widths.argtypes = [ctypes.c_int, ctypes.c_double, _D, ctypes.c_int, _D]
         L.orc_phi_calc.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _D, _D,
                                    ctypes.c_int, _D, _D, _D]
         L.orc_qpm_poling.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
@@ -159,6 +167,50 @@ def phi_calc(L, T, w_abs, w_rel, ref):
     _check(lib().orc_phi_calc(n, L, T, _dp(w_abs), _dp(w_rel), int(ref), _dp(pr), _dp(pa), _dp(gd)),
            "phi_calc")
     return pr, pa, gd
+
+
+MATERIALS = {"PPLN": 1, "LN_e": 1, "LN_o": 2, "PPLN_undoped": 4, "OP-GaP": 10, "OPGaP": 10,
+             "KTP": 11, "PPKTP": 11}
+
+
+def sellmeier_n(lam_um, material, T=20.0):
+    """sellmeier.m:3-21 for the crystal materials (real part)."""
+    lam = np.atleast_1d(np.asarray(lam_um, dtype=np.float64))
+    code = MATERIALS[material]
+    L = lib()
+    return np.array([L.orc_sellmeier_n(float(x), code, float(T)) for x in lam])
+
+
+def phi_calc_mat(L, T, material, w_abs, w_rel, ref):
+    """phi_calc.m for any crystal material ("PPLN" uses T, the others 20 C)."""
+    w_abs = _f64(w_abs)
+    w_rel = _f64(w_rel)
+    n = w_abs.size
+    pr, pa, gd = np.zeros(n), np.zeros(n), np.zeros(n)
+    code = 0 if material == "PPLN" else MATERIALS[material]
+    _check(lib().orc_phi_calc_mat(n, L, T, code, _dp(w_abs), _dp(w_rel), int(ref), _dp(pr),
+                                  _dp(pa), _dp(gd)), "phi_calc_mat")
+    return pr, pa, gd
+
+
+def chirped_grating(z, P1, P2, a, tol):
+    """chirpedgrating.m: domain widths."""
+    z = _f64(z)
+    smallest = max(min(P1, P2) / 2.0 - tol, tol)        # narrowest domain the rounding allows
+    cap = int(np.ceil(z.size * (z[1] - z[0]) / smallest)) + 16
+    w = np.zeros(cap)
+    cnt = ctypes.c_int()
+    _check(lib().orc_chirped_grating(_dp(z), z.size, P1, P2, a, tol, _dp(w), cap,
+                                     ctypes.byref(cnt)), "chirped_grating")
+    return w[:cnt.value].copy()
+
+
+def qpm_poling_widths(nsteps, L, widths):
+    P = np.zeros(int(nsteps))
+    w = _f64(widths)
+    _check(lib().orc_qpm_poling_widths(int(nsteps), L, _dp(w), w.size, _dp(P)),
+           "qpm_poling_widths")
+    return P
 
 
 def qpm_poling(nsteps, L, period, duty_off=0.0):

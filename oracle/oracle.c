@@ -455,6 +455,36 @@ double orc_gayer_n(double lam_um, double T, int col) {
     return x > 0 ? sqrt(x) : 0.0;
 }
 
+/* Crystal materials of sellmeier.m:3-21 (material codes of this file):     */
+/*   1 "PPLN"/"LN_e", 2 "LN_o", 4 "PPLN_undoped": Gayer columns 1, 2, 4;     */
+/*   10 "OP-GaP": Wei et al. 2018 (n_opgap_wei.m:1-15, T in kelvin inside);  */
+/*   11 "KTP"/"PPKTP" n_z: Kato & Takaoka 2002 at 20 C plus the Emanueli &   */
+/*      Arie 2003 thermal terms in (T - 20) (n_ktp_zhao.m:1-38).              */
+/* Real part of MATLAB's complex sqrt, as Dielectric.m:43 and phi_calc.m:22  */
+/* take it.  Pinned by: published indices and thermo-optic coefficients.    */
+double orc_sellmeier_n(double lam_um, int material, double T) {
+    if (material == 1 || material == 2 || material == 4) return orc_gayer_n(lam_um, T, material);
+    double l2 = lam_um * lam_um;
+    if (material == 10) {
+        double Tk = T + 273.15;
+        double A = 10.926 + 7.0787e-4 * Tk + 1.8594e-7 * Tk * Tk;
+        double B = 0.53718 + 5.8035e-5 * Tk + 1.9819e-7 * Tk * Tk;
+        double D = 1504.0 + 0.25935 * Tk - 0.00023326 * Tk * Tk;
+        double x = A + B / (l2 - 0.0911014) + D / (l2 - 758.048);
+        return x > 0 ? sqrt(x) : 0.0;
+    }
+    if (material == 11) {
+        double x = 2.12725 + 1.18431 / (1.0 - 5.14852e-2 / l2) + 0.6603 / (1.0 - 100.00507 / l2) -
+                   9.68956e-3 * l2;
+        double n0 = x > 0 ? sqrt(x) : 0.0;
+        double l3 = l2 * lam_um, dT = T - 20.0;
+        double n1 = -6.1537e-6 + 64.505e-6 / lam_um - 56.447e-6 / l2 + 17.169e-6 / l3;
+        double n2 = -0.96751e-8 + 13.192e-8 / lam_um - 11.78e-8 / l2 + 3.6292e-8 / l3;
+        return n0 + n1 * dT + n2 * dT * dT;
+    }
+    return NAN;
+}
+
 /* phi_calc.m:5-45 for one PPLN length, and phi2GD.m:1-17.
  * w_abs, w_rel: N angular frequencies (SimWindow order).  ref: index of w0.
  * Outputs (any may be NULL):
@@ -468,8 +498,20 @@ double orc_gayer_n(double lam_um, double T, int col) {
  * points because T = 0 outside the 300-6000 nm SpectralLimits
  * (transmission.m:28), which lie inside the knot band.
  * Pinned by: numpy diff/cumulative trapezoid on the same arrays. */
+int orc_phi_calc_mat(int n, double L, double T, int material, const double *w_abs,
+                     const double *w_rel, int ref, double *phi_rel, double *phi_abs_out,
+                     double *gd_out);
 int orc_phi_calc(int n, double L, double T, const double *w_abs, const double *w_rel, int ref,
                  double *phi_rel, double *phi_abs_out, double *gd_out) {
+    return orc_phi_calc_mat(n, L, T, 0, w_abs, w_rel, ref, phi_rel, phi_abs_out, gd_out);
+}
+
+/* material 0 = the string "PPLN" (Gayer column 1 at T, phi_calc.m:17-18);
+ * any other code goes through sellmeier(lam, material) with its default
+ * 20 C (phi_calc.m:19-20 passes no temperature). */
+int orc_phi_calc_mat(int n, double L, double T, int material, const double *w_abs,
+                     const double *w_rel, int ref, double *phi_rel, double *phi_abs_out,
+                     double *gd_out) {
     const double c0 = 299792458.0;
     double *lam = (double *)malloc(sizeof(double) * (size_t)n);
     double *pa = (double *)malloc(sizeof(double) * (size_t)n);
@@ -478,7 +520,8 @@ int orc_phi_calc(int n, double L, double T, const double *w_abs, const double *w
     if (!lam || !pa || !gdrel || !dphi) { free(lam); free(pa); free(gdrel); free(dphi); return ORC_ENOMEM; }
     for (int j = 0; j < n; j++) {
         lam[j] = 2 * M_PI * c0 / w_abs[j];
-        double nm = orc_gayer_n(lam[j] * 1e6, T, 1);
+        double nm = material == 0 ? orc_gayer_n(lam[j] * 1e6, T, 1)
+                                  : orc_sellmeier_n(lam[j] * 1e6, material, 20.0);
         double beta = w_abs[j] * nm / c0;
         pa[j] = L * beta;
     }
@@ -551,6 +594,71 @@ int orc_qpm_poling(int nSteps, double L, double period, double dutyOff, double *
         P[i] = -(2.0 * q - 1.0);
     }
     free(w);
+    free(walls);
+    return ORC_OK;
+}
+
+/* MATLAB round: half away from zero. */
+static double mround(double x) { return x < 0 ? -floor(-x + 0.5) : floor(x + 0.5); }
+
+/* chirpedgrating.m:1-50: domain widths of a power-law chirped grating.
+ * z: nz >= 2 equally spaced positions from 0 (max(z) = z[nz-1]).  widths:
+ * capacity cap; *n_out = domains written.  Pinned by: P1 == P2 (all widths
+ * (P + tol)/2), quantisation to P2/2 + k tol/2, monotone widths for a
+ * monotone chirp, total length within max(z). */
+int orc_chirped_grating(const double *z, int nz, double P1, double P2, double a, double tol,
+                        double *widths, int cap, int *n_out) {
+    if (nz < 2 || tol <= 0 || !widths || !n_out) return ORC_EARG;
+    double zmax = z[0];
+    for (int i = 1; i < nz; i++) zmax = z[i] > zmax ? z[i] : zmax;
+    double *dom = (double *)malloc(sizeof(double) * (size_t)nz);
+    if (!dom) return ORC_ENOMEM;
+    double p1 = P1 + tol, dz = z[1] - z[0], inv_sum = 0.0;
+    for (int i = 0; i < nz; i++) {
+        double g = p1 + (P2 - p1) * pow(z[i] / zmax, a) - (P2 + tol / 2.0);
+        dom[i] = mround(g / tol) * tol / 2.0 + P2 / 2.0;
+        inv_sum += dz / dom[i];
+    }
+    int nd = (int)ceil(inv_sum);
+    int id = 0, count = 0;        /* 0-based stepID */
+    double pos = 0.0;
+    for (int ii = 0; ii < nd; ii++) {
+        if (count >= cap) { free(dom); return ORC_EARG; }
+        widths[count++] = dom[id];
+        int dstep = (int)mround(dom[id] / dz);
+        id += dstep;
+        if (id > nz - 1) id = nz - 1;
+        pos += widths[count - 1];
+        if (pos + dom[id] > zmax) break;
+    }
+    *n_out = count;
+    free(dom);
+    return ORC_OK;
+}
+
+/* QPMcrystal.m:13-16,32-51 for given domain widths (the function-handle and
+ * file branches): walls at the cumulative widths, the last domain extended
+ * or cut to L, then the sign of each z = linspace(0,L,nSteps) by the parity
+ * of its domain.  Pinned by: equal widths period/2 reproduce orc_qpm_poling. */
+int orc_qpm_poling_widths(int nSteps, double L, const double *w, int nw, double *P) {
+    if (nSteps < 2 || nw < 1) return ORC_EARG;
+    double *walls = (double *)calloc((size_t)nw + 2, sizeof(double));
+    if (!walls) return ORC_ENOMEM;
+    for (int i = 0; i < nw; i++) walls[i + 1] = walls[i] + w[i];
+    int nwalls = nw + 1;
+    if (walls[nw] < L) {
+        walls[nw + 1] = L;
+        nwalls = nw + 2;
+    } else {
+        walls[nw] = L;
+    }
+    int bin = 0;
+    for (int i = 0; i < nSteps; i++) {
+        double zz = (i == nSteps - 1) ? L : ((double)i * L) / (double)(nSteps - 1);
+        while (bin + 1 < nwalls - 1 && zz >= walls[bin + 1]) bin++;
+        int q = (bin + 1) % 2;
+        P[i] = -(2.0 * q - 1.0);
+    }
     free(walls);
     return ORC_OK;
 }

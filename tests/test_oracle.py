@@ -206,3 +206,89 @@ def test_low_gain_shg_qpm_law():
     # far off the QPM period the SH stays orders of magnitude weaker
     e3, _ = _shg_energy(0.2e-3, 30e-6)
     assert e3 < 1e-2 * e2
+
+
+# ---------------------------------------------------------------- other crystal materials
+def test_ktp_nz_published_values():
+    """KTP n_z (Kato & Takaoka 2002 at 20 C): 1.8302 at 1064.2 nm and
+    1.8894 at 532.1 nm; thermo-optic dn_z/dT near 1064 nm ~1.6-2e-5 /K
+    (Emanueli & Arie 2003)."""
+    n = oracle.sellmeier_n([1.0642, 0.5321], "KTP", 20.0)
+    assert abs(n[0] - 1.8302) < 3e-4 and abs(n[1] - 1.8894) < 1e-3
+    dndT = (oracle.sellmeier_n(1.0642, "KTP", 30.0) - oracle.sellmeier_n(1.0642, "KTP", 20.0)) / 10
+    assert 1.3e-5 < dndT[0] < 2.5e-5
+    assert np.array_equal(oracle.sellmeier_n([1.0], "PPKTP"), oracle.sellmeier_n([1.0], "KTP"))
+
+
+def test_gap_index_published_values():
+    """GaP near 300 K: n ~ 3.10 at 1.064 um and ~ 3.05 at 1.55 um, normal
+    dispersion over 1-10 um, thermo-optic ~1e-4 /K (Wei et al. 2018)."""
+    n = oracle.sellmeier_n([1.064, 1.55], "OP-GaP", 20.0)
+    assert abs(n[0] - 3.10) < 0.02 and abs(n[1] - 3.05) < 0.02
+    lam = np.linspace(1.0, 10.0, 200)
+    assert np.all(np.diff(oracle.sellmeier_n(lam, "OP-GaP")) < 0)
+    dndT = (oracle.sellmeier_n(1.55, "OP-GaP", 40.0) - oracle.sellmeier_n(1.55, "OP-GaP", 20.0)) / 20
+    assert 0.7e-4 < dndT[0] < 1.8e-4
+
+
+def test_ln_columns_through_sellmeier():
+    lam = [0.8, 1.04, 1.55, 3.0]
+    assert np.array_equal(oracle.sellmeier_n(lam, "LN_e", 35.0), oracle.gayer_n(lam, 35.0, 1))
+    assert np.array_equal(oracle.sellmeier_n(lam, "LN_o", 35.0), oracle.gayer_n(lam, 35.0, 2))
+    assert np.array_equal(oracle.sellmeier_n(lam, "PPLN_undoped", 35.0),
+                          oracle.gayer_n(lam, 35.0, 4))
+    assert np.all(oracle.sellmeier_n(lam, "LN_o") > oracle.sellmeier_n(lam, "LN_e"))
+
+
+def test_phi_calc_other_material_ignores_temperature():
+    """phi_calc.m:17-20: only the string "PPLN" passes T to the index; KTP's
+    phase is the same at any T and its GD matches the group index
+    n - lam dn/dlam of the published formula."""
+    sw = ph.SimWindow(1040e-9, 2048, 6e-12)
+    args = (sw.omegas, sw.rel_omegas, sw.reference_index)
+    a = oracle.phi_calc_mat(1e-3, 20.0, "KTP", *args)
+    b = oracle.phi_calc_mat(1e-3, 80.0, "KTP", *args)
+    assert np.array_equal(a[1], b[1])
+    c = oracle.phi_calc_mat(1e-3, 20.0, "PPLN", *args)
+    d = oracle.phi_calc_mat(1e-3, 80.0, "PPLN", *args)
+    assert not np.array_equal(c[1], d[1])
+    assert np.array_equal(c[1], oracle.phi_calc(1e-3, 20.0, *args)[1])
+    i = sw.reference_index
+    lam = 1.04
+    h = 1e-4
+    n = oracle.sellmeier_n([lam - h, lam, lam + h], "KTP")
+    ng = n[1] - lam * (n[2] - n[0]) / (2 * h)
+    gd = a[2][i] / 1e-3 * ph.C0                      # group index from the phase slope
+    assert abs(gd - ng) < 2e-4
+
+
+# ---------------------------------------------------------------- general gratings
+def test_chirped_grating_invariants():
+    z = np.linspace(0.0, 1e-3, 10000)
+    # equal end periods: with the P1 + tol of chirpedgrating.m:10 the quantised
+    # period is P except within half a step of either end (P + tol, P - tol)
+    w = oracle.chirped_grating(z, 30e-6, 30e-6, 0.5, 0.1e-6)
+    steps = np.round((2 * w - 30e-6) / 0.1e-6)
+    assert np.allclose(2 * w, 30e-6 + steps * 0.1e-6, rtol=0, atol=1e-18)
+    assert set(steps.tolist()) <= {-1.0, 0.0, 1.0} and np.count_nonzero(steps) <= 2
+    assert w.sum() <= 1e-3 and w.sum() + w[-1] > 1e-3
+    # linear chirp 6.3 -> 2.2 um: widths P2/2 + k tol/2, non-increasing, first ~ (P1 + tol)/2
+    w = oracle.chirped_grating(z, 6.3e-6, 2.2e-6, 1.0, 0.1e-6)
+    k = (w - 2.2e-6 / 2) / 0.05e-6
+    assert np.allclose(k, np.round(k), atol=1e-6)
+    assert np.all(np.diff(w) <= 1e-18)
+    assert abs(w[0] - 6.4e-6 / 2) < 0.05e-6 + 1e-18
+    assert w.sum() <= 1e-3
+    # the local period follows P1 + (P2 - P1) z / L within the quantisation
+    zc = np.cumsum(w) - w / 2
+    per = 6.4e-6 + (2.2e-6 - 6.4e-6) * zc / 1e-3
+    assert np.max(np.abs(2 * w - per)) < 0.1e-6 + 2 * np.max(np.abs(np.diff(w)))
+
+
+def test_poling_from_equal_widths_is_the_uniform_grating():
+    L, period, ns = 1e-3, 30e-6, 10000
+    n_dom = int(math.floor(2 * L / period))
+    P = oracle.qpm_poling_widths(ns, L, np.full(n_dom, period / 2))
+    assert np.array_equal(P, oracle.qpm_poling(ns, L, period))
+    # a single wide domain longer than L is cut at L: all -1
+    assert np.all(oracle.qpm_poling_widths(100, L, [2 * L]) == -1.0)

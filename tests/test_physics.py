@@ -76,3 +76,69 @@ def test_cavity_operators_compose_the_reference_trip():
     dw = 2 * np.pi * np.fft.fftfreq(512, sw.dt)
     fused = ops["cav"] * np.exp(-1j * cfg.delay * dw) * np.fft.fft(X) + ops["pump"]
     assert np.linalg.norm(fused - lit) / np.linalg.norm(lit) < 1e-12
+
+
+def test_crystal_materials_match_oracle():
+    lam = np.linspace(0.4, 5.0, 97)
+    for mat in ("PPLN", "LN_e", "LN_o", "PPLN_undoped", "OP-GaP", "KTP", "PPKTP"):
+        for T in (20.0, 65.0):
+            assert np.allclose(ph.sellmeier_n(lam, mat, T), oracle.sellmeier_n(lam, mat, T),
+                               rtol=1e-14, atol=0), (mat, T)
+    with np.testing.assert_raises(ValueError):
+        ph.sellmeier_n(lam, "N-BK7")
+
+
+def test_solver_inputs_other_material_match_oracle():
+    """A KTP crystal: beta from phi_calc's sellmeier branch (20 C whatever the
+    crystal temperature), the index at the crystal temperature."""
+    sw = ph.SimWindow(1040e-9, 2048, 6e-12)
+    xt = ph.NonlinearCrystal(30e-6, 1e-3, material="KTP", temperature=50.0)
+    si = xt.solver_inputs(sw)
+    pr, pa, gd = oracle.phi_calc_mat(1e-3, 50.0, "KTP", sw.omegas, sw.rel_omegas,
+                                     sw.reference_index)
+    hb = np.where(np.isfinite(pr), pr / 1e-3 * 1e-7, 0.0)
+    band = np.fft.ifftshift(sw.in_band())
+    assert np.allclose(si["beta_op"][band], np.fft.ifftshift(hb)[band], rtol=1e-12, atol=1e-15)
+    n0 = oracle.sellmeier_n(1.04, "KTP", 50.0)[0]
+    assert abs(si["n0"] - n0) < 1e-14
+    P = oracle.qpm_poling(10000, 1e-3, 30e-6)
+    assert np.allclose(si["G33"], P * xt.chi2 * sw.reference_omega / n0 / 4 / ph.C0, rtol=1e-14)
+
+
+def test_gratings_match_oracle():
+    L, ns = 1e-3, 10000
+    z = np.linspace(0.0, L, ns)
+    for P1, P2, a in ((6.3e-6, 2.2e-6, 0.48), (28e-6, 32e-6, 1.0), (30e-6, 30e-6, 2.0)):
+        w = ph.chirped_grating(z, P1, P2, a, 0.1e-6)
+        assert np.array_equal(w, oracle.chirped_grating(z, P1, P2, a, 0.1e-6))
+        assert np.array_equal(ph.poling_from_widths(ns, L, w),
+                              oracle.qpm_poling_widths(ns, L, w))
+    # a crystal whose grating is a callable of the step positions (QPMcrystal.m:14-15)
+    xt = ph.NonlinearCrystal(lambda zz: ph.chirped_grating(zz, 31e-6, 29e-6, 1.0, 0.1e-6), L)
+    assert np.array_equal(xt.poling(ns), oracle.qpm_poling_widths(
+        ns, L, oracle.chirped_grating(z, 31e-6, 29e-6, 1.0, 0.1e-6)))
+
+
+def test_fanout_and_stepped_periods():
+    """NonlinearCrystal.m:88-99,131-139: two periods fan out over the height,
+    more periods are stepped gratings picked by ceil(y)."""
+    xt = ph.NonlinearCrystal((29e-6, 31e-6), 1e-3, height=10.0, vertical_position=5.0)
+    assert abs(xt.local_period() - 30e-6) < 1e-18
+    assert np.array_equal(xt.poling(5000), oracle.qpm_poling(5000, 1e-3, 30e-6))
+    xs = ph.NonlinearCrystal((28e-6, 29e-6, 30e-6, 31e-6), 1e-3, vertical_position=2.3)
+    assert xs.local_period() == 30e-6
+
+
+def test_domain_width_uncertainty():
+    """QPMcrystal.m:27-30 with uncertainty: random odd domains, even domains
+    completing each period, so every wall but the random ones stays on the
+    period grid; the draw is seeded (MATLAB's stream is not reproduced)."""
+    L, period, ns = 1e-3, 30e-6, 20000
+    a = ph.qpm_poling(ns, L, period, uncertainty=1e-6, seed=3)
+    b = ph.qpm_poling(ns, L, period, uncertainty=1e-6, seed=3)
+    c = ph.qpm_poling(ns, L, period, uncertainty=1e-6, seed=4)
+    u = ph.qpm_poling(ns, L, period)
+    assert np.array_equal(a, b) and not np.array_equal(a, c) and not np.array_equal(a, u)
+    assert set(np.unique(a)) == {-1.0, 1.0}
+    assert abs(np.count_nonzero(np.diff(a)) - np.count_nonzero(np.diff(u))) <= 1
+    assert np.count_nonzero(a != u) < 0.1 * ns      # walls move by ~0.5 um of 15 um domains
