@@ -165,14 +165,18 @@ int ofx_opo_read(ofx_opo* run, int inst, double* field, double* oc_spec, double*
 int ofx_opo_read_average(ofx_opo* run, int inst, double* oc_esd, double* xout_esd, int* trips);
 /* clear both averages (the next trip starts new ones) */
 int ofx_opo_reset_average(ofx_opo* run, int inst);
-/* Keep the crystal-output spectrum (FFT order, inside the crystal) of the
- * last `trips` round trips of every cavity instance in device memory, from
- * the next trip on (trips x n complex per instance; once per run).  Replaces
- * StoredPulses (OpticalSim.m:173-180,268-272), which gathers every trip's
- * crystal output to the host. */
+/* Keep the crystal-output spectrum (FFT order, inside the crystal) and the
+ * step multipliers of the last `trips` round trips of every cavity instance
+ * in device memory, from the next trip on (trips x (n complex + nsteps
+ * floats) per instance; once per run).  Replaces StoredPulses
+ * (OpticalSim.m:173-180,268-272) and the per-trip rows of StepSizeModifiers
+ * (OpticalSim.m:172,237), which gather every trip to the host. */
 int ofx_opo_keep_history(ofx_opo* run, int trips);
-/* spectrum [2n] of cumulative 0-based trip `trip` of one instance, if held */
-int ofx_opo_read_history(ofx_opo* run, int inst, int trip, double* spectrum);
+/* cumulative 0-based trip `trip` of one instance, if held (either pointer
+ * may be NULL): spectrum [2n]; stepmods [nsteps], the multiplier accepted at
+ * each 1-based minimal step k in entry k-1 and 0 elsewhere, as the row
+ * StepSizeModifiers(trip, :) */
+int ofx_opo_read_history(ofx_opo* run, int inst, int trip, double* spectrum, double* stepmods);
 /* overwrite the crystal-input field of one instance (warm start / checkpoint
  * restore); resets its status to running */
 int ofx_opo_write_field(ofx_opo* run, int inst, const double* field);

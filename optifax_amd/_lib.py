@@ -126,7 +126,7 @@ def load(path: str | None = None):
     L.ofx_opo_read_average.argtypes = [vp, ctypes.c_int, _DP, _DP, _IP]
     L.ofx_opo_reset_average.argtypes = [vp, ctypes.c_int]
     L.ofx_opo_keep_history.argtypes = [vp, ctypes.c_int]
-    L.ofx_opo_read_history.argtypes = [vp, ctypes.c_int, ctypes.c_int, _DP]
+    L.ofx_opo_read_history.argtypes = [vp, ctypes.c_int, ctypes.c_int, _DP, _DP]
     if hasattr(L, "ofx_mlp_create"):   # the CPU emulation library has no surrogate
         L.ofx_mlp_create.argtypes = [ctypes.c_int] * 5 + [ctypes.POINTER(vp)]
         L.ofx_mlp_destroy.argtypes = [vp]

@@ -55,6 +55,7 @@ struct InstDesc {
   double* esd_sum;      // N: sum over trips of |output-coupled spectrum|^2 (FFT order)
   double* xesd_sum;     // N: sum over trips of |crystal-output spectrum|^2 (FFT order)
   void* hist;           // [hist_cap][N] complex: crystal-output spectra of recent trips, or null
+  float* sm_hist;       // [hist_cap][nsteps]: step multipliers of those trips (0 = not accepted)
   int hist_cap;         // trips held; trip t (cumulative, 0-based) sits in slot t % hist_cap
   int pad2;
   double* trip_log;     // [trip_cap][4]: conv metric, oc energy, attempts, accepted

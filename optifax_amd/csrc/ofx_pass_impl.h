@@ -1180,6 +1180,13 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
     long long trip_att = 0, trip_acc = 0;
     double k = 1.0, stepmod = 4.0;
     int chunk = 1;
+    // In round trips every trip's step multipliers start from a zero row, as
+    // the rows of StepSizeModifiers do (OpticalSim.m:172,237).  Rank 0 owns
+    // them; every one of its threads writes each accepted entry, and thread
+    // j clears (and at the trip's end copies) entries j, j + NT, ..., so each
+    // thread's own program order already orders its clear, writes and copy.
+    if (a.trips > 0 && rank == 0 && d.stepmods)
+      for (int j = OFX_TID; j < nsteps; j += OFX_NT) d.stepmods[j] = 0.0f;
     if (d.method == METHOD_RK23) {
       status = rk23_pass<R, MP, XM>(a, cd, d, g, rank, red, sy, breg, areg, trip_att, trip_acc);
       attempts += trip_att;
@@ -1319,6 +1326,10 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
       chunk = cend;
     }
     if (status != ST_RUNNING) break;
+    if (rank == 0 && d.sm_hist && d.stepmods) {
+      float* row = d.sm_hist + (size_t)((trip_base + trip) % d.hist_cap) * (size_t)nsteps;
+      for (int j = OFX_TID; j < nsteps; j += OFX_NT) row[j] = d.stepmods[j];
+    }
     if (rank == 0 && d.trip_log && trip_base + trip < d.trip_cap) {
       double* tl = d.trip_log + 4 * (size_t)(trip_base + trip);
       tl[2] = (double)trip_att;

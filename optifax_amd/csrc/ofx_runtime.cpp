@@ -42,6 +42,7 @@ struct InstBufs {
   void* xesd_sum = nullptr;
   int esd_base = 0;  // trips_done when the run averages were last cleared
   void* hist = nullptr;
+  void* sm_hist = nullptr;
   int hist_cap = 0;
   int hist_base = 0;  // trips_done when the history started
   void* trip_log = nullptr;
@@ -819,9 +820,11 @@ int ofx_opo_keep_history(ofx_opo* r, int trips) {
     const size_t n = (size_t)r->pass[i].n;
     st = trips_done_of(r, i, &b.hist_base);
     if (!st) st = dev_alloc(r, &b.hist, (size_t)trips * n * r->csz);
+    if (!st) st = dev_alloc(r, &b.sm_hist, (size_t)trips * (size_t)r->pass[i].nsteps * 4);
     if (!st) {
       b.hist_cap = trips;
       r->inst_h[i].hist = b.hist;
+      r->inst_h[i].sm_hist = (float*)b.sm_hist;
       r->inst_h[i].hist_cap = trips;
     }
   }
@@ -830,8 +833,8 @@ int ofx_opo_keep_history(ofx_opo* r, int trips) {
   return st;
 }
 
-int ofx_opo_read_history(ofx_opo* r, int inst, int trip, double* spectrum) {
-  if (!r || !spectrum || inst < 0 || inst >= r->n_inst) return OFX_E_ARG;
+int ofx_opo_read_history(ofx_opo* r, int inst, int trip, double* spectrum, double* stepmods) {
+  if (!r || inst < 0 || inst >= r->n_inst) return OFX_E_ARG;
   const InstBufs& b = r->bufs[inst];
   if (!b.hist) return fail(r->ctx, OFX_E_ARG, "instance %d keeps no history", inst);
   int done = 0;
@@ -842,7 +845,19 @@ int ofx_opo_read_history(ofx_opo* r, int inst, int trip, double* spectrum) {
     return fail(r->ctx, OFX_E_ARG, "trip %d of instance %d is not held (held: %d..%d)", trip, inst,
                 lo, done - 1);
   const size_t n = (size_t)r->pass[inst].n;
-  return read_conv(r, (char*)b.hist + (size_t)(trip % b.hist_cap) * n * r->csz, spectrum, 2 * n);
+  if (spectrum) {
+    st = read_conv(r, (char*)b.hist + (size_t)(trip % b.hist_cap) * n * r->csz, spectrum, 2 * n);
+    if (st) return st;
+  }
+  if (stepmods) {
+    const size_t ns = (size_t)r->pass[inst].nsteps;
+    std::vector<float> sm(ns);
+    if (be_d2h(r->ctx->be, sm.data(), (float*)b.sm_hist + (size_t)(trip % b.hist_cap) * ns,
+               ns * 4) || be_sync(r->ctx->be))
+      return fail(r->ctx, OFX_E_HIP, "readback failed: %s", be_error(r->ctx->be));
+    for (size_t j = 0; j < ns; j++) stepmods[j] = sm[j];
+  }
+  return OFX_OK;
 }
 
 int ofx_opo_write_field(ofx_opo* r, int inst, const double* field) {

@@ -304,13 +304,15 @@ class OPOBatch:
         self.ctx.check(self.lib.ofx_opo_keep_history(self.h, int(trips)))
 
     def read_history(self, inst, trip):
-        """Crystal-output spectrum (FFT order) of cumulative 0-based trip `trip`."""
-        n = self.meta[inst]["n"]
-        out = np.zeros(n, np.complex128)
-        self.ctx.check(self.lib.ofx_opo_read_history(
-            self.h, inst, int(trip), out.view(np.float64).ctypes.data_as(
-                ctypes.POINTER(ctypes.c_double))))
-        return out
+        """(spectrum, stepmods) of cumulative 0-based trip `trip`: the
+        crystal-output spectrum (FFT order) and the row of step multipliers
+        (StepSizeModifiers(trip, :), OpticalSim.m:172,237)."""
+        m = self.meta[inst]
+        out = np.zeros(m["n"], np.complex128)
+        sm = np.zeros(m["nsteps"])
+        dp = lambda a: a.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        self.ctx.check(self.lib.ofx_opo_read_history(self.h, inst, int(trip), dp(out), dp(sm)))
+        return out, sm
 
     def write_field(self, inst, field):
         a = np.ascontiguousarray(np.asarray(field, dtype=np.complex128).reshape(-1))

@@ -156,11 +156,17 @@ def test_rk23_round_trips_match_literal_opticalsim(emu_ctx):
     run = OPOBatch([dict(solver=si, field0=ops["field0"], pump=ops["pump"], cav=ops["cav"],
                          ocm=ops["ocm"], delay=cfg.delay, method="rk23")], precision="double",
                    trip_cap=trips, ctx=emu_ctx, tune={"cluster": 2})
+    run.keep_history(trips)
     run.launch(trips)
     run.sync()
     r = run.read(0)
     assert np.array_equal(r["trip_log"][:, 2:].astype(int), o["steps"])
     assert rel_l2(r["field"], oracle_next_input(b, o["E_cav"])) < 1e-10
+    # the first trip's row of step multipliers is the oracle RK23 pass's
+    _, _, sm_o, _ = oracle.rk23(ops["field0"], si["alpha_w"], si["G33"], si["w0"], si["bdiffw0"],
+                                si["h"], si["nSteps"], si["dt"], si["beta_op"], si["maxErr"],
+                                si["minErr"], si["sel"], n_cols=0, want_stepmods=True)
+    assert np.array_equal(run.read_history(0, 0)[1], sm_o)
     run.close()
 
 
@@ -243,9 +249,11 @@ def test_wait_limit_gives_up_and_the_next_launch_recovers(emu_ctx, exchange):
 def test_run_averages_and_trip_history(emu_ctx, exchange):
     """Device-side run averages (ESDOutAverage and the crystal-output ESD
     behind ESDPumpDepAverage / ESDIdlerICAverage, OpticalSim.m:302-313) and
-    the trip history (StoredPulses, OpticalSim.m:268-272) against the
-    oracle's output-coupled spectrum of each trip: the held crystal output
-    times the output-coupling transfer must give that spectrum."""
+    the trip history (StoredPulses, OpticalSim.m:268-272, and the rows of
+    StepSizeModifiers, OpticalSim.m:172,237) against the oracle: the held
+    crystal output times the output-coupling transfer must give the oracle's
+    output spectrum of that trip, and the held step multipliers the oracle
+    pass's on that trip's crystal input."""
     cfg, b = opo_case()
     si, ops = b["solver"], b["ops"]
     oc = oracle_oc_spectra(b, cfg.delay, 6)
@@ -254,17 +262,28 @@ def test_run_averages_and_trip_history(emu_ctx, exchange):
                    ctx=emu_ctx, tune={"cluster": 2, "exchange": exchange})
     e, x, k = run.read_average(0)
     assert k == 0 and not e.any() and not x.any()
+    run.keep_history(3)
     run.launch(1)
-    run.keep_history(3)                        # held from the second trip (index 1) on
     run.launch(2)
     run.sync()
     e, x, k = run.read_average(0)
     assert k == 3 and rel_l2(e, np.mean([abs(s) ** 2 for s in oc[:3]], axis=0)) < 1e-10
-    hs = [run.read_history(0, t) for t in (1, 2)]
-    for t, h in zip((1, 2), hs):
+    R, T, phi = b["oc"].masks(b["simwin"])
+    sw = b["simwin"]
+    inputs = [ops["field0"]] + [
+        oracle_next_input(b, oracle.opo_run(t, b["noise"], b["pump"], np.ones(sw.n), si["n_xtal"],
+                                            R, T, phi, cfg.delay, sw.rel_omegas, si)["E_cav"])
+        for t in (1, 2)]
+    for t in range(3):
+        h, sm = run.read_history(0, t)
         assert rel_l2(h * ops["ocm"], oc[t]) < 1e-10
+        _, _, sm_o, _ = oracle.nee_heun(inputs[t], si["alpha_w"], si["G33"], si["w0"],
+                                        si["bdiffw0"], si["h"], si["nSteps"], si["dt"],
+                                        si["beta_op"], si["maxErr"], si["minErr"], si["sel"],
+                                        n_cols=0, want_stepmods=True)
+        assert np.array_equal(sm, sm_o)
     with pytest.raises(OfxError):
-        run.read_history(0, 0)                 # before the history started
+        run.read_history(0, 3)                 # not run yet
     with pytest.raises(OfxError):
         run.keep_history(2)                    # once per run
     run.reset_average(0)
@@ -272,7 +291,7 @@ def test_run_averages_and_trip_history(emu_ctx, exchange):
     run.sync()
     e, x, k = run.read_average(0)
     assert k == 3 and rel_l2(e, np.mean([abs(s) ** 2 for s in oc[3:6]], axis=0)) < 1e-10
-    hs = [run.read_history(0, t) for t in (3, 4, 5)]
+    hs = [run.read_history(0, t)[0] for t in (3, 4, 5)]
     for t, h in zip((3, 4, 5), hs):
         assert rel_l2(h * ops["ocm"], oc[t]) < 1e-10
     assert rel_l2(x, np.mean([abs(h) ** 2 for h in hs], axis=0)) < 1e-12
