@@ -150,6 +150,30 @@ def test_config2_full_size_invariants(gpu_ctx):
     assert abs(s["trip_log"][1, 0] - d["trip_log"][1, 0]) < 1e-2
 
 
+_MAX_CASE = {}
+
+
+def _max_size_case():
+    """The largest grid the device path takes (N = 2^16, 24 ps, 0.1 mm):
+    oracle result computed once."""
+    if not _MAX_CASE:
+        sw, si, E, args = crystal_case(n=65536, window=24e-12, L=0.1e-3)
+        _MAX_CASE["case"] = (E, args, oracle.nee_heun(E, *args, n_cols=4))
+    return _MAX_CASE["case"]
+
+
+@pytest.mark.parametrize("C,exchange", [(64, "barrier"), (128, "tags")])
+def test_pass_max_size(gpu_ctx, C, exchange):
+    """(FP64 fields need 64 workgroups or more at this N to fit the LDS.)"""
+    E, args, (Eo, apo, smo, cnt) = _max_size_case()
+    Eg, apg, smg = _run(gpu_ctx, E, args, 65536, "double", C, exchange=exchange)
+    assert np.array_equal(smg, smo)
+    assert rel_l2(Eg, Eo) < 1e-9 and rel_l2(apg, apo) < 1e-9
+    Ef, _, smf = _run(gpu_ctx, E, args, 65536, "single", C, exchange=exchange)
+    assert np.array_equal(smf, smo)
+    assert rel_l2(Ef, Eo) < 2e-4
+
+
 # ---------------------------------------------------------------- RK23 (ode23 variant)
 def _run_rk(ctx, E, args, n, precision, C=None, cols=4, exchange="auto"):
     tune = {"cluster": C, "exchange": exchange} if C else None
