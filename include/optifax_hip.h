@@ -94,7 +94,8 @@ int ofx_device_info(const ofx_context* ctx, char* name, int name_len, int* cus, 
 
 /*
  * One crystal pass over `batch` fields in host memory (OPOmexBatch semantics,
- * Heun step control of NEE_CPU.m:39-94 applied independently per instance).
+ * Heun step control of NEE_CPU.m:39-94 applied independently per instance;
+ * batch 0 returns OFX_OK with nothing done, as an empty Et does there).
  *  Et       [2*n*batch] in/out, complex64 interleaved, fftshifted time
  *  alpha_w  [n]          T^(1/NSteps) in FFT order (NonlinearCrystal.m:141-144)
  *  G        [nsteps*g_batch] coupling per minimal step; g_batch 1 (shared) or batch

@@ -71,6 +71,7 @@ extern "C" void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* 
   const size_t nE = mxGetNumberOfElements(prhs[0]);
   if (N == 0 || nE % N) mexErrMsgIdAndTxt("optifax:OPOmexCavity:size", "numel(Et) must be N*B");
   const int B = (int)(nE / N);
+  if (B == 0) mexErrMsgIdAndTxt("optifax:OPOmexCavity:size", "Et holds no instance");
   std::vector<double> E = to_double(prhs[0], true, "Et");
   std::vector<double> alpha = to_double(prhs[1], false, "alpha_w");
   std::vector<double> G = to_double(prhs[2], false, "G");

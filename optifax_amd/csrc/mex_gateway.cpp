@@ -89,6 +89,14 @@ extern "C" void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* 
     mexErrMsgIdAndTxt("optifax:OPOmexBatch:size", "numel(Et)=%zu is not a multiple of numel(beta_op)=%zu",
                       nE, N);
   const int B = (int)(nE / N);
+  if (B == 0) {
+    // an empty batch: nothing to solve; the outputs are copies of the
+    // inputs, as the reference returns its copied arrays (OPO_mex_BATCH.cpp:42,55-56,120-122)
+    plhs[0] = mxDuplicateArray(aEt);
+    plhs[1] = mxDuplicateArray(prhs[12]);
+    plhs[2] = mxDuplicateArray(prhs[13]);
+    return;
+  }
   std::vector<float> Et = complex_values(aEt, "Et");
   std::vector<float> alpha = real_values(prhs[1], "alpha_w");
   std::vector<float> G = real_values(prhs[2], "G");

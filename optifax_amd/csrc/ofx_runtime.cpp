@@ -902,8 +902,9 @@ static int pass_host_impl(ofx_context* ctx, int prec, const ofx_pass_params* p, 
   if (!ctx) return OFX_E_ARG;
   int st = check_pass(ctx, p);
   if (st) return st;
-  if (batch < 1 || !Et || !alpha_w || !G || !beta_op)
-    return fail(ctx, OFX_E_ARG, "Et, alpha_w, G, beta_op are required and batch >= 1");
+  if (batch == 0) return OFX_OK;  // an empty batch: nothing to solve
+  if (batch < 0 || !Et || !alpha_w || !G || !beta_op)
+    return fail(ctx, OFX_E_ARG, "Et, alpha_w, G, beta_op are required and batch >= 0");
   if (g_batch != 1 && g_batch != batch)
     return fail(ctx, OFX_E_ARG, "G must hold NSteps or NSteps*batch values (g_batch=%d)", g_batch);
   const size_t n = (size_t)p->n;

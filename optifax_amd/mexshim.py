@@ -96,6 +96,8 @@ def from_mx(h):
     cpx = L.mxIsComplex(h)
     base = {mxSINGLE: np.float32, mxDOUBLE: np.float64, mxUINT32: np.uint32}[cls]
     n = int(np.prod(shape))
+    if n == 0:
+        return np.zeros(shape, np.result_type(base, np.complex64) if cpx else base)
     raw = np.ctypeslib.as_array((ctypes.c_char * (n * np.dtype(base).itemsize * (2 if cpx else 1)))
                                 .from_address(L.mxGetData(h))).view(base).copy()
     arr = raw[0::2] + 1j * raw[1::2] if cpx else raw

@@ -125,6 +125,10 @@ def OPOmexBatch(Et, alpha_w, G, w0, bdiffw0, h, NSteps, dt, beta_op, maxErr, min
     squeeze = Et_a.ndim == 1
     E2 = Et_a.reshape(N, -1)
     B = E2.shape[1]
+    if B == 0:       # an empty batch: nothing to solve, copies returned
+        out = (Et_a.copy(), None if ApFT is None else np.array(ApFT, copy=True),
+               None if stepMods is None else np.array(stepMods, copy=True))
+        return (*out, np.zeros(0, np.int32)) if return_status else out
     Ef = np.ascontiguousarray(E2.T.astype(cdt))          # (B, N) rows
     G_a = np.asarray(G, dtype=rdt)
     nsteps = int(NSteps)

@@ -116,6 +116,15 @@ def test_forced_rejection_terminates(emu_ctx):
     assert rel_l2(Eg, Eo) < 1e-10
 
 
+def test_empty_batch_is_a_no_op(emu_ctx):
+    sw, si, E, args = crystal_case(n=256, window=2e-12)
+    sm_in = np.arange(args[5], dtype=np.float32)
+    Eg, apg, smg, st = OPOmexBatch(np.zeros((256, 0), complex), *args, None, sm_in, ctx=emu_ctx,
+                                   return_status=True)
+    assert Eg.shape == (256, 0) and apg is None and st.size == 0
+    assert np.array_equal(smg, sm_in)
+
+
 def test_argument_errors(emu_ctx):
     sw, si, E, args = crystal_case(n=256, window=2e-12)
     with pytest.raises(ValueError):

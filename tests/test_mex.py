@@ -46,6 +46,20 @@ def test_shape_errors_before_any_device_work():
     assert e.value.identifier == "optifax:OPOmexBatch:size"
 
 
+def test_empty_batch_returns_copies():
+    """numel(Et) = 0: nothing to solve; the gateway returns copies of Et,
+    ApFT and stepMods (OPO_mex_BATCH.cpp:42,55-56,120-122) without touching
+    a device."""
+    sw, si, E, args = crystal_case(n=256, window=2e-12)
+    a = _mex_args(E, args)
+    a[0] = np.zeros((256, 0), np.complex64)
+    a[12] = np.zeros((256, 4, 0), np.complex64)
+    a[13] = np.arange(a[13].size, dtype=np.float32).reshape(a[13].shape)
+    Et, Ap, sm = mexshim.call("OPOmexBatch", *a)
+    assert Et.shape == (256, 0) and Ap.size == 0
+    assert np.array_equal(sm, a[13])
+
+
 @pytest.mark.gpu
 def test_opomexbatch_matches_c_api_and_oracle(gpu_ctx):
     sw, si, E, args = crystal_case(n=1024, window=4e-12)
