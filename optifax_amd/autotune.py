@@ -59,7 +59,7 @@ def _store_cache(key, value):
         pass
 
 
-def candidates(n, max_cluster=64):
+def candidates(n, max_cluster=128):
     """(cluster, n1, block) triples to try for grid size n."""
     lg = int(round(math.log2(n)))
     if 1 << lg != n:
@@ -107,14 +107,16 @@ def measure(inst, cand, k, precision="single", steps=1500, ctx=None, repeats=2):
         return None
     try:
         best = None
+        before = 0
         for _ in range(repeats):
             run.launch(0)
             run.sync(allow_nonfinite=True)
             ms = run.last_ms()
-            r = run.read(0)
-            if r["attempts"] < 1:
+            total = run.read(0)["attempts"]       # cumulative over the run's launches
+            att, before = total - before, total
+            if att < 1:
                 return None
-            t = ms * 1e-3 / r["attempts"]
+            t = ms * 1e-3 / att
             best = t if best is None else min(best, t)
         return best
     finally:

@@ -19,7 +19,7 @@ def test_candidates_cover_valid_splits():
     assert all((1 << 15) % n1 == 0 and C <= min(n1, (1 << 15) // n1) for C, n1, _ in c)
     assert {n1 for _, n1, _ in c} == {128, 256, 512}
     assert {b for _, _, b in c} == {256, 512}
-    assert max(C for C, _, _ in c) == 64
+    assert max(C for C, _, _ in c) == 128
     with pytest.raises(ValueError):
         autotune.candidates(1000)
 
@@ -73,3 +73,14 @@ def test_measurement_pass_takes_steps(emu_ctx):
     run.sync()
     assert run.read(0)["attempts"] >= 3
     run.close()
+
+
+def test_measure_counts_each_launch_once(emu_ctx, monkeypatch):
+    """The run's attempt counter is cumulative over its launches: every
+    repeat must be divided by its own attempts, not the running total."""
+    from optifax_amd import solver
+    monkeypatch.setattr(solver.OPOBatch, "last_ms", lambda self: 1.0)
+    inst = _inst()
+    t1 = autotune.measure(inst, (2, 16, 256), 1, "double", steps=12, ctx=emu_ctx, repeats=1)
+    t3 = autotune.measure(inst, (2, 16, 256), 1, "double", steps=12, ctx=emu_ctx, repeats=3)
+    assert t1 is not None and t3 == t1
