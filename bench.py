@@ -131,7 +131,7 @@ def main_warmstart(args):
     plain = warm = None
     if args.ws_phase in ("all", "plain"):
         plain = ws.solve_plain(insts, conv_tol=1e-4, budget=1000, log=log,
-                               max_seconds=args.ws_max_seconds)
+                               max_seconds=args.ws_max_seconds, precision=args.precision)
         cfg_out.update(plain_seconds=plain["seconds"],
                        plain_conv_seconds=[None if not np.isfinite(x) else round(float(x), 2)
                                            for x in plain["conv_seconds"]],
@@ -189,7 +189,7 @@ def main_warmstart(args):
         "higher_is_better": plain is not None and warm is not None,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 fields, bf16 surrogate",
+        "dtype": ("f32" if args.precision == "single" else "f64") + " fields (plain), f32 fields + bf16 surrogate (warm)",
         "data": "synthetic (sech^2 pump + seeded one-photon-per-mode noise; surrogate "
                 "trained on solver trajectories)",
         "config": cfg_out,
