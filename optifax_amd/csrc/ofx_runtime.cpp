@@ -812,11 +812,13 @@ int ofx_opo_reset_average(ofx_opo* r, int inst) {
 
 int ofx_opo_keep_history(ofx_opo* r, int trips) {
   if (!r || trips < 1) return OFX_E_ARG;
+  for (int i = 0; i < r->n_inst; i++)
+    if (r->bufs[i].hist) return fail(r->ctx, OFX_E_ARG, "the run already keeps a history");
   int st = OFX_OK;
   for (int i = 0; i < r->n_inst && !st; i++) {
     InstBufs& b = r->bufs[i];
     if (!b.esd_sum) continue;
-    if (b.hist) return fail(r->ctx, OFX_E_ARG, "instance %d already keeps a history", i);
+
     const size_t n = (size_t)r->pass[i].n;
     st = trips_done_of(r, i, &b.hist_base);
     if (!st) st = dev_alloc(r, &b.hist, (size_t)trips * n * r->csz);
@@ -830,6 +832,16 @@ int ofx_opo_keep_history(ofx_opo* r, int trips) {
   }
   if (!st) st = upload(r, r->inst_d, r->inst_h.data(), sizeof(InstDesc) * r->n_inst);
   if (!st && be_sync(r->ctx->be)) st = fail(r->ctx, OFX_E_HIP, "upload failed");
+  if (st) {
+    // leave no instance half set up (the buffers stay with the run until it is destroyed)
+    for (int i = 0; i < r->n_inst; i++) {
+      r->bufs[i].hist = r->bufs[i].sm_hist = nullptr;
+      r->bufs[i].hist_cap = 0;
+      r->inst_h[i].hist = nullptr;
+      r->inst_h[i].sm_hist = nullptr;
+      r->inst_h[i].hist_cap = 0;
+    }
+  }
   return st;
 }
 
