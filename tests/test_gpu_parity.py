@@ -93,6 +93,13 @@ def test_batch_invariance_and_uneven_batch(gpu_ctx):
     run.close()
 
 
+def test_empty_batch_is_a_no_op(gpu_ctx):
+    sw, si, E, args = crystal_case(n=256, window=2e-12)
+    Eg, apg, smg, st = OPOmexBatch(np.zeros((256, 0), complex), *args, None, None, ctx=gpu_ctx,
+                                   return_status=True)
+    assert Eg.shape == (256, 0) and st.size == 0
+
+
 def test_nonfinite_instance_reports_status(gpu_ctx):
     sw, si, E, args = crystal_case(n=256, window=2e-12)
     E2 = np.stack([E, E], axis=1)
