@@ -66,7 +66,6 @@ std::vector<double> to_double(const mxArray* a, bool want_complex, const char* n
 
 extern "C" void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
   if (nrhs != 18) mexErrMsgIdAndTxt("optifax:OPOmexCavity:nrhs", "OPOmexCavity takes 18 inputs");
-  (void)nlhs;
   const size_t N = mxGetNumberOfElements(prhs[8]);
   const size_t nE = mxGetNumberOfElements(prhs[0]);
   if (N == 0 || nE % N) mexErrMsgIdAndTxt("optifax:OPOmexCavity:size", "numel(Et) must be N*B");
@@ -136,6 +135,12 @@ extern "C" void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* 
   plhs[2] = mxCreateNumericArray(3, dL, mxDOUBLE_CLASS, mxREAL);
   mwSize dS[2] = {1, (mwSize)B};
   plhs[3] = mxCreateNumericArray(2, dS, mxDOUBLE_CLASS, mxREAL);
+  // optional 5th / 6th outputs: the run-averaged output-coupled and
+  // crystal-output |E(w)|^2 (N x B, FFT order), ESDOutAverage and the sum
+  // behind ESDPumpDepAverage / ESDIdlerICAverage (OpticalSim.m:302-313)
+  mwSize dA[2] = {(mwSize)N, (mwSize)B};
+  if (nlhs > 4) plhs[4] = mxCreateNumericArray(2, dA, mxDOUBLE_CLASS, mxREAL);
+  if (nlhs > 5) plhs[5] = mxCreateNumericArray(2, dA, mxDOUBLE_CLASS, mxREAL);
   std::vector<double> f(2 * N), oc(2 * N);
   for (int b = 0; b < B; b++) {
     int s_b = 0;
@@ -148,6 +153,9 @@ extern "C" void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* 
       po[j] = (float)oc[j];
     }
     mxGetDoubles(plhs[3])[b] = s_b;
+    if (nlhs > 4)
+      ofx_opo_read_average(run, b, mxGetDoubles(plhs[4]) + N * b,
+                           nlhs > 5 ? mxGetDoubles(plhs[5]) + N * b : nullptr, nullptr);
   }
   ofx_opo_destroy(run);
 }

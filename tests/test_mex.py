@@ -88,7 +88,7 @@ def test_opomexcavity_matches_device_batch(gpu_ctx):
          si["beta_op"].astype(np.float32), np.float32(si["maxErr"]), np.float32(si["minErr"]),
          np.float64(si["sel"]), ops["pump"], ops["cav"], ops["ocm"], np.float64(cfg.delay),
          np.float64(trips), np.float64(0.0)]
-    Et, oc, log, status = mexshim.call("OPOmexCavity", *a, nargout=4)
+    Et, oc, log, status, esd_oc, esd_x = mexshim.call("OPOmexCavity", *a, nargout=6)
     f32 = lambda x: np.asarray(x, np.float32).astype(np.float64)
     s32 = dict(si, beta_op=f32(si["beta_op"]), alpha_w=f32(si["alpha_w"]), G33=f32(si["G33"]),
                w0=float(f32(si["w0"])), h=float(f32(si["h"])), maxErr=float(f32(si["maxErr"])),
@@ -101,5 +101,7 @@ def test_opomexcavity_matches_device_batch(gpu_ctx):
     assert status[0, 0] == 1
     assert np.array_equal(log[2:, :, 0].T, r["trip_log"][:, 2:])
     assert rel_l2(Et[:, 0], r["field"]) < 1e-6
+    e, x, k = run.read_average(0)
+    assert k == trips and rel_l2(esd_oc[:, 0], e) < 1e-6 and rel_l2(esd_x[:, 0], x) < 1e-6
     run.close()
     mexshim.clear()
