@@ -3,9 +3,9 @@
 // OpticalSim.run (toolbox/lib/classes/OpticalSim.m:230-288) and its per-trip
 // OPOmexBatch call.
 //
-//   [Et, ocSpec, tripLog, status] = OPOmexCavity(Et, alpha_w, G, w0, bdiffw0, h,
-//       NSteps, dt, beta_op, maxErr, minErr, sel, pumpSpec, cavMask, ocMask,
-//       delay, trips, tol)
+//   [Et, ocSpec, tripLog, status, esdOut, esdX] = OPOmexCavity(Et, alpha_w, G, w0,
+//       bdiffw0, h, NSteps, dt, beta_op, maxErr, minErr, sel, pumpSpec, cavMask,
+//       ocMask, delay, trips, tol)
 //
 //   Et       N x B  crystal input of the first trip (fftshifted time); returns
 //                   the crystal input of the next trip
@@ -17,6 +17,8 @@
 //                   change of its output power spectrum falls below tol
 //   ocSpec   N x B  last output-coupled spectrum;  tripLog 4 x trips x B
 //                   (conv metric, output energy, attempts, accepted steps)
+//   esdOut, esdX    (optional) N x B means over the trips of the output-coupled
+//                   and the crystal-output |E(w)|^2, FFT order
 #ifdef OFX_MX_SHIM
 #include "../../include/optifax_mex.h"
 #else
