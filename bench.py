@@ -80,17 +80,17 @@ def batch_workload(n_inst=256, seed=3):
     return insts, meta
 
 
-def warmstart_configs(n_inst=32, seed=5):
-    """Config 5 at a reduced grid: cavity detunings (+50 .. +110 fs from
-    synchronism) of an 8 mm fan-out PPLN (local period 29-31 um, one position
-    per instance), 4 W at 1 GHz of 1 ps sech^2 pump in pi (50 um)^2, 16 ps
-    window on 2^14 points, Heun with fixed 0.4 um steps (error control off:
-    maxErr = inf, minErr = 0)."""
+def warmstart_configs(n_inst=32, seed=5, n=2 ** 14):
+    """Config 5: cavity detunings (+50 .. +110 fs from synchronism) of an
+    8 mm fan-out PPLN (local period 29-31 um, one position per instance),
+    4 W at 1 GHz of 1 ps sech^2 pump in pi (50 um)^2, 16 ps window on n
+    points (2^14 by default; config 5's own grid is 2^16), Heun with fixed
+    0.4 um steps (error control off: maxErr = inf, minErr = 0)."""
     from optifax_amd import physics as ph
     rng = np.random.default_rng(seed)
     periods = rng.permutation(np.linspace(29e-6, 31e-6, n_inst))
     dets = np.linspace(50e-15, 110e-15, n_inst)
-    return [ph.OPOConfig(n=2 ** 14, window=16e-12, length=8e-3, period=float(p),
+    return [ph.OPOConfig(n=n, window=16e-12, length=8e-3, period=float(p),
                          pump_energy=4e-9, pump_fwhm=1e-12, area=np.pi * (50e-6) ** 2,
                          delay=float(d), delay_mode="sync", error_bounds=(0.0, 1e30),
                          noise_seed=1000 + i)
@@ -117,13 +117,14 @@ def main_warmstart(args):
     data, training, surrogate trips, gated plain trips); "all" runs both."""
     from optifax_amd import warmstart as ws
     log = lambda m: print(m, file=sys.stderr, flush=True)
-    cfgs = warmstart_configs(args.ws_instances)
+    cfgs = warmstart_configs(args.ws_instances, n=args.ws_grid)
     pairs = [ws.opo_instance(c) for c in cfgs]
     insts = [p[0] for p in pairs]
     builds = [p[1] for p in pairs]
     log(f"warmstart: {len(insts)} instances, phase {args.ws_phase}")
     cfg_out = {
-        "workload": f"config 5 at 2^14 points: {len(insts)} detunings +50..+110 fs, 8 mm fan-out PPLN "
+        "workload": f"config 5 at 2^{int(np.log2(args.ws_grid))} points: {len(insts)} detunings "
+                    "+50..+110 fs, 8 mm fan-out PPLN "
                     "29-31 um, 4 W @ 1 GHz 1 ps sech^2 in pi (50 um)^2, 5% OC, fixed 0.4 um "
                     "steps; steady state = change between trips < 1e-4, budget 1000 trips",
         "instances": len(insts),
@@ -481,6 +482,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ws-instances", type=int, default=32)
     ap.add_argument("--ws-train-instances", type=int, default=32)
+    ap.add_argument("--ws-grid", type=int, default=2 ** 14,
+                    help="grid points of the warmstart workload's instances (config 5: 65536)")
     ap.add_argument("--ws-max-seconds", type=float, default=900.0,
                     help="stop each solve after this many seconds (results so far are reported)")
     ap.add_argument("--ws-train-trips", type=int, default=250)
