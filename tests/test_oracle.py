@@ -292,3 +292,25 @@ def test_poling_from_equal_widths_is_the_uniform_grating():
     assert np.array_equal(P, oracle.qpm_poling(ns, L, period))
     # a single wide domain longer than L is cut at L: all -1
     assert np.all(oracle.qpm_poling_widths(100, L, [2 * L]) == -1.0)
+
+
+def test_fft_against_committed_golden_vectors():
+    """The oracle's FFT at the configs' grid sizes against DFT values written
+    once by tests/golden/make_golden.py (numpy.fft), to 1e-12 of the norm."""
+    import json
+    import os
+    import sys
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    sys.path.insert(0, here)
+    try:
+        from make_golden import golden_input
+    finally:
+        sys.path.remove(here)
+    with open(os.path.join(here, "fft_golden.json")) as f:
+        cases = json.load(f)["cases"]
+    assert [c["n"] for c in cases] == [1 << 13, 1 << 14, 1 << 15, 1 << 16]
+    for c in cases:
+        X = oracle.fft(golden_input(c["n"], c["seed"]))
+        ref = np.array(c["re"]) + 1j * np.array(c["im"])
+        assert np.max(np.abs(X[c["bins"]] - ref)) < 1e-12 * c["norm"]
+        assert abs(np.linalg.norm(X) - c["norm"]) < 1e-12 * c["norm"]
