@@ -501,10 +501,11 @@ def cpu_baseline_batch(insts, meta, attempts_per_trip, sample_steps=3000):
         cores = min(16, os.cpu_count() or 1)
     sizes = sorted({n for n, _ in meta})
     order = sorted(range(len(insts)), key=lambda i: (sizes.index(meta[i][0]), i))
-    pick = [order[(k * len(order)) // cores] for k in range(cores)]
-    for n in sizes:                                   # every grid size at least once
-        if not any(meta[i][0] == n for i in pick):
-            pick[-1 - sizes.index(n)] = next(i for i in order if meta[i][0] == n)
+    # every grid size once, then the other cores spread over the batch
+    pick = [next(i for i in order if meta[i][0] == n) for n in sizes]
+    rest = [i for i in order if i not in pick]
+    extra = max(cores - len(pick), 0) if rest else 0
+    pick += [rest[(k * len(rest)) // extra] for k in range(extra)]
 
     def one(i):
         si = insts[i]["solver"]

@@ -35,3 +35,14 @@ def test_batch_cpu_baseline_runs_one_instance_per_core():
     assert r["kind"] == "port" and r["cores"] >= 1
     assert r["value"] > 0 and r["single_core_value"] > 0
     assert "2^8" in r["sample"] and "2^9" in r["sample"]
+
+
+def test_batch_cpu_baseline_on_fewer_cores_than_grid_sizes(monkeypatch):
+    insts, meta = [], []
+    for n in (256, 512, 1024):
+        cfg, b = opo_case(n=n, window=2e-12 * n / 256, length=0.05e-3)
+        insts.append(dict(solver=b["solver"], field0=b["ops"]["field0"]))
+        meta.append((n, cfg.length))
+    monkeypatch.setattr(bench.os, "sched_getaffinity", lambda pid: {0}, raising=False)
+    r = bench.cpu_baseline_batch(insts, meta, [50.0, 50.0, 50.0], sample_steps=20)
+    assert r["cores"] == 1 and r["value"] > 0
