@@ -121,9 +121,45 @@ def ktp_z_n(lam_um, T=20.0):
     return n0 + n1 * dT + n2 * dT ** 2
 
 
-def sellmeier_n(lam_um, material="PPLN", T=20.0):
-    """Crystal materials of sellmeier.m:3-21 (real part).  The glass table
-    (Schott / CDGM forms of Materials.csv) serves cavity optics, not crystals."""
+def load_material_table(path):
+    """A material table in the layout of the reference's Materials.csv
+    (sellmeier.m:22-51): columns Material, six coefficients, Equivalent,
+    Equation ("Schott" or "CDGM"), ...  Returns {name: (equation, coeffs)}."""
+    import csv
+    out = {}
+    with open(path, newline="") as f:
+        for row in csv.DictReader(f):
+            name = (row.get("Material") or "").strip()
+            eq = (row.get("Equation") or "").strip()
+            if not name or eq not in ("Schott", "CDGM"):
+                continue
+            keys = ["B1", "B2", "B3", "C1", "C2", "C3"]
+            out[name] = (eq, [float(row[k]) for k in keys])
+    return out
+
+
+def glass_n(lam_um, equation, coeffs):
+    """sellmeier.m:27-51: the Schott form n^2 = 1 + sum B_i l^2 / (l^2 - C_i)
+    (coefficients B1..B3, C1..C3, C in um^2) or the CDGM form
+    n^2 = A0 + A1 l^2 + A2 l^-2 + A3 l^-4 + A4 l^-6 + A5 l^-8 (A0..A5 in the
+    same six columns); real part of the square root."""
+    x = np.asarray(lam_um, dtype=np.float64)
+    x2 = x * x
+    if equation == "Schott":
+        B1, B2, B3, C1, C2, C3 = coeffs
+        nf = 1 + B1 * x2 / (x2 - C1) + B2 * x2 / (x2 - C2) + B3 * x2 / (x2 - C3)
+    elif equation == "CDGM":
+        A0, A1, A2, A3, A4, A5 = coeffs
+        nf = A0 + A1 * x2 + A2 / x2 + A3 / x2 ** 2 + A4 / x2 ** 3 + A5 / x2 ** 4
+    else:
+        raise ValueError(f"unknown equation {equation!r}")
+    return _re_sqrt(nf)
+
+
+def sellmeier_n(lam_um, material="PPLN", T=20.0, table=None):
+    """sellmeier.m:3-56 (real part): the crystal materials by name, any other
+    material from `table` (load_material_table of a Materials.csv-layout
+    file; glasses for cavity optics, no temperature dependence)."""
     if material in ("PPLN", "LN_e"):
         return gayer_n(lam_um, T, 1)
     if material == "LN_o":
@@ -134,10 +170,12 @@ def sellmeier_n(lam_um, material="PPLN", T=20.0):
         return opgap_wei_n(lam_um, T)
     if material in ("KTP", "PPKTP"):
         return ktp_z_n(lam_um, T)
-    raise ValueError(f"unknown crystal material {material!r}")
+    if table is not None and material in table:
+        return glass_n(lam_um, *table[material])
+    raise ValueError(f"unknown material {material!r} (glasses need a material table)")
 
 
-def phi_calc(L, T, simwin: SimWindow, material="PPLN"):
+def phi_calc(L, T, simwin: SimWindow, material="PPLN", table=None):
     """phi_calc.m:5-45: (phi_rel, phi_abs).  The string "PPLN" takes the
     Gayer index at T; every other material goes through sellmeier() at its
     default 20 C (phi_calc.m:17-20 passes no temperature there).
@@ -149,7 +187,7 @@ def phi_calc(L, T, simwin: SimWindow, material="PPLN"):
     w_abs, w_rel = simwin.omegas, simwin.rel_omegas
     lam = 2 * np.pi * C0 / w_abs
     n_mat = gayer_n(lam * 1e6, T, 1) if material == "PPLN" else \
-        sellmeier_n(lam * 1e6, material, 20.0)
+        sellmeier_n(lam * 1e6, material, 20.0, table)
     beta_abs = w_abs * n_mat / C0
     phi_abs = L * beta_abs
     gd_abs = np.concatenate([[0.0], np.diff(phi_abs) / np.diff(w_abs)])
@@ -165,6 +203,14 @@ def phi_calc(L, T, simwin: SimWindow, material="PPLN"):
     phi = np.full(simwin.n, np.nan)
     phi[idx] = acc
     return phi - phi[ref], phi_abs
+
+
+def bulk_dispersion(simwin: SimWindow, material, path_length, table=None, T=20.0):
+    """Dielectric.Dispersion (Dielectric.m:106-114): the relative spectral
+    phase of a bulk optic, phi_calc of its path length (NaN outside the
+    0.2-20 um knot band).  Its factor exp(-i phi) multiplies a cavity's
+    spectral transfer where the optic sits in the chain."""
+    return phi_calc(path_length, T, simwin, material, table)[0]
 
 
 def phi2gd(phi, dw):

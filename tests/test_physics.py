@@ -142,3 +142,44 @@ def test_domain_width_uncertainty():
     assert set(np.unique(a)) == {-1.0, 1.0}
     assert abs(np.count_nonzero(np.diff(a)) - np.count_nonzero(np.diff(u))) <= 1
     assert np.count_nonzero(a != u) < 0.1 * ns      # walls move by ~0.5 um of 15 um domains
+
+
+def test_glass_table_forms_against_published_indices(tmp_path):
+    """Both equation forms of a Materials.csv-layout table (sellmeier.m:22-51)
+    against published catalogue indices: N-BK7 (Schott form, Schott
+    catalogue coefficients) n_d = 1.5168 at 587.56 nm and 1.5066 at 1064 nm;
+    fused silica (Malitson 1965, Schott form) 1.4585 and 1.4496; BK7 in the
+    Laurent form of the older Schott catalogue (the CDGM layout) n_d = 1.5168."""
+    p = tmp_path / "Materials.csv"
+    p.write_text(
+        "Material,B1,B2,B3,C1,C2,C3,Equivalent,Equation,T_file\n"
+        "N-BK7,1.03961212,0.231792344,1.01046945,0.00600069867,0.0200179144,103.560653,,Schott,x\n"
+        "SiO2,0.6961663,0.4079426,0.8974794,0.004679148,0.01351206,97.93400,,Schott,x\n"
+        "BK7L,2.2718929,-0.010108077,0.010592509,0.00020816965,-7.6472538e-06,4.9240991e-07,,CDGM,x\n"
+        "skip,1,1,1,1,1,1,,Other,x\n")
+    t = ph.load_material_table(str(p))
+    assert set(t) == {"N-BK7", "SiO2", "BK7L"}
+    n = ph.sellmeier_n([0.5875618, 1.064], "N-BK7", table=t)
+    assert abs(n[0] - 1.5168) < 1e-4 and abs(n[1] - 1.5066) < 2e-4
+    n = ph.sellmeier_n([0.5875618, 1.064], "SiO2", table=t)
+    assert abs(n[0] - 1.4585) < 1e-4 and abs(n[1] - 1.4496) < 2e-4
+    assert abs(ph.sellmeier_n(0.5875618, "BK7L", table=t) - 1.5168) < 2e-4
+    assert abs(ph.sellmeier_n(1.0, "BK7L", table=t) - ph.sellmeier_n(1.0, "N-BK7", table=t)) < 5e-4
+    with np.testing.assert_raises(ValueError):
+        ph.sellmeier_n(1.0, "SF10", table=t)
+
+
+def test_glass_bulk_dispersion_gdd(tmp_path):
+    """Fused silica's published GVD near 1040 nm is ~18.9 fs^2/mm (Malitson
+    index): 10 mm of it gives ~189 fs^2 as the curvature of phi_rel at w0,
+    with no group delay left there."""
+    p = tmp_path / "Materials.csv"
+    p.write_text("Material,B1,B2,B3,C1,C2,C3,Equivalent,Equation\n"
+                 "SiO2,0.6961663,0.4079426,0.8974794,0.004679148,0.01351206,97.93400,,Schott\n")
+    t = ph.load_material_table(str(p))
+    sw = ph.SimWindow(1040e-9, 4096, 10e-12)
+    phi = ph.bulk_dispersion(sw, "SiO2", 10e-3, table=t)
+    i, dw = sw.reference_index, sw.d_omega
+    gdd = (phi[i + 1] - 2 * phi[i] + phi[i - 1]) / dw ** 2
+    assert abs(gdd / 189e-30 - 1) < 0.05
+    assert abs((phi[i + 1] - phi[i - 1]) / (2 * dw)) < 1e-3 * 10e-3 / ph.C0
