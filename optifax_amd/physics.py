@@ -213,6 +213,54 @@ def bulk_dispersion(simwin: SimWindow, material, path_length, table=None, T=20.0
     return phi_calc(path_length, T, simwin, material, table)[0]
 
 
+def fresnel_t(n1, n2, theta_deg=0.0, exit=False):
+    """fresnel.m:1-38: power transmission (Tp, Ts) of a plane interface from
+    index n1 into n2 at incidence theta (degrees); with exit=True the same
+    surface crossed the other way (the angles and the index ratio swap).
+    Real part, as the reference takes it (no transmission past the
+    critical angle)."""
+    n1 = np.asarray(n1, dtype=np.complex128)
+    n2 = np.asarray(n2, dtype=np.complex128)
+    ti = np.deg2rad(theta_deg) + 0j
+    nr = n1 / n2
+    tt = np.arcsin(nr * np.sin(ti))
+    if exit:
+        ti, tt, nr = tt, ti, 1.0 / nr
+    ts = 2 * nr * np.cos(ti) / (nr * np.cos(ti) + np.cos(tt))
+    tp = 2 * nr * np.cos(ti) / (nr * np.cos(tt) + np.cos(ti))
+    geo = (1.0 / nr) * (np.cos(tt) / np.cos(ti))
+    return (tp ** 2 * geo).real, (ts ** 2 * geo).real
+
+
+def surface_transmission(coating, simwin: SimWindow, material=None, theta_deg=0.0, order=1,
+                         table=None, T=20.0):
+    """OpticalSurface.Transmission (OpticalSurface.m:45-71) for the coating
+    forms that need no data file: a number (percent if > 1), a table of
+    [T, upper wavelength limit] rows, a callable of the wavelengths, "AR"
+    (1), or "None" (the bare surface: fresnel's p transmission from air into
+    the material, or out of it for the second surface, order 2)."""
+    lam = simwin.wavelengths
+    if isinstance(coating, str):
+        if coating == "AR":
+            return np.ones(lam.size)
+        if coating == "None":
+            n = sellmeier_n(lam * 1e6, material, T, table)
+            return fresnel_t(1.0, n, theta_deg, exit=(order == 2))[0]
+        raise ValueError(f"coating {coating!r} needs a transmission data file")
+    if callable(coating):
+        return np.asarray(coating(lam), dtype=np.float64)
+    c = np.asarray(coating, dtype=np.float64)
+    if c.ndim == 0:
+        t = np.full(lam.size, float(c))
+    else:
+        vals, lims = c[:, 0], c[:, 1]
+        t = np.ones(lam.size)
+        t[lam < lims[0]] = vals[0]
+        for k in range(1, len(vals)):
+            t[(lam > lims[k - 1]) & (lam < lims[k])] = vals[k]
+    return t / 100.0 if np.any(t > 1) else t
+
+
 def phi2gd(phi, dw):
     """phi2GD.m:1-17 (GD only)."""
     n = phi.size

@@ -183,3 +183,34 @@ def test_glass_bulk_dispersion_gdd(tmp_path):
     gdd = (phi[i + 1] - 2 * phi[i] + phi[i - 1]) / dw ** 2
     assert abs(gdd / 189e-30 - 1) < 0.05
     assert abs((phi[i + 1] - phi[i - 1]) / (2 * dw)) < 1e-3 * 10e-3 / ph.C0
+
+
+def test_fresnel_textbook_cases():
+    """Normal incidence into n = 1.5: 1 - ((n-1)/(n+1))^2 = 0.96 for both
+    polarisations; p at Brewster's angle: 1; s against 1 - r_s^2 of the
+    textbook reflection coefficient; the exit surface at the refracted angle
+    transmits what the entry surface does."""
+    tp, ts = ph.fresnel_t(1.0, 1.5, 0.0)
+    assert abs(tp - 0.96) < 1e-12 and abs(ts - 0.96) < 1e-12
+    tb = np.rad2deg(np.arctan(1.5))
+    assert abs(ph.fresnel_t(1.0, 1.5, tb)[0] - 1.0) < 1e-12
+    th = 40.0
+    ti, tt = np.deg2rad(th), np.arcsin(np.sin(np.deg2rad(th)) / 1.5)
+    rs = (np.cos(ti) - 1.5 * np.cos(tt)) / (np.cos(ti) + 1.5 * np.cos(tt))
+    assert abs(ph.fresnel_t(1.0, 1.5, th)[1] - (1 - rs ** 2)) < 1e-12
+    a = ph.fresnel_t(1.0, 1.5, th)
+    b = ph.fresnel_t(1.0, 1.5, th, exit=True)
+    assert abs(a[0] - b[0]) < 1e-12 and abs(a[1] - b[1]) < 1e-12
+
+
+def test_surface_transmission_forms():
+    sw = ph.SimWindow(1040e-9, 1024, 4e-12)
+    assert np.all(ph.surface_transmission("AR", sw) == 1.0)
+    assert np.allclose(ph.surface_transmission(95, sw), 0.95)          # percent syntax
+    lam = sw.wavelengths
+    t = ph.surface_transmission(np.array([[0.2, 1.0e-6], [0.9, 1.2e-6], [0.5, 9e-6]]), sw)
+    assert np.all(t[lam < 1.0e-6] == 0.2) and np.all(t[(lam > 1.0e-6) & (lam < 1.2e-6)] == 0.9)
+    bare = ph.surface_transmission("None", sw, material="PPLN")
+    n = ph.sellmeier_n(lam * 1e6, "PPLN")
+    band = (lam > 0.4e-6) & (lam < 4e-6)
+    assert np.allclose(bare[band], (1 - ((n - 1) / (n + 1)) ** 2)[band], rtol=1e-12)
