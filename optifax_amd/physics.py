@@ -261,6 +261,56 @@ def surface_transmission(coating, simwin: SimWindow, material=None, theta_deg=0.
     return t / 100.0 if np.any(t > 1) else t
 
 
+def _movmean(x, k):
+    """MATLAB movmean(x, k) for odd k with its default shrinking endpoints."""
+    h = k // 2
+    c = np.concatenate([[0.0], np.cumsum(x)])
+    lo = np.maximum(np.arange(x.size) - h, 0)
+    hi = np.minimum(np.arange(x.size) + h + 1, x.size)
+    return (c[hi] - c[lo]) / (hi - lo)
+
+
+def load_gdd(path):
+    """A GDD data file as GDDimport2phi reads it: a table with columns
+    Wavelength [nm] and GDD [fs^2]."""
+    import csv
+    with open(path, newline="") as f:
+        rows = list(csv.DictReader(f))
+    return (np.array([float(r["Wavelength"]) for r in rows]),
+            np.array([float(r["GDD"]) for r in rows]))
+
+
+def gdd_to_phi(simwin: SimWindow, wavelength_nm, gdd_fs2):
+    """GDDimport2phi.m:1-33: relative spectral phase of a measured GDD curve
+    (a mirror coating).  The GDD is smoothed over 5 samples, spline
+    interpolated onto the grid's frequencies (held at the end values outside
+    the data), integrated once to the group delay (relative to the reference
+    frequency) and once more to the phase.  NaN below 200 nm, where the
+    reference's spline extrapolates."""
+    from scipy.interpolate import CubicSpline
+    w = 2e9 * np.pi * C0 / np.asarray(wavelength_nm, dtype=np.float64)
+    gdd = _movmean(np.asarray(gdd_fs2, dtype=np.float64) / 1e30, 5)
+    order = np.argsort(w)
+    spline = CubicSpline(w[order], gdd[order], bc_type="not-a-knot")
+    w_abs, w_rel = simwin.omegas, simwin.rel_omegas
+    g = spline(w_abs)
+    g[w_abs > w.max()] = gdd[0]
+    g[w_abs < w.min()] = gdd[-1]
+    dw = np.diff(w_abs)
+    dw = np.concatenate([[dw[0]], dw])
+    y = g * dw
+    gd = np.concatenate([[0.0], np.cumsum(0.5 * (y[1:] + y[:-1]))])
+    ref = simwin.reference_index
+    gd_rel = gd - gd[ref]
+    dphi = gd_rel * np.concatenate([[0.0], np.diff(w_rel)])
+    lam = 2 * np.pi * C0 / w_abs
+    idx = np.nonzero(lam > 2e-7)[0]
+    phi = np.full(simwin.n, np.nan)
+    yy = dphi[idx]
+    phi[idx] = np.concatenate([[0.0], np.cumsum(0.5 * (yy[1:] + yy[:-1]))])
+    return phi - phi[ref]
+
+
 def phi2gd(phi, dw):
     """phi2GD.m:1-17 (GD only)."""
     n = phi.size

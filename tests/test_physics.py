@@ -214,3 +214,24 @@ def test_surface_transmission_forms():
     n = ph.sellmeier_n(lam * 1e6, "PPLN")
     band = (lam > 0.4e-6) & (lam < 4e-6)
     assert np.allclose(bare[band], (1 - ((n - 1) / (n + 1)) ** 2)[band], rtol=1e-12)
+
+
+def test_gdd_import_constant_gdd_is_a_parabola(tmp_path):
+    """A flat GDD curve D gives phi_rel = D (w - w0)^2 / 2 on the grid (the
+    two trapezoid integrations are exact for a constant and a linear
+    integrand) except at the first bin, where the [0 diff(w_rel)] padding of
+    GDDimport2phi.m:26 drops a half trapezoid; movmean matches MATLAB's
+    shrinking endpoints."""
+    p = tmp_path / "mirror.csv"
+    lam = np.linspace(500.0, 2500.0, 81)
+    p.write_text("Wavelength,GDD\n" + "".join(f"{l},{-150.0}\n" for l in lam))
+    wl, gdd = ph.load_gdd(str(p))
+    sw = ph.SimWindow(1040e-9, 2048, 8e-12)
+    phi = ph.gdd_to_phi(sw, wl, gdd)
+    x = sw.omegas - sw.reference_omega
+    ok = np.isfinite(phi)
+    ok[0] = False
+    assert ok[sw.reference_index] and ok.sum() > 1000
+    assert np.allclose(phi[ok], 0.5 * (-150e-30) * x[ok] ** 2, rtol=1e-9, atol=1e-12)
+    assert np.allclose(ph._movmean(np.array([1.0, 2, 3, 4, 5, 6]), 5),
+                       [2.0, 2.5, 3.0, 4.0, 4.5, 5.0])
