@@ -311,6 +311,35 @@ def gdd_to_phi(simwin: SimWindow, wavelength_nm, gdd_fs2):
     return phi - phi[ref]
 
 
+def optic_response(simwin: SimWindow, regime="T", s1="None", material="FS", length=1e-3,
+                   theta_deg=0.0, s2=None, table=None, T=20.0, gdd=None):
+    """Optic.simulate (Optic.m:79-99) for a lossless bulk: (transmission,
+    reflection, dispersion phase).  Transmission = S1 * S2 surface
+    transmissions (surface_transmission; s2 defaults to s1, the second
+    surface crossed on the way out), reflection = 1 - S1's; the phase is
+    S1's (a measured GDD curve `gdd` = (wavelength_nm, gdd_fs2), else 0)
+    plus, in transmission, the bulk's relative phase over `length`."""
+    s2 = s1 if s2 is None else s2
+    t1 = surface_transmission(s1, simwin, material, theta_deg, 1, table, T)
+    t2 = surface_transmission(s2, simwin, material, theta_deg, 2, table, T)
+    phi = np.zeros(simwin.n) if gdd is None else gdd_to_phi(simwin, *gdd)
+    if regime == "T":
+        phi = phi + bulk_dispersion(simwin, material, length, table, T)
+    return t1 * t2, 1.0 - t1, phi
+
+
+def cavity_response(optics):
+    """Cavity.simulate (Cavity.m:50-87): the power transmission of a chain
+    of optic_response tuples (the reflection of "R" optics, the
+    transmission of the others; given as (regime, response)) and the sum of
+    their phases."""
+    t, phi = 1.0, 0.0
+    for regime, (tr, rf, ph_) in optics:
+        t = t * (rf if regime == "R" else tr)
+        phi = phi + ph_
+    return t, phi
+
+
 def phi2gd(phi, dw):
     """phi2GD.m:1-17 (GD only)."""
     n = phi.size
@@ -557,7 +586,8 @@ class OutputCoupler:
         return R, T, ph
 
 
-def cavity_operators(simwin: SimWindow, n_xtal, oc: OutputCoupler, E_pump, E_cav0, n_air=None):
+def cavity_operators(simwin: SimWindow, n_xtal, oc: OutputCoupler, E_pump, E_cav0, n_air=None,
+                     chain_before_oc=None, chain_after_oc=None):
     """Crystal-frame spectral operators of one round trip, FFT order.
 
     The reference trip (OpticalSim.m:230-288) is
@@ -569,8 +599,11 @@ def cavity_operators(simwin: SimWindow, n_xtal, oc: OutputCoupler, E_pump, E_cav
                   = cav * e^{-i delay dw} * X + pump
       output      = sqrt(T) e^{-i phi} r_out X = ocm * X
     with r_in = sqrt(n_air/n_xtal) = 1/r_out (OpticalPulse.m:190-197).
-    The delay factor is applied on the device (FP64 phase).  Also returns the
-    crystal input of the first trip (fftshifted time)."""
+    Other optics (cavity_response of a chain: power transmission, phase)
+    multiply by sqrt(T) e^{-i phi}: those before the coupler both paths,
+    those after it the fed-back path only.  The delay factor is applied on
+    the device (FP64 phase).  Also returns the crystal input of the first
+    trip (fftshifted time)."""
     if n_air is None:
         n_air = np.ones(simwin.n)
     R, T, ph = oc.masks(simwin)
@@ -578,6 +611,13 @@ def cavity_operators(simwin: SimWindow, n_xtal, oc: OutputCoupler, E_pump, E_cav
     r_out = np.sqrt(n_xtal / n_air)
     cav = np.sqrt(R) * np.exp(-1j * ph) * r_out * r_in
     ocm = np.sqrt(T) * np.exp(-1j * ph) * r_out
+    for chain, both in ((chain_before_oc, True), (chain_after_oc, False)):
+        if chain is not None:
+            t_c, phi_c = chain
+            f = np.sqrt(t_c) * np.exp(-1j * np.nan_to_num(phi_c))
+            cav = cav * f
+            if both:
+                ocm = ocm * f
     P = r_in * SimWindow.t2k(E_pump)
     X0k = r_in * (SimWindow.t2k(E_cav0) + SimWindow.t2k(E_pump))
     X0 = SimWindow.k2t(X0k)                          # natural time order

@@ -235,3 +235,32 @@ def test_gdd_import_constant_gdd_is_a_parabola(tmp_path):
     assert np.allclose(phi[ok], 0.5 * (-150e-30) * x[ok] ** 2, rtol=1e-9, atol=1e-12)
     assert np.allclose(ph._movmean(np.array([1.0, 2, 3, 4, 5, 6]), 5),
                        [2.0, 2.5, 3.0, 4.0, 4.5, 5.0])
+
+
+def test_optic_chain_into_cavity_operators(tmp_path):
+    """Cavity.simulate's products and sums (Cavity.m:50-87): a 50 % mirror
+    used in reflection and an AR-coated 1 mm fused-silica window give a
+    chain transmission 0.5 x 1 and the window's bulk phase; placed before
+    the coupler it scales both spectral operators by sqrt(T) e^{-i phi},
+    after it only the fed-back one."""
+    p = tmp_path / "Materials.csv"
+    p.write_text("Material,B1,B2,B3,C1,C2,C3,Equivalent,Equation\n"
+                 "FS,0.6961663,0.4079426,0.8974794,0.004679148,0.01351206,97.93400,,Schott\n")
+    t = ph.load_material_table(str(p))
+    sw = ph.SimWindow(1040e-9, 1024, 4e-12)
+    mirror = ("R", ph.optic_response(sw, "R", 0.5, "FS", 6.35e-3, table=t))
+    window = ("T", ph.optic_response(sw, "T", "AR", "FS", 1e-3, table=t))
+    assert np.allclose(mirror[1][1], 0.5) and not mirror[1][2].any()
+    T_c, phi_c = ph.cavity_response([mirror, window])
+    assert np.allclose(T_c, 0.5)
+    ok = np.isfinite(phi_c)
+    assert np.allclose(phi_c[ok], ph.bulk_dispersion(sw, "FS", 1e-3, table=t)[ok])
+    n_x = np.full(sw.n, 2.1)
+    oc = ph.OutputCoupler(0.05, band=None)
+    E0 = np.zeros(sw.n, complex)
+    base = ph.cavity_operators(sw, n_x, oc, E0, E0)
+    pre = ph.cavity_operators(sw, n_x, oc, E0, E0, chain_before_oc=(T_c, phi_c))
+    post = ph.cavity_operators(sw, n_x, oc, E0, E0, chain_after_oc=(T_c, phi_c))
+    f = sw.fft_order(np.sqrt(T_c) * np.exp(-1j * np.nan_to_num(phi_c)))
+    assert np.allclose(pre["cav"], base["cav"] * f) and np.allclose(pre["ocm"], base["ocm"] * f)
+    assert np.allclose(post["cav"], base["cav"] * f) and np.allclose(post["ocm"], base["ocm"])
