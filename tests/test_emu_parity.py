@@ -307,3 +307,32 @@ def test_run_averages_and_trip_history(emu_ctx, exchange):
     with pytest.raises(OfxError):
         run.read_history(0, 2)                 # overwritten by trip 5
     run.close()
+
+
+def test_round_trips_with_an_optic_chain_match_literal_opticalsim(emu_ctx):
+    """A lossy, dispersive optic before the coupler (power transmission 0.9,
+    the phase of 1 mm of dispersive bulk) folded into the device operators
+    gives the trips of the literal sequence whose coupler masks carry that
+    optic (R T_c, T T_c, phi + phi_c)."""
+    from optifax_amd import physics as ph
+    cfg, b = opo_case()
+    si, sw = b["solver"], b["simwin"]
+    T_c = np.full(sw.n, 0.9)
+    phi_c = np.nan_to_num(ph.bulk_dispersion(sw, "LN_o", 1e-3))
+    ops = ph.cavity_operators(sw, si["n_xtal"], b["oc"], b["pump"], b["noise"],
+                              chain_before_oc=(T_c, phi_c))
+    R, T, phi = b["oc"].masks(sw)
+    trips = 3
+    o = oracle.opo_run(trips, b["noise"], b["pump"], np.ones(sw.n), si["n_xtal"], R * T_c,
+                       T * T_c, phi + phi_c, cfg.delay, sw.rel_omegas, si)
+    run = OPOBatch([dict(solver=si, field0=ops["field0"], pump=ops["pump"], cav=ops["cav"],
+                         ocm=ops["ocm"], delay=cfg.delay)], precision="double", trip_cap=trips,
+                   ctx=emu_ctx, tune={"cluster": 2})
+    run.launch(trips)
+    run.sync()
+    r = run.read(0)
+    assert np.array_equal(r["trip_log"][:, 2:].astype(int), o["steps"])
+    assert rel_l2(r["field"], oracle_next_input(b, o["E_cav"])) < 1e-10
+    from optifax_amd.physics import SimWindow
+    assert rel_l2(r["oc_spec"], SimWindow.fft_order(SimWindow.t2k(o["E_oc"]))) < 1e-10
+    run.close()
