@@ -336,3 +336,22 @@ def test_round_trips_with_an_optic_chain_match_literal_opticalsim(emu_ctx):
     from optifax_amd.physics import SimWindow
     assert rel_l2(r["oc_spec"], SimWindow.fft_order(SimWindow.t2k(o["E_oc"]))) < 1e-10
     run.close()
+
+
+def test_pass_with_ktp_and_a_chirped_grating_matches_oracle(emu_ctx):
+    """Solver inputs of another material and a chirped grating (G(z) no
+    longer periodic) through the emulated kernel: the FP64 pass reproduces
+    the oracle's step sequence and field."""
+    from optifax_amd import physics as ph
+    sw = ph.SimWindow(1040e-9, 512, 3e-12)
+    xt = ph.NonlinearCrystal(lambda z: ph.chirped_grating(z, 40e-6, 36e-6, 1.0, 0.1e-6), 0.3e-3,
+                             chi2=np.pi * 14e-12, material="KTP")
+    si = xt.solver_inputs(sw)
+    assert len(np.unique(np.diff(np.nonzero(np.diff(si["G33"]))[0]))) > 1   # walls not periodic
+    E = np.fft.fftshift(ph.sech_pump(sw, 200e-15, 2e-10, 20e-12) + ph.quantum_noise(sw, 20e-12, 3))
+    args = (si["alpha_w"], si["G33"], si["w0"], si["bdiffw0"], si["h"], si["nSteps"], si["dt"],
+            si["beta_op"], 5e-2, 2.5e-3, float(np.ceil(si["nSteps"] / 4)))
+    Eo, apo, smo, cnt = oracle.nee_heun(E, *args, n_cols=4)
+    Eg, apg, smg = _run(emu_ctx, E, args, 512, "double", 2)
+    assert np.array_equal(smg, smo)
+    assert rel_l2(Eg, Eo) < 1e-10 and rel_l2(apg, apo) < 1e-10
