@@ -150,7 +150,10 @@ int ofx_opo_graph_replay(ofx_opo* run, int trips, double conv_tol, int replays);
 int ofx_opo_last_ms(ofx_opo* run, double* ms);
 /* read back one instance (any pointer may be NULL).
  *  field [2n], oc_spec [2n], trip_log [4*trip_cap] (conv, oc energy, attempts, accepted),
- *  apft [2n*n_cols], stepmods [nsteps], counters [2] (attempts, accepted) */
+ *  apft [2n*n_cols], stepmods [nsteps] (after round trips: the last trip's
+ *  row, zero where no step ended, as StepSizeModifiers(trip,:),
+ *  OpticalSim.m:172,237), counters [2] (attempts, accepted, summed over the
+ *  run's launches) */
 int ofx_opo_read(ofx_opo* run, int inst, double* field, double* oc_spec, double* trip_log,
                  int* status, int* trips_done, long long* counters, double* apft,
                  double* stepmods);
