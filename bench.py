@@ -35,7 +35,7 @@ def algorithmic_bytes(n, accepted, rejected, trips):
 
 
 def workload():
-    """Config 2: the output coupler is Optic("R", 0.05) over the whole band
+    """The ring OPO: the output coupler is Optic("R", 0.05) over the whole band
     (intracavity sqrt(0.95), output sqrt(0.05)), the delay 0.3 ps per trip."""
     from optifax_amd import physics as ph
     cfg = ph.OPOConfig(oc_band=None)
@@ -81,10 +81,10 @@ def batch_workload(n_inst=256, seed=3):
 
 
 def warmstart_configs(n_inst=32, seed=5, n=2 ** 14):
-    """Config 5: cavity detunings (+50 .. +110 fs from synchronism) of an
+    """Detuned ring cavities: cavity detunings (+50 .. +110 fs from synchronism) of an
     8 mm fan-out PPLN (local period 29-31 um, one position per instance),
     4 W at 1 GHz of 1 ps sech^2 pump in pi (50 um)^2, 16 ps window on n
-    points (2^14 by default; config 5's own grid is 2^16), Heun with fixed
+    points (2^14 by default; the specified grid is 2^16), Heun with fixed
     0.4 um steps (error control off: maxErr = inf, minErr = 0)."""
     from optifax_amd import physics as ph
     rng = np.random.default_rng(seed)
@@ -123,7 +123,7 @@ def main_warmstart(args):
     builds = [p[1] for p in pairs]
     log(f"warmstart: {len(insts)} instances, phase {args.ws_phase}")
     cfg_out = {
-        "workload": f"config 5 at 2^{int(np.log2(args.ws_grid))} points: {len(insts)} detunings "
+        "workload": f"detuned ring OPOs at 2^{int(np.log2(args.ws_grid))} points: {len(insts)} detunings "
                     "+50..+110 fs, 8 mm fan-out PPLN "
                     "29-31 um, 4 W @ 1 GHz 1 ps sech^2 in pi (50 um)^2, 5% OC, fixed 0.4 um "
                     "steps; steady state = change between trips < 1e-4, budget 1000 trips",
@@ -220,7 +220,7 @@ def mlp_step_flops(batch, n_in_padded, width, n_out):
 
 
 def main_steady(args):
-    """Accuracy of the FP32 steady state: eight config-5 instances run to
+    """Accuracy of the FP32 steady state: eight detuned 8 mm cavities run to
     steady state (change between trips < 1e-4) with FP32 fields and with
     FP64 fields (the FP64 device mode reproduces the FP64 oracle's step
     sequence exactly and its fields to rounding); value = median relative L2
@@ -252,7 +252,7 @@ def main_steady(args):
         "dtype": "f32 vs f64",
         "data": "synthetic (sech^2 pump + seeded one-photon-per-mode noise)",
         "config": {
-            "workload": "config 5 at 2^14 points, 8 of the 32 detunings (+50..+110 fs), each run "
+            "workload": "detuned ring OPOs (8 mm fan-out PPLN) at 2^14 points, 8 of the 32 detunings (+50..+110 fs), each run "
                         "to a change between trips < 1e-4 in FP32 and in FP64",
             "instances": len(insts),
             "converged_both": int(both.sum()),
@@ -267,7 +267,7 @@ def main_steady(args):
 
 
 def main_esd(args):
-    """Config 2 as specified: 300 round trips from noise with FP32 fields and
+    """The ring OPO over 300 round trips from noise with FP32 fields and
     with FP64 fields (the FP64 device mode reproduces the FP64 oracle's step
     sequence exactly and its fields to rounding; the oracle itself needs ~55 s
     per trip here).  value = relative L2 distance of the run-averaged
@@ -338,7 +338,7 @@ def main_esd(args):
 
 
 def main_pass(args):
-    """Config 1's single crystal pass: 2^13 points, 10 ps window, 1040 nm;
+    """A single SHG/OPG crystal pass: 2^13 points, 10 ps window, 1040 nm;
     sech^2 200 fs pump in 5 um^2 with seeded one-photon noise; 30 um PPLN,
     d_eff 14 pm/V; RK23 at tolerance 1e-6 (maxErr 1e-4 %, minErr 5e-6 %);
     P = 5.  As stated (1 nJ over 1 mm) the FP64 oracle itself diverges to
@@ -385,7 +385,7 @@ def main_pass(args):
         "dtype": "f32" if args.precision == "single" else "f64",
         "data": "synthetic (sech^2 pump + seeded one-photon-per-mode noise)",
         "config": {
-            "workload": "config 1 grid: single SHG/OPG pass, N=2^13, 10 ps, 1040 nm; 0.2 mm "
+            "workload": "single SHG/OPG pass, N=2^13, 10 ps, 1040 nm; 0.2 mm "
                         "MgO:PPLN 30 um, d_eff 14 pm/V; 0.1 nJ 200 fs sech^2 in 5 um^2; RK23 "
                         "tol 1e-6; P = 5",
             "adaptive_attempts_per_solve": att,
@@ -416,7 +416,7 @@ def main_pass(args):
 
 
 def main_surrogate(args):
-    """Config 4 shape: 2050 -> 2048 x3 -> 2048 MLP, batch 512, Adam, on a
+    """The surrogate's shape: 2050 -> 2048 x3 -> 2048 MLP, batch 512, Adam, on a
     synthetic device-resident set of 8192 pairs; steps = Adam steps."""
     from optifax_amd.surrogate import SurrogateMLP
     n_in, width, n_out, batch = 2050, 2048, 2048, 512
@@ -563,7 +563,7 @@ def main():
     ap.add_argument("--ws-instances", type=int, default=32)
     ap.add_argument("--ws-train-instances", type=int, default=32)
     ap.add_argument("--ws-grid", type=int, default=2 ** 14,
-                    help="grid points of the warmstart workload's instances (config 5: 65536)")
+                    help="grid points of the warmstart workload's instances (specified: 65536)")
     ap.add_argument("--ws-max-seconds", type=float, default=900.0,
                     help="stop each solve after this many seconds (results so far are reported)")
     ap.add_argument("--ws-train-trips", type=int, default=250)
@@ -576,7 +576,7 @@ def main():
                     help="round trips of each run of the esd workload")
     ap.add_argument("--workload", default="ring",
                     choices=["ring", "batch", "warmstart", "surrogate", "steady", "esd", "pass"],
-                    help="ring: one config-2 instance (the headline line); batch: 256 uneven "
+                    help="ring: one synchronously pumped ring OPO (the headline line); batch: 256 uneven "
                          "instances in one persistent launch")
     ap.add_argument("--tune", default="heuristic", choices=["heuristic", "auto"])
     ap.add_argument("--copy-calibration", action="store_true",
@@ -696,7 +696,7 @@ def _geometry_report(run, meta, tune):
 
 
 def main_batch(args):
-    """Config 3: all instances in one persistent launch per round trip."""
+    """The uneven batch: all instances in one persistent launch per round trip."""
     from optifax_amd.solver import Context, OPOBatch
     ctx = Context(0)
     t_build = time.perf_counter()

@@ -131,7 +131,7 @@ def test_graph_replay_matches_launches(gpu_ctx):
     r2.close()
 
 
-def test_config2_full_size_invariants(gpu_ctx):
+def test_ring_opo_full_size_invariants(gpu_ctx):
     """Synchronously pumped ring OPO at its real shape (N = 2^15, 12 ps, 5 mm
     PPLN, 5% OC): FP32 and FP64 device runs of two round trips (25000 adaptive
     steps) take the identical step sequence; the output-coupled power spectrum
@@ -201,8 +201,8 @@ def test_rk23_pass_fp64_exact(gpu_ctx, n, C, window, exchange):
     assert rel_l2(apg, apo) < 1e-9
 
 
-def test_rk23_config1_grid_fp64_and_fp32(gpu_ctx):
-    """Config-1 grid and pump (2^13 points, 10 ps, 1040 nm, 200 fs sech^2,
+def test_rk23_single_pass_grid_fp64_and_fp32(gpu_ctx):
+    """The 2^13-point single-pass grid and pump (2^13 points, 10 ps, 1040 nm, 200 fs sech^2,
     d_eff 14 pm/V, 5 um^2) with its RK23 tolerance 1e-6 (maxErr 1e-4 %,
     minErr 5e-6 %) over the first 0.2 mm of the 1 mm crystal at 100 pJ
     (1 nJ in 5 um^2 drives the FP64 oracle itself to NaN)."""
@@ -215,7 +215,7 @@ def test_rk23_config1_grid_fp64_and_fp32(gpu_ctx):
     assert np.array_equal(smg, smo)
     assert rel_l2(Eg, Eo) < 1e-9
     Ef, apf, smf = _run_rk(gpu_ctx, E, args, 8192, "single")
-    print(f"RK23 config-1 grid: FP32 rel L2 {rel_l2(Ef, Eo):.2e}, steps equal "
+    print(f"RK23 2^13-point grid: FP32 rel L2 {rel_l2(Ef, Eo):.2e}, steps equal "
           f"{np.array_equal(smf, smo)}")
     # FP32 fields over ~20,000 four-stage attempts (measured 1.5e-4)
     assert np.array_equal(smf, smo)
@@ -276,8 +276,8 @@ def test_wait_limit_gives_up_and_the_next_launch_recovers(gpu_ctx, exchange):
     run.close()
 
 
-def test_config5_saturated_trip_fp64_matches_oracle(gpu_ctx):
-    """Config 5's cavity (8 mm fan-out PPLN at 29.5 um, band coupler,
+def test_long_crystal_saturated_trip_fp64_matches_oracle(gpu_ctx):
+    """The detuned long-crystal cavity (8 mm fan-out PPLN at 29.5 um, band coupler,
     synchronous delay + 60 fs, 1 ps pump, fixed 0.4 um steps) at 2^14 points:
     40 FP64 device trips from noise reach the saturated regime; one more trip
     from that field matches the FP64 oracle's trip from the same field, and the
@@ -315,7 +315,7 @@ def test_config5_saturated_trip_fp64_matches_oracle(gpu_ctx):
     f32.sync()
     e32 = rel_l2(f32.read(0)["field"], want)
     f32.close()
-    print(f"config-5 saturated trip: FP32 rel L2 vs oracle {e32:.2e}")
+    print(f"8 mm cavity saturated trip: FP32 rel L2 vs oracle {e32:.2e}")
     assert e32 < 5e-3
 
 
