@@ -1326,11 +1326,13 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
       chunk = cend;
     }
     if (status != ST_RUNNING) break;
-    if (rank == 0 && d.sm_hist && d.stepmods) {
+    // A plain pass (trips == 0) is not a round trip: it leaves the trip
+    // count, the trip log and the history ring to the round trips.
+    if (a.trips > 0 && rank == 0 && d.sm_hist && d.stepmods) {
       float* row = d.sm_hist + (size_t)((trip_base + trip) % d.hist_cap) * (size_t)nsteps;
       for (int j = OFX_TID; j < nsteps; j += OFX_NT) row[j] = d.stepmods[j];
     }
-    if (rank == 0 && d.trip_log && trip_base + trip < d.trip_cap) {
+    if (a.trips > 0 && rank == 0 && d.trip_log && trip_base + trip < d.trip_cap) {
       double* tl = d.trip_log + 4 * (size_t)(trip_base + trip);
       tl[2] = (double)trip_att;
       tl[3] = (double)trip_acc;
@@ -1423,8 +1425,10 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
   }
   if (rank == 0 && OFX_TID == 0) {
     d.status[0] = status;
-    if (trip > 0) d.fresh[0] = 0;
-    d.trips_done[0] += trip;
+    if (a.trips > 0) {
+      if (trip > 0) d.fresh[0] = 0;
+      d.trips_done[0] += trip;
+    }
     d.counters[0] += attempts;
     d.counters[1] += accepted;
   }

@@ -124,17 +124,26 @@ def ktp_z_n(lam_um, T=20.0):
 def load_material_table(path):
     """A material table in the layout of the reference's Materials.csv
     (sellmeier.m:22-51): columns Material, six coefficients, Equivalent,
-    Equation ("Schott" or "CDGM"), ...  Returns {name: (equation, coeffs)}."""
+    Equation ("Schott" or "CDGM"), ...  Returns {name: (equation, coeffs)}.
+    Rows whose six coefficient cells are not all numbers are skipped: the
+    reference's own table tags PPKTP "Schott" with blank coefficients, and
+    sellmeier() never reads that row (it dispatches PPKTP by name)."""
     import csv
     out = {}
+    keys = ["B1", "B2", "B3", "C1", "C2", "C3"]
     with open(path, newline="") as f:
         for row in csv.DictReader(f):
             name = (row.get("Material") or "").strip()
             eq = (row.get("Equation") or "").strip()
             if not name or eq not in ("Schott", "CDGM"):
                 continue
-            keys = ["B1", "B2", "B3", "C1", "C2", "C3"]
-            out[name] = (eq, [float(row[k]) for k in keys])
+            try:
+                coeffs = [float(row[k]) for k in keys]
+            except (TypeError, ValueError, KeyError):
+                continue
+            if not all(np.isfinite(coeffs)):
+                continue
+            out[name] = (eq, coeffs)
     return out
 
 
@@ -311,20 +320,50 @@ def gdd_to_phi(simwin: SimWindow, wavelength_nm, gdd_fs2):
     return phi - phi[ref]
 
 
+def _matlab_round(x):
+    """MATLAB round: halves away from zero."""
+    return int(np.sign(x) * np.floor(abs(x) + 0.5))
+
+
+def coating_phase(simwin: SimWindow, gdd, theta_deg=0.0):
+    """OpticalSurface.Dispersion (OpticalSurface.m:96-114): the relative
+    phase of a coating's measured GDD curve `gdd` = (wavelength_nm, gdd_fs2)
+    (gdd_to_phi), zero without one.  At oblique incidence the curve moves to
+    shorter wavelengths: circshift by -round(0.2 lam0 sin^2(theta) / dlam)
+    samples, dlam the (signed) wavelength spacing at the reference index."""
+    if gdd is None:
+        return np.zeros(simwin.n)
+    phi = gdd_to_phi(simwin, *gdd)
+    if theta_deg != 0:
+        lam = simwin.wavelengths
+        ref = simwin.reference_index
+        dlam = (lam[ref + 1] - lam[ref - 1]) / 2
+        shift = lam[ref] * 0.2 * np.sin(np.deg2rad(theta_deg)) ** 2
+        phi = np.roll(phi, -_matlab_round(shift / dlam))
+    return phi
+
+
 def optic_response(simwin: SimWindow, regime="T", s1="None", material="FS", length=1e-3,
-                   theta_deg=0.0, s2=None, table=None, T=20.0, gdd=None):
+                   theta_deg=0.0, s2=None, table=None, T=20.0, gdd=None, gdd2="s1"):
     """Optic.simulate (Optic.m:79-99) for a lossless bulk: (transmission,
     reflection, dispersion phase).  Transmission = S1 * S2 surface
     transmissions (surface_transmission; s2 defaults to s1, the second
-    surface crossed on the way out), reflection = 1 - S1's; the phase is
-    S1's (a measured GDD curve `gdd` = (wavelength_nm, gdd_fs2), else 0)
-    plus, in transmission, the bulk's relative phase over `length`."""
+    surface crossed on the way out), reflection = 1 - S1's.  The phase is
+    S1's coating phase (coating_phase of a measured GDD curve `gdd` =
+    (wavelength_nm, gdd_fs2), else 0) and, in transmission, also the bulk's
+    relative phase over `length` and S2's coating phase.  S2 carries the
+    same curve as S1 unless `gdd2` names another (None: no curve): in the
+    reference the second surface defaults to a copy of the first, GDD
+    included (Optic.m:57,72-75).  `T` is the bulk's temperature; the bare
+    surfaces take fresnel's default 20 C index (fresnel.m:10-16)."""
     s2 = s1 if s2 is None else s2
-    t1 = surface_transmission(s1, simwin, material, theta_deg, 1, table, T)
-    t2 = surface_transmission(s2, simwin, material, theta_deg, 2, table, T)
-    phi = np.zeros(simwin.n) if gdd is None else gdd_to_phi(simwin, *gdd)
+    t1 = surface_transmission(s1, simwin, material, theta_deg, 1, table, 20.0)
+    t2 = surface_transmission(s2, simwin, material, theta_deg, 2, table, 20.0)
+    phi = coating_phase(simwin, gdd, theta_deg)
     if regime == "T":
-        phi = phi + bulk_dispersion(simwin, material, length, table, T)
+        g2 = gdd if (isinstance(gdd2, str) and gdd2 == "s1") else gdd2
+        phi = phi + bulk_dispersion(simwin, material, length, table, T) + \
+            coating_phase(simwin, g2, theta_deg)
     return t1 * t2, 1.0 - t1, phi
 
 

@@ -15,9 +15,7 @@ energy and poling period, and replaces it for the first trips of a solve:
     1. surrogate trips: iterate the bin-level map from a weak smooth seed
        through the build-up, while the signal still grows by 2 % per trip
        and stays within the input norms seen in training (at most
-       `max_surrogate_trips`); one batched inference per trip.  (Anderson
-       mixing to the surrogate's fixed point is available as accel=
-       "anderson"; measured, it converged to the trivial zero-signal root);
+       `max_surrogate_trips`); one batched inference per trip;
     2. handover: the bins become a full-grid field (band-limited
        interpolation) plus the exact pump;
     3. gate: the first plain trip's band output is compared with the
@@ -348,68 +346,6 @@ def _growth_band(sur, cav, X, pe, pr, max_trips, growth_stop, max_norm):
     return X, trips
 
 
-def _canon(V):
-    return V * np.exp(-1j * canonical_phase(V))[:, None]
-
-
-def _anderson_band(sur, cav, X, pe, pr, max_trips, tol, depth=5):
-    """Fixed point of the bin-level round trip under the surrogate, in the
-    frame with the signal's mean phase removed (the steady state is defined
-    up to that phase, which the cavity rotates every trip).
-
-    Anderson mixing (type II, depth `depth`) of
-        G(S) = canon(D F(S + P)),   S = signal part of the crystal input,
-    one surrogate trip per evaluation, every iterate projected back to the
-    canonical frame (a mix of canonical vectors is not canonical).  The
-    history restarts when the residual grows 100-fold over its best.  An
-    instance stops when ||G(S) - S|| / ||S|| < tol; near-unit contraction
-    makes the error ~ residual / (1 - lambda), so tol is small.  Returns
-    (crystal inputs X, trips per instance)."""
-    B = X.shape[0]
-    trips = np.zeros(B, int)
-    P = cav.pump
-    def G(idx, S):
-        F = sur(S + P[idx], pe[idx], pr[idx], cav.lin[idx])
-        return _canon(cav.dphase[idx] * F)
-    S = _canon(X - P)
-    hist_s = [[] for _ in range(B)]
-    hist_g = [[] for _ in range(B)]
-    active = np.ones(B, bool)
-    best = np.full(B, np.inf)
-    for _ in range(max_trips):
-        idx = np.where(active)[0]
-        if idx.size == 0:
-            break
-        g = G(idx, S[idx])
-        trips[idx] += 1
-        for j, i in enumerate(idx):
-            r = g[j] - S[i]
-            res = np.linalg.norm(r) / max(np.linalg.norm(S[i]), 1e-300)
-            if res < tol:
-                S[i] = g[j]
-                active[i] = False
-                continue
-            if res > 100.0 * best[i]:
-                hist_s[i].clear()
-                hist_g[i].clear()
-            best[i] = min(best[i], res)
-            hist_s[i].append(S[i].copy())
-            hist_g[i].append(g[j].copy())
-            if len(hist_s[i]) > depth + 1:
-                hist_s[i].pop(0)
-                hist_g[i].pop(0)
-            m = len(hist_s[i]) - 1
-            if m < 1:
-                S[i] = g[j]
-                continue
-            R = np.stack([(hist_g[i][k + 1] - hist_s[i][k + 1]) - (hist_g[i][k] - hist_s[i][k])
-                          for k in range(m)], axis=1)
-            Gd = np.stack([hist_g[i][k + 1] - hist_g[i][k] for k in range(m)], axis=1)
-            gam = np.linalg.lstsq(R, r, rcond=None)[0]
-            S[i] = _canon((g[j] - Gd @ gam)[None])[0]
-    return S + P, trips
-
-
 def solve_warm(cfgs, insts, builds, sur: BandSurrogate, conv_tol=1e-4, budget=1000,
                max_surrogate_trips=100, surrogate_tol=1e-5, gate_tol=0.3, seed_rel=1e-3,
                chunk=10, log=print, ctx=None, max_seconds=None, accel="growth",
@@ -428,9 +364,7 @@ def solve_warm(cfgs, insts, builds, sur: BandSurrogate, conv_tol=1e-4, budget=10
     seeds = [smooth_seed(b["simwin"], c, seed_rel) for c, b in zip(cfgs, builds)]
     X = np.stack([sb.bins(ph.SimWindow.fft_order(ph.SimWindow.t2k(s))) for s in seeds]) + cav.pump
     s_trips = np.zeros(B, int)
-    if accel == "anderson":
-        X, s_trips = _anderson_band(sur, cav, X, pe, pr, max_surrogate_trips, surrogate_tol)
-    elif accel == "growth":
+    if accel == "growth":
         X, s_trips = _growth_band(sur, cav, X, pe, pr, max_surrogate_trips, growth_stop,
                                   getattr(sur, "max_norm", np.inf))
     else:

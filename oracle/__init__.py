@@ -11,17 +11,25 @@ RK23 control flow of toolbox/lib/MEXlib/src/ode23/OPO_CUDA_ODE23.cu and the
 round-trip order of toolbox/lib/classes/OpticalSim.m:207-301.  This module is
 a ctypes wrapper plus numpy conveniences.
 
-Parity status per function (what pins it; tests/test_oracle_*.py):
-  fft            : brute-force long-double DFT, numpy.fft
-  nee_heun       : linear closed form (Chi2 = 0), Heun 2nd-order convergence,
-                   energy conservation, low-gain SHG sinc^2 law
-  rk23           : 3rd-order convergence, linear closed form
+Parity status per function (what pins it; tests/test_oracle.py,
+tests/test_oracle_pins.py):
+  fft            : brute-force long-double DFT, numpy.fft, committed golden
+                   vectors at 2^13-2^16
+  nee_heun       : linear closed form (Chi2 = 0, incl. the k bookkeeping),
+                   hand-worked step tables for growth, clipping at NSteps and
+                   forced rejection, Heun 2nd-order convergence, low-gain SHG
+                   sinc^2 law (magnitude and L^2), low-gain difference-frequency
+                   law of the 2|A|^2 term
+  rk23           : 3rd-order convergence, hand-worked step tables for doubling,
+                   chunk-end truncation and forced rejection (k -= s, s -= 1),
+                   linear closed form
   gayer_n        : published index value of 5% MgO:CLN
   phi_calc       : numpy diff + cumulative trapezoid on the same arrays,
                    GVD sign/magnitude check
   qpm_poling     : exact square-wave structure
-  opo_run        : linear cavity closed form (Chi2 = 0) and the single-pass
-                   solver (trips = 1 equals one nee_heun call)
+  opo_run        : linear cavity (Chi2 = 0) geometric series for the field,
+                   the output-coupled spectrum, its energy and the
+                   convergence metric (Heun and RK23); stop at conv_tol
   Trajectories of the full reference path (MATLAB/CUDA) cannot be run here:
   parity against the reference itself is unpinned beyond these checks.
 """

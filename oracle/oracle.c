@@ -32,7 +32,7 @@ typedef double complex cplx;
 /* FFT: iterative radix-2 decimation-in-time, twiddles from a per-size table.  */
 /* Forward is the unnormalised DFT X[k] = sum_j x[j] exp(-2 pi i jk/N) (MATLAB  */
 /* fft); inverse is (1/N) sum_k X[k] exp(+2 pi i jk/N) (MATLAB ifft).          */
-/* Pinned by: brute-force O(N^2) DFT in long double (tests/test_oracle_fft.py)  */
+/* Pinned by: brute-force O(N^2) DFT in long double (tests/test_oracle.py)   */
 /* and numpy.fft on random inputs up to 2^16.                                  */
 /* ------------------------------------------------------------------------- */
 typedef struct {
@@ -207,8 +207,9 @@ static int any_nan(int n, const cplx *E) {
 /*  - nChunks = nSteps/sel iterates floor(nChunks) chunks (:39-41);            */
 /*  - max-norm error against |Et|+1 with strict '>' (:65-68,:77);              */
 /*  - NaN in Et leaves the chunk loop early (:44-46).                          */
-/* Pinned by: linear closed form (Chi2 = 0), Heun 2nd-order convergence with   */
-/* beta = 0, energy conservation with alpha = 1, low-gain SHG sinc^2 law.      */
+/* Pinned by: linear closed form (Chi2 = 0), hand-worked step tables (growth,  */
+/* clip at nSteps, forced rejection), Heun 2nd-order convergence with beta = 0, */
+/* low-gain SHG sinc^2 law, low-gain difference-frequency law.                 */
 /* ------------------------------------------------------------------------- */
 int orc_nee_heun(int n, double *Et_io, const double *xT_h, const double *G33, double w0,
                  double bdiffw0, double h, int nSteps, double dt, const double *hBarg,
@@ -303,7 +304,9 @@ done:
 /* (NEE_CPU.m:53) instead of gx*dt; the NEE_CPU absorber (NEE_CPU.m:114-121); */
 /* exact rational tableau instead of float-rounded constants (:425-427).      */
 /* Pinned by: 3rd-order global convergence with beta = 0 (a wrong tableau     */
-/* entry or stage abscissa drops the order), linear closed form.              */
+/* entry or stage abscissa drops the order); with Chi2 = 0, hand-worked step  */
+/* tables (doubling, chunk-end truncation, forced rejection) and the linear   */
+/* closed form of the field.                                                  */
 /* ------------------------------------------------------------------------- */
 int orc_rk23(int n, double *Et_io, const double *xT_h, const double *G, double w0,
              double bdiffw0, double h, int Nsteps, double dt, const double *hBarg,
@@ -706,6 +709,9 @@ static void k2t(const fft_plan *p, cplx *x, cplx *tmp) {
 /* Outputs: E_cav (in/out), E_oc_last (field of the last trip's output),       */
 /*   oc_energy[trips] = sum |Ek_oc|^2 (proportional to pulse energy),         */
 /*   conv[trips], steps[2*trips] (attempts, accepted).                         */
+/* Pinned by: with Chi2 = 0 a trip is a diagonal operator in frequency, so the */
+/* field after t trips, the output-coupled spectrum, its energy and conv[] are */
+/* a geometric series in closed form (Heun and RK23); the conv_tol stop.       */
 /* ------------------------------------------------------------------------- */
 int orc_opo_run(int n, int method, int trips, double *Ecav_io, const double *Epump_in,
                 const double *n_air, const double *n_xtal, const double *Roc, const double *Toc,

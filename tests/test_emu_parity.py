@@ -355,3 +355,39 @@ def test_pass_with_ktp_and_a_chirped_grating_matches_oracle(emu_ctx):
     Eg, apg, smg = _run(emu_ctx, E, args, 512, "double", 2)
     assert np.array_equal(smg, smo)
     assert rel_l2(Eg, Eo) < 1e-10 and rel_l2(apg, apo) < 1e-10
+
+
+def test_plain_pass_is_not_a_round_trip(emu_ctx):
+    """A plain crystal pass (launch(0)) on a cavity run counts as no trip:
+    the trip count, the run averages and the history ring are left to the
+    round trips, so a later read_average divides by the trips actually
+    averaged and read_history holds no stale row for the pass."""
+    cfg, b = opo_case()
+    si, ops = b["solver"], b["ops"]
+    run = OPOBatch([dict(solver=si, field0=ops["field0"], pump=ops["pump"], cav=ops["cav"],
+                         ocm=ops["ocm"], delay=cfg.delay)], precision="double", trip_cap=4,
+                   ctx=emu_ctx, tune={"cluster": 2})
+    run.keep_history(4)
+    run.launch(1)
+    run.sync()
+    e1, x1, k1 = run.read_average(0)
+    h0, sm0 = run.read_history(0, 0)
+    log1 = run.read(0)["trip_log"].copy()
+    run.launch(0)
+    run.sync()
+    r = run.read(0)
+    assert r["trips_done"] == 1 and k1 == 1
+    e, x, k = run.read_average(0)
+    assert k == 1 and np.array_equal(e, e1) and np.array_equal(x, x1)
+    with pytest.raises(OfxError):
+        run.read_history(0, 1)
+    h, sm = run.read_history(0, 0)
+    assert np.array_equal(h, h0) and np.array_equal(sm, sm0)
+    assert np.array_equal(r["trip_log"], log1)
+    run.launch(1)
+    run.sync()
+    assert run.read(0)["trips_done"] == 2
+    assert run.read_average(0)[2] == 2
+    h1, sm1 = run.read_history(0, 1)
+    assert sm1.any() and np.isfinite(h1).all()
+    run.close()

@@ -264,3 +264,82 @@ def test_optic_chain_into_cavity_operators(tmp_path):
     f = sw.fft_order(np.sqrt(T_c) * np.exp(-1j * np.nan_to_num(phi_c)))
     assert np.allclose(pre["cav"], base["cav"] * f) and np.allclose(pre["ocm"], base["ocm"] * f)
     assert np.allclose(post["cav"], base["cav"] * f) and np.allclose(post["ocm"], base["ocm"])
+
+
+def test_material_table_skips_rows_without_coefficients(tmp_path):
+    """The reference table's layout (extra trailing columns, an unnamed one)
+    with a PPKTP row tagged Schott whose coefficient cells are blank: that
+    row is skipped, the Schott and CDGM glasses around it load."""
+    p = tmp_path / "Materials.csv"
+    p.write_text(
+        "Material,B1,B2,B3,C1,C2,C3,Equivalent,Equation,T_file,T_length_m,Fresnel_Reflections,,D1,D2\n"
+        "SiO2,0.6961663,0.4079426,0.8974794,0.004679148,0.01351206,97.93400,,Schott,a.txt,0.01,2,,,\n"
+        "BK7L,2.2718929,-0.010108077,0.010592509,0.00020816965,-7.6472538e-06,4.9240991e-07,N-BK7,CDGM,a.txt,0.01,2,,,\n"
+        "PPKTP,,,,,,,,Schott,b.txt,0.001,2,,,\n")
+    t = ph.load_material_table(str(p))
+    assert set(t) == {"SiO2", "BK7L"}
+    assert abs(ph.sellmeier_n(0.5875618, "SiO2", table=t) - 1.4585) < 1e-4
+    assert abs(ph.sellmeier_n(0.5875618, "BK7L", table=t) - 1.5168) < 2e-4
+
+
+def _flat_gdd_optic(tmp_path, theta=0.0, **kw):
+    p = tmp_path / "Materials.csv"
+    p.write_text("Material,B1,B2,B3,C1,C2,C3,Equivalent,Equation\n"
+                 "FS,0.6961663,0.4079426,0.8974794,0.004679148,0.01351206,97.93400,,Schott\n")
+    t = ph.load_material_table(str(p))
+    sw = ph.SimWindow(1040e-9, 2048, 8e-12)
+    lam = np.linspace(500.0, 2500.0, 81)
+    gdd = (lam, np.full(lam.size, -150.0))
+    return sw, t, gdd
+
+
+def test_transmitted_optic_counts_both_coated_surfaces(tmp_path):
+    """Optic.simulate in the 'T' regime adds S1's, the bulk's and S2's
+    phases (Optic.m:93-97); S2 defaults to a copy of S1, measured GDD
+    included (Optic.m:57,72-75).  A flat -150 fs^2 coating on a 1 mm FS
+    window therefore gives 2 D (w - w0)^2 / 2 plus the bulk phase; naming
+    no curve for S2 leaves one D; in reflection only S1's counts."""
+    sw, t, gdd = _flat_gdd_optic(tmp_path)
+    x = sw.omegas - sw.reference_omega
+    bulk = ph.bulk_dispersion(sw, "FS", 1e-3, table=t)
+    _, _, phi2 = ph.optic_response(sw, "T", "AR", "FS", 1e-3, table=t, gdd=gdd)
+    _, _, phi1 = ph.optic_response(sw, "T", "AR", "FS", 1e-3, table=t, gdd=gdd, gdd2=None)
+    _, _, phir = ph.optic_response(sw, "R", "AR", "FS", 1e-3, table=t, gdd=gdd)
+    ok = np.isfinite(phi2) & np.isfinite(bulk)
+    ok[0] = False
+    D = -150e-30
+    assert ok.sum() > 1000
+    assert np.allclose(phi2[ok], (2 * D * x ** 2 / 2 + bulk)[ok], rtol=1e-9, atol=1e-9)
+    assert np.allclose(phi1[ok], (D * x ** 2 / 2 + bulk)[ok], rtol=1e-9, atol=1e-9)
+    assert np.allclose(phir[ok], (D * x ** 2 / 2)[ok], rtol=1e-9, atol=1e-9)
+
+
+def test_coating_phase_moves_to_shorter_wavelengths_off_normal(tmp_path):
+    """OpticalSurface.m:105-110: at incidence theta the GDD phase is
+    circshifted by -round(0.2 lam0 sin^2 theta / dlam) samples.  The grid's
+    wavelengths fall with the index (dlam < 0), so the curve moves to higher
+    indices, i.e. shorter wavelengths."""
+    sw, t, gdd = _flat_gdd_optic(tmp_path)
+    lam, ref = sw.wavelengths, sw.reference_index
+    dlam = (lam[ref + 1] - lam[ref - 1]) / 2
+    assert dlam < 0
+    th = 30.0
+    k = -int(np.sign(0.2 * lam[ref] * 0.25 / dlam) * np.floor(abs(0.2 * lam[ref] * 0.25 / dlam) + 0.5))
+    assert k > 0
+    p0 = ph.coating_phase(sw, gdd, 0.0)
+    p1 = ph.coating_phase(sw, gdd, th)
+    assert np.array_equal(np.isnan(p1), np.isnan(np.roll(p0, k)))
+    ok = np.isfinite(p1)
+    assert np.allclose(p1[ok], np.roll(p0, k)[ok], rtol=0, atol=0)
+
+
+def test_bare_surface_uses_the_20C_index():
+    """fresnel(1, material, theta, lam, exit) calls sellmeier without a
+    temperature (fresnel.m:10-16), so a bare 'None' surface's transmission
+    does not follow the optic's temperature; the bulk phase does."""
+    sw = ph.SimWindow(1040e-9, 1024, 4e-12)
+    a = ph.optic_response(sw, "T", "None", "PPLN", 1e-3, T=20.0)
+    b = ph.optic_response(sw, "T", "None", "PPLN", 1e-3, T=150.0)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    ok = np.isfinite(a[2]) & np.isfinite(b[2])
+    assert not np.allclose(a[2][ok], b[2][ok])

@@ -98,32 +98,3 @@ def test_sync_delay_and_qpm_wavelength():
     assert 0.4e-12 < d5 < 0.6e-12 and d8 == pytest.approx(d5 * 8 / 5, rel=0.02)
     b = _cfg(delay=20e-15).build()
     assert b["delay"] == pytest.approx(d8 + 20e-15, rel=1e-6)
-
-
-def test_anderson_band_solves_a_slow_rotating_contraction():
-    """A slowly contracting band map (eigenvalues 0.985-0.995) whose output
-    phase turns every trip: plain iteration leaves ~60 % error after 100
-    trips; the canonical-frame Anderson solve reaches the fixed point."""
-    rng = np.random.default_rng(0)
-    B, n = 3, ws.NBINS
-    A = [0.985 + 0.01 * rng.random(n) for _ in range(B)]
-    target = rng.standard_normal((B, n)) + 1j * rng.standard_normal((B, n))
-    tc = ws._canon(target)
-
-    class Cav:
-        pump = np.zeros((B, n), complex)
-        dphase = np.exp(0.7j) * np.ones((B, n))
-        lin = np.ones((B, n), complex)
-
-    def sur(X, pe, pr, lin):
-        out = []
-        for x, k in zip(X, pe.astype(int)):
-            th = ws.canonical_phase(x[None])[0]
-            out.append((tc[k] + A[k] * (x * np.exp(-1j * th) - tc[k])) * np.exp(1j * th))
-        return np.array(out)
-
-    X, trips = ws._anderson_band(sur, Cav, 1e-3 * target, np.arange(B, dtype=float),
-                                 np.zeros(B), 100, 1e-5)
-    err = np.linalg.norm(ws._canon(X) - tc, axis=1) / np.linalg.norm(tc, axis=1)
-    assert np.all(trips <= 100)
-    assert np.all(err < 1e-2)
