@@ -42,19 +42,19 @@ def workload():
     return cfg, cfg.build()
 
 
-def measured_traffic(key, adaptive_steps_per_launch):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary
-    (profiles/pmc_traffic.json: FETCH_SIZE + WRITE_SIZE of the pass kernel,
-    each divided by what the counter reads for a copy of known size), scaled
-    by this run's adaptive steps per launch; None without a summary."""
+def measured_traffic(key):
+    """HBM bytes per launch of the pass kernel from the committed rocprofv3
+    --pmc summary of this same command (profiles/pmc_traffic.json:
+    FETCH_SIZE + WRITE_SIZE averaged over the run's dispatches, each divided
+    by what the counter reads for a device copy of known size); None without
+    a summary."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     try:
         with open(path) as f:
             rec = json.load(f)[key]
     except (OSError, ValueError, KeyError):
         return None
-    return rec["traffic_bytes_per_launch"] * adaptive_steps_per_launch / \
-        rec["adaptive_steps_per_launch"]
+    return rec["traffic_bytes_per_launch"]
 
 
 def batch_workload(n_inst=256, seed=3):
@@ -457,31 +457,41 @@ def main_surrogate(args):
     print(json.dumps(out))
 
 
-def cpu_baseline(b, attempts_per_trip, method="heun"):
-    """Oracle (FP64, one core) on the first 1 mm of the 5 mm pass; the time per
-    adaptive step is scaled by the steps of one full round trip."""
-    import oracle
+def cavity_state(b, X):
+    """The oracle's intracavity field E_cav (natural time order, before the
+    pump is added) whose next crystal input is the device field X (fftshifted
+    time): inverts pump addition and refraction into the crystal."""
+    from optifax_amd.physics import SimWindow
     si = b["solver"]
-    sample_steps = si["nSteps"] // 5
-    sel = float(sample_steps)
-    solve = oracle.nee_heun if method == "heun" else oracle.rk23
+    r_in = np.sqrt(1.0 / si["n_xtal"])
+    return SimWindow.k2t(SimWindow.t2k(np.fft.ifftshift(X)) / r_in - SimWindow.t2k(b["pump"]))
+
+
+def cpu_baseline(b, X, method="heun"):
+    """The FP64 oracle (one core) runs one whole round trip (OpticalSim.run's
+    order: pump, refraction, the adaptive crystal pass, output coupling,
+    delay) from the crystal input X of the first timed trip.  Returns the
+    baseline record, the oracle's output-coupled spectrum (FFT order) of
+    that trip (the parity reference) and its (attempted, accepted) steps."""
+    import oracle
+    from optifax_amd.physics import SimWindow
+    si, sw = b["solver"], b["simwin"]
+    R, T, phi = b["oc"].masks(sw)
+    E_cav = cavity_state(b, X)
     t0 = time.perf_counter()
-    _, _, _, cnt = solve(b["ops"]["field0"], si["alpha_w"], si["G33"][:sample_steps],
-                         si["w0"], si["bdiffw0"], si["h"], sample_steps, si["dt"],
-                         si["beta_op"], si["maxErr"], si["minErr"], sel, n_cols=0,
-                         want_stepmods=False)
+    o = oracle.opo_run(1, E_cav, b["pump"], np.ones(sw.n), si["n_xtal"], R, T, phi, b["delay"],
+                       sw.rel_omegas, si, method=0 if method == "heun" else 1)
     dt = time.perf_counter() - t0
-    per_step = dt / max(int(cnt[0]), 1)
-    trip_s = per_step * attempts_per_trip
-    return {
-        "value": 1.0 / trip_s,
+    att, acc = (int(v) for v in o["steps"][0])
+    rec = {
+        "value": 1.0 / dt,
         "unit": "round trips/s",
         "cores": 1,
         "kind": "port",
-        "sample": (f"first {sample_steps} of {si['nSteps']} minimal steps of the pass "
-                   f"({int(cnt[0])} adaptive steps, {dt:.1f} s), scaled by the "
-                   f"{attempts_per_trip} adaptive steps of one round trip"),
+        "sample": (f"one whole round trip from the first timed trip's crystal input "
+                   f"({att} adaptive steps, {acc} accepted, N={sw.n}), {dt:.1f} s"),
     }
+    return rec, SimWindow.fft_order(SimWindow.t2k(o["E_oc"])), (att, acc)
 
 
 def cpu_baseline_batch(insts, meta, attempts_per_trip, sample_steps=3000):
@@ -615,14 +625,15 @@ def main():
     n = si["beta_op"].size
     ctx = Context(0)
     trip_cap = args.warmup + args.steps + 1
-    run = OPOBatch([dict(solver=si, field0=ops["field0"], pump=ops["pump"], cav=ops["cav"],
-                         ocm=ops["ocm"], delay=b["delay"], method=args.method)],
-                   precision=args.precision, trip_cap=trip_cap, ctx=ctx)
+    inst = dict(solver=si, field0=ops["field0"], pump=ops["pump"], cav=ops["cav"],
+                ocm=ops["ocm"], delay=b["delay"], method=args.method)
+    run = OPOBatch([inst], precision=args.precision, trip_cap=trip_cap, ctx=ctx)
     geo = run.geometry(0)
     # warm-up trips (also builds the one-trip graph)
     if args.warmup > 0:
         run.graph_replay(1, args.warmup)
         run.sync()
+    X0 = run.read(0)["field"]                  # crystal input of the first timed trip
     t0 = time.perf_counter()
     run.graph_replay(1, args.steps)
     run.sync()
@@ -633,10 +644,32 @@ def main():
     att = int(log[:, 2].sum())
     acc = int(log[:, 3].sum())
     rej = att - acc
+    # the first timed trip again from the same input (the kernel is
+    # deterministic), for its output-coupled spectrum
+    run.write_field(0, X0)
+    run.launch(1)
+    run.sync()
+    oc_run = run.read(0)["oc_spec"]
+    run.close()
+    # the other field precision from the same input: its first trip's output
+    # and its rate over the same number of graph-replayed trips
+    other = "double" if args.precision == "single" else "single"
+    run2 = OPOBatch([dict(inst, field0=X0)], precision=other, trip_cap=args.steps + 3, ctx=ctx)
+    run2.launch(1)
+    run2.sync()
+    oc_other = run2.read(0)["oc_spec"]
+    run2.graph_replay(1, 1)
+    run2.sync()
+    t1 = time.perf_counter()
+    run2.graph_replay(1, args.steps)
+    run2.sync()
+    wall2 = time.perf_counter() - t1
+    run2.close()
     bytes_total = algorithmic_bytes(n, acc, rej, args.steps)
     per_launch_ms = ev_ms / args.steps
     achieved = bytes_total / args.steps / (per_launch_ms * 1e-3) / 1e9
     value = args.steps / wall
+    rates = {args.precision: value, other: args.steps / wall2}
     out = {
         "metric": METRIC,
         "value": value,
@@ -661,6 +694,8 @@ def main():
             "cluster_workgroups": geo["cluster"],
             "block": geo["block"],
         },
+        "fp32_round_trips_per_s": rates["single"],
+        "fp64_round_trips_per_s": rates["double"],
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
@@ -669,13 +704,24 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(("ring_n32768_f32" if args.precision == "single"
                                          else "ring_n32768_f64") +
-                                        ("" if args.method == "heun" else "_rk23"),
-                                        att / args.steps),
+                                        ("" if args.method == "heun" else "_rk23")),
         },
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(b, att / args.steps, args.method)
-    run.close()
+        base, oc_oracle, (o_att, o_acc) = cpu_baseline(b, X0, args.method)
+        out["cpu_baseline"] = base
+        f32 = oc_run if args.precision == "single" else oc_other
+        f64 = oc_other if args.precision == "single" else oc_run
+        rel = lambda a, ref: float(np.linalg.norm(a - ref) / np.linalg.norm(ref))
+        out["parity"] = {
+            "against": "FP64 CPU oracle (oracle.opo_run): one round trip from the first timed "
+                       "trip's crystal input; relative L2 of the output-coupled spectrum",
+            "rel_l2_fp32": rel(f32, oc_oracle),
+            "rel_l2_fp64": rel(f64, oc_oracle),
+            "rel_l2_fp32_power_spectrum": rel(np.abs(f32) ** 2, np.abs(oc_oracle) ** 2),
+            "oracle_adaptive_steps": o_att,
+            "oracle_accepted_steps": o_acc,
+        }
     ctx.close()
     print(json.dumps(out))
 

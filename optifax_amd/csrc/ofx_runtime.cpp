@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -18,11 +19,21 @@
 
 using namespace ofx;
 
+// What a cached single-pass run was created for: a later OPOmexBatch call
+// with the same key reuses its device buffers and only re-uploads the data.
+struct PassKey {
+  int prec, batch, g_batch, has_tune, apft, stepmods;
+  ofx_pass_params p;
+  ofx_tuning t;
+};
+
 struct ofx_context {
   Backend* be = nullptr;
   DeviceInfo info{};
   std::string err;
   std::map<std::pair<int, int>, void*> tw;  // (nmax, precision) -> table
+  ofx_opo* pass_run = nullptr;              // the last single-pass run, kept for reuse
+  PassKey pass_key;
 };
 
 struct InstBufs {
@@ -151,17 +162,22 @@ const void* convert(const double* src, size_t count, int prec, std::vector<std::
   return out;
 }
 
+// t[2m], t[2m+1] = exp(-2 pi i m / nmax), m < nmax; angles are dyadic
+// fractions of a turn
+void fill_twiddle(int nmax, double* t) {
+  for (int m = 0; m < nmax; m++) {
+    const double a = -2.0 * M_PI * (double)m / (double)nmax;
+    t[2 * m] = cos(a);
+    t[2 * m + 1] = sin(a);
+  }
+}
+
 void* twiddle_table(ofx_context* ctx, int nmax, int prec) {
   auto key = std::make_pair(nmax, prec);
   auto it = ctx->tw.find(key);
   if (it != ctx->tw.end()) return it->second;
   std::vector<double> t(2 * (size_t)nmax);
-  for (int m = 0; m < nmax; m++) {
-    // exp(-2 pi i m / nmax); angles are dyadic fractions of a turn
-    const double a = -2.0 * M_PI * (double)m / (double)nmax;
-    t[2 * m] = cos(a);
-    t[2 * m + 1] = sin(a);
-  }
+  fill_twiddle(nmax, t.data());
   void* p = nullptr;
   const size_t bytes = (size_t)nmax * (prec == OFX_F64 ? 16 : 8);
   if (be_malloc(ctx->be, &p, bytes)) return nullptr;
@@ -188,9 +204,7 @@ double reduce_2pi_host(double x) {
 
 // exp(i w0 t_j) and the absorbing-boundary factor for the fftshifted time
 // axis t_j = fftshift((-N/2:N/2-1)*dt) (NEE_CPU.m:12-14, 53, 60, 114-121)
-void time_tables(int n, double dt, double w0, std::vector<double>& wt, std::vector<double>& ab) {
-  wt.resize(2 * (size_t)n);
-  ab.resize((size_t)n);
+void time_tables(int n, double dt, double w0, double* wt, double* ab) {
   const double tr = (double)(n / 2 - 1) * dt - (double)(-(n / 2)) * dt;  // range(t)
   for (int j = 0; j < n; j++) {
     const double t = (double)(j < n / 2 ? j : j - n) * dt;
@@ -225,6 +239,72 @@ int check_pass(ofx_context* ctx, const ofx_pass_params* p) {
   return OFX_OK;
 }
 
+// Launch geometry of one instance of a batch of n_inst: the four-step split
+// n1 (N = n1 n2) and the cluster size C (workgroups per instance).
+int plan_instance(ofx_context* ctx, const ofx_pass_params& p, int n_inst, const ofx_tuning* tune,
+                  int spec_cluster, int spec_n1, size_t csz, int* C_out, int* n1_out) {
+  const int n = p.n, lg = ilog2i(n);
+  int n1 = 1 << ((lg + 1) / 2);
+  if (tune && tune->n1) n1 = tune->n1;
+  if (spec_n1) n1 = spec_n1;
+  if (ilog2i(n1) < 2 || n % n1 || n / n1 < 4)
+    return fail(ctx, OFX_E_ARG, "n1=%d does not split N=%d into two powers of two >= 4", n1, n);
+  const int n2 = n / n1;
+  const int meth = p.method;
+  const int cmax = max_cluster(meth, n1, n2);
+  int cfit = 0;
+  for (int C = 1; C <= cmax; C <<= 1) {
+    if ((int)lds_bytes_for(n1, n2, C, meth, csz) <= ctx->info.lds_max - 1024) {
+      cfit = C;
+      break;
+    }
+  }
+  if (!cfit)
+    return fail(ctx, OFX_E_ARG, "N=%d does not fit the %d-byte LDS limit at any cluster size", n,
+                ctx->info.lds_max);
+  int C = cfit;
+  const int want_C = spec_cluster ? spec_cluster : (tune ? tune->cluster : 0);
+  if (want_C) {
+    C = want_C;
+    if (ilog2i(C) < 0 || C > cmax ||
+        (int)lds_bytes_for(n1, n2, C, meth, csz) > ctx->info.lds_max - 1024)
+      return fail(ctx, OFX_E_ARG, "cluster size %d is invalid for N=%d (n1=%d)", C, n, n1);
+  } else {
+    // Spread a small batch over more workgroups: a step's latency is set by
+    // the per-workgroup share until it falls to ~256 points (measured on
+    // N = 2^15, FP32 / FP64 us per adaptive step: 20.1 / 24.2 at 32
+    // workgroups, 16.6 / 21.1 at 64, 15.5 / 19.6 at 128 with 256-thread
+    // workgroups).
+    const int slots_est = ctx->info.cus;
+    while (C * 2 <= cmax && (long long)n_inst * C * 2 <= slots_est && C * 2 <= 128 &&
+           n / (C * 2) >= 256)
+      C *= 2;
+  }
+  *C_out = C;
+  *n1_out = n1;
+  return OFX_OK;
+}
+
+// Threads per workgroup: 256 once every cluster has 128 members or more (a
+// share of <= 256 points per workgroup), 512 below.
+int plan_block(ofx_context* ctx, const ofx_tuning* tune, int cmin, int* block) {
+  const int exchange = tune ? tune->exchange : 0;
+  if (exchange < 0 || exchange > 2)
+    return fail(ctx, OFX_E_ARG, "exchange=%d must be 0 (auto), 1 (tags) or 2 (barrier)", exchange);
+  *block = (tune && tune->block) ? tune->block : (cmin >= 128 ? 256 : 512);
+  if (*block < 64 || *block > 512 || (*block & 63))
+    return fail(ctx, OFX_E_ARG, "block=%d must be a multiple of 64 in [64,512]", *block);
+  return OFX_OK;
+}
+
+// Exchange mode of a launch: polling the tagged words wins when every
+// cluster has at least 64 workgroups (a few elements per thread, latency
+// bound); with larger shares per workgroup the plain words plus a counter
+// barrier move half the bytes and win (measured on N = 2^15).
+int exchange_mode(int exchange, int cmin) {
+  return exchange == 1 ? 0 : exchange == 2 ? 1 : (cmin >= 64 ? 0 : 1);
+}
+
 }  // namespace
 
 extern "C" {
@@ -244,6 +324,8 @@ int ofx_create(int device, ofx_context** out) {
 
 void ofx_destroy(ofx_context* ctx) {
   if (!ctx) return;
+  ofx_opo_destroy(ctx->pass_run);
+  ctx->pass_run = nullptr;
   for (auto& kv : ctx->tw) be_free(ctx->be, kv.second);
   be_destroy(ctx->be);
   delete ctx;
@@ -302,67 +384,22 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
   r->inst_n1.resize(n_inst);
   int nmax = 0;
   for (int i = 0; i < n_inst; i++) {
-    const int n = specs[i].pass.n, lg = ilog2i(n);
-    nmax = std::max(nmax, n);
-    int n1 = 1 << ((lg + 1) / 2);
-    if (tune && tune->n1) n1 = tune->n1;
-    if (specs[i].n1) n1 = specs[i].n1;
-    if (ilog2i(n1) < 2 || n % n1 || n / n1 < 4) {
+    nmax = std::max(nmax, specs[i].pass.n);
+    st = plan_instance(ctx, specs[i].pass, n_inst, tune, specs[i].cluster, specs[i].n1, r->csz,
+                       &r->inst_C[i], &r->inst_n1[i]);
+    if (st) {
       delete r;
-      return fail(ctx, OFX_E_ARG, "n1=%d does not split N=%d into two powers of two >= 4", n1, n);
+      return st;
     }
-    const int n2 = n / n1;
-    const int meth = specs[i].pass.method;
-    const int cmax = max_cluster(meth, n1, n2);
-    int cfit = 0;
-    for (int C = 1; C <= cmax; C <<= 1) {
-      if ((int)lds_bytes_for(n1, n2, C, meth, r->csz) <= ctx->info.lds_max - 1024) {
-        cfit = C;
-        break;
-      }
-    }
-    if (!cfit) {
-      delete r;
-      return fail(ctx, OFX_E_ARG, "N=%d does not fit the %d-byte LDS limit at any cluster size", n,
-                  ctx->info.lds_max);
-    }
-    int C = cfit;
-    const int want_C = specs[i].cluster ? specs[i].cluster : (tune ? tune->cluster : 0);
-    if (want_C) {
-      C = want_C;
-      if (ilog2i(C) < 0 || C > cmax ||
-          (int)lds_bytes_for(n1, n2, C, meth, r->csz) > ctx->info.lds_max - 1024) {
-        delete r;
-        return fail(ctx, OFX_E_ARG, "cluster size %d is invalid for N=%d (n1=%d)", C, n, n1);
-      }
-    } else {
-      // Spread a small batch over more workgroups: a step's latency is set by
-      // the per-workgroup share until it falls to ~256 points (measured on
-      // N = 2^15, FP32 / FP64 us per adaptive step: 20.1 / 24.2 at 32
-      // workgroups, 16.6 / 21.1 at 64, 15.5 / 19.6 at 128 with 256-thread
-      // workgroups).
-      const int slots_est = ctx->info.cus;
-      while (C * 2 <= cmax && (long long)n_inst * C * 2 <= slots_est &&
-             C * 2 <= 128 && n / (C * 2) >= 256)
-        C *= 2;
-    }
-    r->inst_C[i] = C;
-    r->inst_n1[i] = n1;
   }
   r->nmax = nmax;
   int cmin0 = 1 << 30;
   for (int i = 0; i < n_inst; i++) cmin0 = std::min(cmin0, r->inst_C[i]);
-  // 256-thread workgroups once every cluster has 128 members or more (a
-  // share of <= 256 points per workgroup), 512 below
-  r->block = (tune && tune->block) ? tune->block : (cmin0 >= 128 ? 256 : 512);
   const int exchange = tune ? tune->exchange : 0;
-  if (exchange < 0 || exchange > 2) {
+  st = plan_block(ctx, tune, cmin0, &r->block);
+  if (st) {
     delete r;
-    return fail(ctx, OFX_E_ARG, "exchange=%d must be 0 (auto), 1 (tags) or 2 (barrier)", exchange);
-  }
-  if (r->block < 64 || r->block > 512 || (r->block & 63)) {
-    delete r;
-    return fail(ctx, OFX_E_ARG, "block=%d must be a multiple of 64 in [64,512]", r->block);
+    return st;
   }
   size_t lds = 0;
   int cmin = 1 << 30;
@@ -372,11 +409,7 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
     cmin = std::min(cmin, r->inst_C[i]);
   }
   r->lds = lds;
-  // Exchange mode of the launch: polling the tagged words wins when every
-  // cluster has at least 64 workgroups (a few elements per thread, latency
-  // bound); with larger shares per workgroup the plain words plus a counter
-  // barrier move half the bytes and win (measured on N = 2^15).
-  r->xmode = exchange == 1 ? 0 : exchange == 2 ? 1 : (cmin >= 64 ? 0 : 1);
+  r->xmode = exchange_mode(exchange, cmin);
   const int bpc = be_blocks_per_cu(ctx->be, precision, r->xmode, r->block, lds);
   if (bpc < 1) {
     delete r;
@@ -506,8 +539,8 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
     if (st) break;
     upload(r, b.field, convert(s.field, 2 * n, precision, keep), n * r->csz);
     {
-      std::vector<double> wt, ab;
-      time_tables(p.n, p.dt, p.w0, wt, ab);
+      std::vector<double> wt(2 * n), ab(n);
+      time_tables(p.n, p.dt, p.w0, wt.data(), ab.data());
       upload(r, b.wt, convert(wt.data(), wt.size(), precision, keep), n * r->csz);
       upload(r, b.absorb, convert(ab.data(), ab.size(), precision, keep), n * rsz);
     }
@@ -907,6 +940,43 @@ int ofx_opo_xcc_ids(ofx_opo* r, int* ids, int cap, int* count) {
 }  // extern "C"
 
 // ---------------- host single pass (OPOmexBatch computation) ----------------
+// OpticalSim.run calls the solver once per round trip with the same shapes
+// and tables (OpticalSim.m:237-252), so the context keeps the device run of
+// the last call: a call with the same precision, pass parameters, batch,
+// G layout, outputs and tuning re-uploads the field, tables, ApFT and
+// stepMods into it instead of allocating and planning a new one (the
+// reference creates and destroys its buffers and FFT plan per call,
+// OPO_CUDA_BATCH_XTAL.cu:415-419,518-522).
+static PassKey pass_key_of(int prec, const ofx_pass_params* p, int batch, int g_batch, bool apft,
+                           bool sm, const ofx_tuning* tune) {
+  PassKey k;
+  memset(&k, 0, sizeof k);  // padding too: keys are compared bytewise
+  k.prec = prec;
+  k.batch = batch;
+  k.g_batch = g_batch;
+  k.apft = apft ? 1 : 0;
+  k.stepmods = sm ? 1 : 0;
+  k.p.n = p->n;
+  k.p.nsteps = p->nsteps;
+  k.p.w0 = p->w0;
+  k.p.bdiffw0 = p->bdiffw0;
+  k.p.h = p->h;
+  k.p.dt = p->dt;
+  k.p.max_err = p->max_err;
+  k.p.min_err = p->min_err;
+  k.p.sel = p->sel;
+  k.p.n_cols = apft ? p->n_cols : 0;
+  k.p.method = p->method;
+  if (tune) {
+    k.has_tune = 1;
+    k.t.cluster = tune->cluster;
+    k.t.n1 = tune->n1;
+    k.t.block = tune->block;
+    k.t.exchange = tune->exchange;
+  }
+  return k;
+}
+
 template <typename T>
 static int pass_host_impl(ofx_context* ctx, int prec, const ofx_pass_params* p, int batch, T* Et,
                           const T* alpha_w, const T* G, int g_batch, const T* beta_op, T* ApFT,
@@ -920,6 +990,7 @@ static int pass_host_impl(ofx_context* ctx, int prec, const ofx_pass_params* p, 
   if (g_batch != 1 && g_batch != batch)
     return fail(ctx, OFX_E_ARG, "G must hold NSteps or NSteps*batch values (g_batch=%d)", g_batch);
   const size_t n = (size_t)p->n;
+  const bool want_ap = ApFT && p->n_cols > 0;
   std::vector<double> beta(n), alpha(n), field(2 * n * batch), Gd((size_t)p->nsteps * g_batch);
   for (size_t j = 0; j < n; j++) {
     beta[j] = (double)beta_op[j];
@@ -927,25 +998,44 @@ static int pass_host_impl(ofx_context* ctx, int prec, const ofx_pass_params* p, 
   }
   for (size_t j = 0; j < field.size(); j++) field[j] = (double)Et[j];
   for (size_t j = 0; j < Gd.size(); j++) Gd[j] = (double)G[j];
-  std::vector<ofx_opo_spec> specs((size_t)batch);
-  for (int b = 0; b < batch; b++) {
-    ofx_opo_spec& s = specs[b];
-    memset(&s, 0, sizeof s);
-    s.pass = *p;
-    if (!ApFT) s.pass.n_cols = 0;
-    s.beta = beta.data();
-    s.alpha = alpha.data();
-    s.G = Gd.data() + (g_batch == 1 ? 0 : (size_t)b * p->nsteps);
-    s.field = field.data() + 2 * n * b;
-  }
-  ofx_opo* r = nullptr;
-  st = ofx_opo_create(ctx, prec, batch, specs.data(), 0, tune, &r);
-  if (st) return st;
-  const size_t cs = r->csz;
+  const PassKey key = pass_key_of(prec, p, batch, g_batch, want_ap, stepMods != nullptr, tune);
+  ofx_opo* r = ctx->pass_run;
   std::vector<std::vector<char>> keep;
+  if (r && memcmp(&key, &ctx->pass_key, sizeof key) == 0) {
+    // same shapes: new data into the kept buffers (the time tables depend
+    // only on N, dt and w0, which are part of the key)
+    for (int b = 0; b < batch && !st; b++) {
+      InstBufs& bf = r->bufs[b];
+      st = upload(r, bf.field, convert(field.data() + 2 * n * b, 2 * n, prec, keep), n * r->csz);
+      if (!st) st = upload(r, bf.beta, beta.data(), n * 8);
+      if (!st) st = upload(r, bf.alpha, alpha.data(), n * 8);
+      if (!st)
+        st = upload(r, bf.G, Gd.data() + (g_batch == 1 ? 0 : (size_t)b * p->nsteps),
+                    (size_t)p->nsteps * 8);
+    }
+  } else {
+    ofx_opo_destroy(ctx->pass_run);
+    ctx->pass_run = r = nullptr;
+    std::vector<ofx_opo_spec> specs((size_t)batch);
+    for (int b = 0; b < batch; b++) {
+      ofx_opo_spec& s = specs[b];
+      memset(&s, 0, sizeof s);
+      s.pass = *p;
+      if (!want_ap) s.pass.n_cols = 0;
+      s.beta = beta.data();
+      s.alpha = alpha.data();
+      s.G = Gd.data() + (g_batch == 1 ? 0 : (size_t)b * p->nsteps);
+      s.field = field.data() + 2 * n * b;
+    }
+    st = ofx_opo_create(ctx, prec, batch, specs.data(), 0, tune, &r);
+    if (st) return st;
+    ctx->pass_run = r;
+    ctx->pass_key = key;
+  }
+  const size_t cs = r->csz;
   // the MEX contract returns the input ApFT/stepMods with written entries replaced
   for (int b = 0; b < batch && !st; b++) {
-    if (ApFT && p->n_cols > 0) {
+    if (want_ap) {
       std::vector<double> tmp(2 * n * p->n_cols);
       for (size_t j = 0; j < tmp.size(); j++) tmp[j] = (double)ApFT[2 * n * p->n_cols * b + j];
       st = upload(r, r->bufs[b].apft, convert(tmp.data(), tmp.size(), prec, keep), n * p->n_cols * cs);
@@ -959,25 +1049,30 @@ static int pass_host_impl(ofx_context* ctx, int prec, const ofx_pass_params* p, 
   }
   if (!st) st = ofx_opo_launch(r, 0, 0.0);
   int sync_st = st ? st : ofx_opo_sync(r);
-  if (sync_st == OFX_E_HIP || sync_st == OFX_E_TIMEOUT) {
-    ofx_opo_destroy(r);
-    return sync_st;
+  if (sync_st == OFX_E_HIP || sync_st == OFX_E_TIMEOUT || st) {
+    // a run in an unknown state is not kept
+    ofx_opo_destroy(ctx->pass_run);
+    ctx->pass_run = nullptr;
+    return st ? st : sync_st;
   }
   std::vector<double> f(2 * n), ap(2 * n * std::max(1, p->n_cols)), sm((size_t)p->nsteps);
   for (int b = 0; b < batch && (st == OFX_OK); b++) {
     int s_b = 0;
     st = ofx_opo_read(r, b, f.data(), nullptr, nullptr, &s_b, nullptr, nullptr,
-                      (ApFT && p->n_cols > 0) ? ap.data() : nullptr, stepMods ? sm.data() : nullptr);
+                      want_ap ? ap.data() : nullptr, stepMods ? sm.data() : nullptr);
     if (st) break;
     for (size_t j = 0; j < 2 * n; j++) Et[2 * n * b + j] = (T)f[j];
-    if (ApFT && p->n_cols > 0)
+    if (want_ap)
       for (size_t j = 0; j < 2 * n * p->n_cols; j++) ApFT[2 * n * p->n_cols * b + j] = (T)ap[j];
     if (stepMods)
       for (int j = 0; j < p->nsteps; j++) stepMods[(size_t)b * p->nsteps + j] = (T)sm[j];
     if (status) status[b] = s_b;
   }
-  ofx_opo_destroy(r);
-  if (st) return st;
+  if (st) {
+    ofx_opo_destroy(ctx->pass_run);
+    ctx->pass_run = nullptr;
+    return st;
+  }
   return sync_st;
 }
 
@@ -999,3 +1094,4 @@ int ofx_crystal_pass_host_f64(ofx_context* ctx, const ofx_pass_params* p, int ba
 }
 
 }  // extern "C"
+

@@ -143,8 +143,11 @@ def nee_heun(Et, xT_h, G33, w0, bdiffw0, h, nSteps, dt, hBarg, maxErr, minErr, s
 
 
 def rk23(Et, xT_h, G, w0, bdiffw0, h, nSteps, dt, hBarg, maxErr, minErr, sel, n_cols=0,
-         want_stepmods=True):
-    """Bogacki-Shampine RK23 pass (OPO_CUDA_ODE23.cu control) in FP64."""
+         want_stepmods=True, allow_nonfinite=False):
+    """Bogacki-Shampine RK23 pass (OPO_CUDA_ODE23.cu control) in FP64.
+    Returns (Et, ApFT, stepMods, counters); with allow_nonfinite a pass whose
+    field turns NaN returns what it reached (counters count the attempt that
+    found it) instead of raising."""
     Et = _c128(Et).copy()
     n = Et.size
     ap = np.zeros((max(n_cols, 1), n), np.complex128)
@@ -154,7 +157,8 @@ def rk23(Et, xT_h, G, w0, bdiffw0, h, nSteps, dt, hBarg, maxErr, minErr, sel, n_
                         w0, bdiffw0, h, int(nSteps), dt, _dp(_f64(hBarg)), maxErr, minErr,
                         int(sel), ap.view(np.float64).ctypes.data_as(_D) if n_cols else None,
                         n_cols, _dp(sm), cnt.ctypes.data_as(_LL))
-    _check(st, "rk23")
+    if not (allow_nonfinite and st == 3):
+        _check(st, "rk23")
     return Et, ap[:n_cols].T.copy(), sm, cnt
 
 

@@ -46,3 +46,25 @@ def test_batch_cpu_baseline_on_fewer_cores_than_grid_sizes(monkeypatch):
     monkeypatch.setattr(bench.os, "sched_getaffinity", lambda pid: {0}, raising=False)
     r = bench.cpu_baseline_batch(insts, meta, [50.0, 50.0, 50.0], sample_steps=20)
     assert r["cores"] == 1 and r["value"] > 0
+
+
+def test_ring_cpu_baseline_is_one_oracle_round_trip_from_the_device_state():
+    """cavity_state inverts the pump addition and refraction, so the oracle
+    round trip it starts takes the device's crystal input; the baseline's
+    output spectrum equals the oracle's own for that trip."""
+    import oracle
+    from cases import oracle_next_input, rel_l2
+    from optifax_amd.physics import SimWindow
+    cfg, b = opo_case(n=256, window=2e-12, length=0.1e-3)
+    sw, si = b["simwin"], b["solver"]
+    R, T, phi = b["oc"].masks(sw)
+    o1 = oracle.opo_run(1, b["noise"], b["pump"], np.ones(sw.n), si["n_xtal"], R, T, phi,
+                        b["delay"], sw.rel_omegas, si)
+    X = oracle_next_input(b, o1["E_cav"])          # a device-style crystal input
+    assert rel_l2(oracle_next_input(b, bench.cavity_state(b, X)), X) < 1e-13
+    rec, oc, (att, acc) = bench.cpu_baseline(b, X)
+    o2 = oracle.opo_run(1, o1["E_cav"], b["pump"], np.ones(sw.n), si["n_xtal"], R, T, phi,
+                        b["delay"], sw.rel_omegas, si)
+    assert rel_l2(oc, SimWindow.fft_order(SimWindow.t2k(o2["E_oc"]))) < 1e-12
+    assert rec["kind"] == "port" and rec["cores"] == 1 and rec["value"] > 0
+    assert (att, acc) == tuple(int(v) for v in o2["steps"][0])

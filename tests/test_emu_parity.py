@@ -391,3 +391,27 @@ def test_plain_pass_is_not_a_round_trip(emu_ctx):
     h1, sm1 = run.read_history(0, 1)
     assert sm1.any() and np.isfinite(h1).all()
     run.close()
+
+
+def test_repeated_pass_calls_reuse_the_run(emu_ctx):
+    """OpticalSim.run calls the solver once per trip with the same shapes;
+    the context keeps the device run between such calls.  Results of a
+    sequence of calls (same shapes and new data, then another grid, then the
+    first shapes again) equal those of a fresh context per call."""
+    from optifax_amd.solver import Context
+    import os
+    ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    cases = []
+    for seed, n in ((1, 256), (2, 256), (3, 512), (4, 256)):
+        sw, si, E, args = crystal_case(n=n, window=2e-12 * n / 256, seed=seed)
+        G = args[1] * (1.0 + 0.1 * seed)      # new coupling per call
+        cases.append((E, (args[0], G) + args[2:], n))
+    got = []
+    for E, args, n in cases:
+        got.append(_run(emu_ctx, E, args, n, "double", 2))
+    for (E, args, n), g in zip(cases, got):
+        fresh = Context(lib_path=os.path.join(ROOT, "tests", "emu", "libofx_emu.so"))
+        want = _run(fresh, E, args, n, "double", 2)
+        fresh.close()
+        for a, b in zip(g, want):
+            assert np.array_equal(a, b)
