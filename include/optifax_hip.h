@@ -228,6 +228,27 @@ int ofx_mlp_train(ofx_mlp* m, int batch, int steps, const int* idx, float lr, fl
 /* device time of the last ofx_mlp_forward / ofx_mlp_train, milliseconds */
 double ofx_mlp_last_ms(const ofx_mlp* m);
 
+/* ---- QPM gain maps (phase-matching analysis) ------------------------------
+ * Synchronous calls on `device`; host arrays in and out; *ms (may be NULL)
+ * receives the kernel's device time in milliseconds.
+ *
+ * ofx_qpm_zsum: the sum over the crystal of qpmgain.m:140-149
+ *     (dAs_dz = g_coeff .* exp(1i*dk.*z) .* P;  As = sum(dAs_dz.*dz, 3)),
+ * without the prefactor g_coeff:
+ *     out[o] = sum_r wgt[r] sum_{q < len[r]} exp(i dk[o] (z0[r] + q dz)),
+ * o < n_out (complex, interleaved).  A run r stands for len[r] consecutive
+ * samples of spacing dz with one weight P(z) dz; runs of one sample give the
+ * literal sum over any sampling.  FP64. */
+int ofx_qpm_zsum(int device, int n_out, const double* dk, int n_run, const double* z0,
+                 const int* len, const double* wgt, double dz, double* out, double* ms);
+/* ofx_sinc_gain: the narrowband mismatch factor of sincgain.m (as
+ * sincgain_sparse.m:48-70 forms it, in consistent units):
+ *     gain[g * n_out + o] = sum_{j < n_m} (sinc((dk0[o] - m[j] kc[g]) length / 2) / m[j])^2,
+ * sinc(x) = sin(pi x) / (pi x) (MATLAB's), dk0 and kc in rad/m, length in m.
+ * FP64 with the argument reduced mod 2 pi. */
+int ofx_sinc_gain(int device, int n_out, const double* dk0, int n_g, const double* kc, int n_m,
+                  const double* m, double length, double* gain, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

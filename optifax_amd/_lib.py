@@ -53,6 +53,7 @@ EXPORTED = [
     "ofx_mlp_create", "ofx_mlp_destroy", "ofx_mlp_last_error", "ofx_mlp_num_params",
     "ofx_mlp_set_params", "ofx_mlp_get_params", "ofx_mlp_forward", "ofx_mlp_gradients",
     "ofx_mlp_step", "ofx_mlp_upload_dataset", "ofx_mlp_train", "ofx_mlp_last_ms",
+    "ofx_qpm_zsum", "ofx_sinc_gain",
 ]
 
 
@@ -144,6 +145,11 @@ def load(path: str | None = None):
         L.ofx_mlp_train.argtypes = [vp, ctypes.c_int, ctypes.c_int, _IP, ctypes.c_float, _FP]
         L.ofx_mlp_last_ms.argtypes = [vp]
         L.ofx_mlp_last_ms.restype = ctypes.c_double
+    if hasattr(L, "ofx_qpm_zsum"):     # nor the gain maps
+        L.ofx_qpm_zsum.argtypes = [ctypes.c_int, ctypes.c_int, _DP, ctypes.c_int, _DP, _IP, _DP,
+                                   ctypes.c_double, _DP, _DP]
+        L.ofx_sinc_gain.argtypes = [ctypes.c_int, ctypes.c_int, _DP, ctypes.c_int, _DP,
+                                    ctypes.c_int, _DP, ctypes.c_double, _DP, _DP]
     if path is None:
         _lib = L
     return L

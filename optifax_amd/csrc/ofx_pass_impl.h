@@ -684,9 +684,12 @@ OFX_DEV double dw_of(int k, int n, double dt) {
 // Forward transform of the own time columns held in `src` (layout lc*sT + r,
 // own columns lc in [HW, HW+W)) into the tagged buffer t1 (row-major
 // k2*n1 + c, twiddled).
+// Returns the buffer (src or other) that holds the transforms: data read
+// from the exchange may go only to the other one, since a thread can
+// complete its reads while another is still reading these.
 template <typename R>
-OFX_DEV void fwd_cols_to_t1(const Geo<R>& g, cpx<R>* src, cpx<R>* other, u64* t1,
-                            unsigned tag) {
+OFX_DEV cpx<R>* fwd_cols_to_t1(const Geo<R>& g, cpx<R>* src, cpx<R>* other, u64* t1,
+                               unsigned tag) {
   typedef cpx<R> C;
   typedef Tagged<C> TT;
   C* res = lds_fft<R, -1>(src + g.HW * g.sT, other + g.HW * g.sT, g.lg2, g.W, g.sT, g.tw2, 1);
@@ -699,6 +702,7 @@ OFX_DEV void fwd_cols_to_t1(const Geo<R>& g, cpx<R>* src, cpx<R>* other, u64* t1
     const TagPtr t = tp_plane<C>(g.plain, t1, g.n, (size_t)(k2 * g.n1 + c));
     TT::put(const_cast<u64*>(t.p), t.ws, t.plain, v, tag);
   }
+  return res - g.HW * g.sT;
 }
 
 // Own frequency rows from t1 into `dst` (layout k2l*sF + c), then length-n1
@@ -727,10 +731,11 @@ OFX_DEV cpx<R>* t1_to_rows_fwd(const Geo<R>& g, const u64* t1, unsigned tag, cpx
 template <typename R>
 OFX_DEV bool gather_err_and_rows(const ClusterDesc& cd, int par, const Geo<R>& g, const u64* t1,
                                  const u64* kx, cpx<R>* K1, unsigned tag, double* emax,
-                                 double* red, Spin& sp) {
+                                 double* red, Spin& sp, cpx<R>* rows, int rec_per_member) {
   typedef cpx<R> C;
   const int nw = slot_waves();
-  const int ns = cd.C * nw;
+  const int nrec = rec_per_member;  // records per member: one per wave, or one
+  const int ns = cd.C * nrec;
   const int nr = g.Rr * g.n1;
   const int nk = kx ? 6 * g.n2 : 0;
   const int n2 = g.n2;
@@ -753,7 +758,7 @@ OFX_DEV bool gather_err_and_rows(const ClusterDesc& cd, int par, const Geo<R>& g
   bool ok = read_tagged<C>(
       ns + nr + nk, tag,
       [&](int idx) -> TagPtr {
-        if (idx < ns) return TagPtr{slot_at<R>(cd, par, idx / nw, idx % nw, 0), 1, 0};
+        if (idx < ns) return TagPtr{slot_at<R>(cd, par, idx / nrec, idx % nrec, 0), 1, 0};
         idx -= ns;
         if (idx < nr)
           return tp_plane<C>(g.plain, t1, g.n, (size_t)((g.k20 + (idx >> g.lg1)) * g.n1 + (idx & (g.n1 - 1))));
@@ -766,7 +771,7 @@ OFX_DEV bool gather_err_and_rows(const ClusterDesc& cd, int par, const Geo<R>& g
         }
         idx -= ns;
         if (idx < nr) {
-          g.B1[(idx >> g.lg1) * g.sF + (idx & (g.n1 - 1))] = x;
+          rows[(idx >> g.lg1) * g.sF + (idx & (g.n1 - 1))] = x;
           return;
         }
         idx -= nr;
@@ -778,6 +783,7 @@ OFX_DEV bool gather_err_and_rows(const ClusterDesc& cd, int par, const Geo<R>& g
       sp, g.plain);
 #ifdef OFX_EMU
   (void)red;
+  (void)nw;
   *emax = v;
   return ok;
 #else
@@ -794,9 +800,10 @@ OFX_DEV bool gather_err_and_rows(const ClusterDesc& cd, int par, const Geo<R>& g
 
 // Inverse length-n1 transforms of the own rows, conjugate twiddle, write the
 // tagged buffer t2 (column-major c*n2 + k2).
+// Returns the buffer (src or other) holding the transforms (see fwd_cols_to_t1).
 template <typename R>
-OFX_DEV void rows_inv_to_t2(const Geo<R>& g, cpx<R>* src, cpx<R>* other, u64* t2,
-                            unsigned tag) {
+OFX_DEV cpx<R>* rows_inv_to_t2(const Geo<R>& g, cpx<R>* src, cpx<R>* other, u64* t2,
+                               unsigned tag) {
   typedef cpx<R> C;
   typedef Tagged<C> TT;
   C* res = lds_fft<R, +1>(src, other, g.lg1, g.Rr, g.sF, g.tw1, 1);
@@ -809,13 +816,17 @@ OFX_DEV void rows_inv_to_t2(const Geo<R>& g, cpx<R>* src, cpx<R>* other, u64* t2
     const TagPtr t = tp_plane<C>(g.plain, t2, g.n, (size_t)(c * g.n2 + k2));
     TT::put(const_cast<u64*>(t.p), t.ws, t.plain, v, tag);
   }
+  return res;
 }
 
 // Own+halo columns from t2, inverse length-n2 transforms, scale 1/N and
 // place into A0 (B0) with the row shift that wrapped halo columns carry.
 // Returns false (uniform over the workgroup) when the cluster gave up.
+// dst: a buffer no thread still reads (not the one the inverse rows left
+// their transforms in), other: the second work buffer.
 template <typename R>
-OFX_DEV bool t2_to_cols_inv(const Geo<R>& g, const u64* t2, unsigned tag, Spin& sp) {
+OFX_DEV bool t2_to_cols_inv(const Geo<R>& g, const u64* t2, unsigned tag, Spin& sp, cpx<R>* dst,
+                            cpx<R>* other) {
   typedef cpx<R> C;
   const int tot = g.LC * g.n2;
   bool ok = read_tagged<C>(
@@ -826,9 +837,9 @@ OFX_DEV bool t2_to_cols_inv(const Geo<R>& g, const u64* t2, unsigned tag, Spin& 
         gc = gc < 0 ? gc + g.n1 : (gc >= g.n1 ? gc - g.n1 : gc);
         return tp_plane<C>(g.plain, t2, g.n, (size_t)(gc * g.n2 + k2));
       },
-      [&](int idx, C v) { g.B1[(idx >> g.lg2) * g.sT + (idx & (g.n2 - 1))] = v; }, sp, g.plain);
+      [&](int idx, C v) { dst[(idx >> g.lg2) * g.sT + (idx & (g.n2 - 1))] = v; }, sp, g.plain);
   ok = wg_all(ok, g.plain);
-  C* res = lds_fft<R, +1>(g.B1, g.B2, g.lg2, g.LC, g.sT, g.tw2, 1);
+  C* res = lds_fft<R, +1>(dst, other, g.lg2, g.LC, g.sT, g.tw2, 1);
   const R inv = R(1) / R(g.n);
   for (int idx = OFX_TID; idx < tot; idx += OFX_NT) {
     const int lc = idx >> g.lg2, rp = idx & (g.n2 - 1);
@@ -973,7 +984,8 @@ OFX_DEV int rk23_pass(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc
       {
         Spin sp = spin_begin(cd, a.timeout_ticks);
         if (!xchg_ready<XM>(cd, sy, a.timeout_ticks) ||
-            !gather_err_and_rows<R>(cd, par, g, t1, kx, K1, tag1, &emax, red, sp))
+            !gather_err_and_rows<R>(cd, par, g, t1, kx, K1, tag1, &emax, red, sp, g.B1,
+                                    slot_waves()))
           return ST_BARRIER_TIMEOUT;
       }
       att++;
@@ -1023,9 +1035,10 @@ OFX_DEV int rk23_pass(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc
       {
         const unsigned x2 = ++sy.nx, tag2 = sy.base + x2;
         u64* t2 = (u64*)cd.t2 + (size_t)(x2 & 1u) * n * TT::NW;
-        rows_inv_to_t2<R>(g, F, Fo, t2, tag2);
+        C* ri = rows_inv_to_t2<R>(g, F, Fo, t2, tag2);
         Spin sp = spin_begin(cd, a.timeout_ticks);
-        if (!xchg_ready<XM>(cd, sy, a.timeout_ticks) || !t2_to_cols_inv<R>(g, t2, tag2, sp))
+        if (!xchg_ready<XM>(cd, sy, a.timeout_ticks) ||
+            !t2_to_cols_inv<R>(g, t2, tag2, sp, ri == F ? Fo : F, ri))
           return ST_BARRIER_TIMEOUT;
       }
       // step control after an accepted step (OPO_CUDA_ODE23.cu order)
@@ -1246,18 +1259,32 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
       };
       for (int idx = OFX_TID; idx < g.W * n2; idx += OFX_NT)
         corrector(idx, g.ab[idx]);
+      // the workgroup's error maximum (+inf for a non-finite field): each
+      // wave's before the barrier, then one record for the whole member, so
+      // the cluster reads C records instead of C times the waves
+      double wmax = bad ? INFINITY : fmin(lmax, DBL_MAX);
+#ifndef OFX_EMU
+      for (int o = 32; o > 0; o >>= 1) wmax = red_op(wmax, __shfl_xor(wmax, o), 0);
+      if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = wmax;
+#endif
       OFX_SYNC();
       // exchange: error records and the speculative forward column transforms of E
       const unsigned x1 = ++sy.nx, tag1 = sy.base + x1;
       const int par = (int)(x1 & 1u);
-      publish_slots<R>(cd, par, rank, 0, bad ? INFINITY : fmin(lmax, DBL_MAX), 0.0, 0.0, tag1);
-      fwd_cols_to_t1<R>(g, X, K, t1_of(x1), tag1);
+      if (OFX_TID == 0) {
+#ifndef OFX_EMU
+        for (int w = 1; w < slot_waves(); w++) wmax = red_op(wmax, red[w], 0);
+#endif
+        Tagged<C>::put(slot_at<R>(cd, par, rank, 0, 0), 1, 0, pack_d<R>(wmax), tag1);
+      }
+      C* cols = fwd_cols_to_t1<R>(g, X, K, t1_of(x1), tag1);
+      C* rows = cols == X ? K : X;
       double gmax;
       {
         Spin sp = spin_begin(cd, a.timeout_ticks);
         if (!xchg_ready<XM>(cd, sy, a.timeout_ticks) ||
             !gather_err_and_rows<R>(cd, par, g, t1_of(x1), nullptr, nullptr, tag1, &gmax, red,
-                                    sp)) {
+                                    sp, rows, 1)) {
           status = ST_BARRIER_TIMEOUT;
           break;
         }
@@ -1286,8 +1313,8 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
       int cend = chunk;  // chunks [chunk, cend) end with this step
       while ((double)cend <= nChunks && !(knew < cend * sel && knew <= nsteps)) cend++;
       // frequency phase: rows, dispersion, snapshots, inverse rows
-      C* F = lds_fft<R, -1>(g.B1, g.B2, g.lg1, g.Rr, g.sF, g.tw1, 1);
-      C* Fo = (F == g.B1) ? g.B2 : g.B1;
+      C* F = lds_fft<R, -1>(rows, cols, g.lg1, g.Rr, g.sF, g.tw1, 1);
+      C* Fo = (F == rows) ? cols : rows;
       auto disperse = [&](int idx, double beta, R al) {
         const int k2l = idx >> g.lg1, k1 = idx & (n1 - 1);
         const int kk = (g.k20 + k2l) + n2 * k1;
@@ -1314,9 +1341,10 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
       OFX_SYNC();
       {
         const unsigned x2 = ++sy.nx, tag2 = sy.base + x2;
-        rows_inv_to_t2<R>(g, F, Fo, t2_of(x2), tag2);
+        C* ri = rows_inv_to_t2<R>(g, F, Fo, t2_of(x2), tag2);
         Spin sp = spin_begin(cd, a.timeout_ticks);
-        if (!xchg_ready<XM>(cd, sy, a.timeout_ticks) || !t2_to_cols_inv<R>(g, t2_of(x2), tag2, sp)) {
+        if (!xchg_ready<XM>(cd, sy, a.timeout_ticks) ||
+            !t2_to_cols_inv<R>(g, t2_of(x2), tag2, sp, ri == F ? Fo : F, ri)) {
           status = ST_BARRIER_TIMEOUT;
           break;
         }
@@ -1340,19 +1368,20 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
     if (!cavity) continue;
     // ---------------- cavity: OC split, detuning, pump (one FFT pair) ----------------
     const unsigned xc1 = ++sy.nx, tagc1 = sy.base + xc1;
-    fwd_cols_to_t1<R>(g, g.B0, g.B1, t1_of(xc1), tagc1);
+    C* ccols = fwd_cols_to_t1<R>(g, g.B0, g.B1, t1_of(xc1), tagc1);
+    C* crows = ccols == g.B0 ? g.B1 : g.B0;
     {
       bool ok = xchg_ready<XM>(cd, sy, a.timeout_ticks);
-      C* F = g.B0;
+      C* F = crows;
       if (ok) {
         Spin sp = spin_begin(cd, a.timeout_ticks);
-        F = t1_to_rows_fwd<R>(g, t1_of(xc1), tagc1, g.B0, g.B1, sp, ok);
+        F = t1_to_rows_fwd<R>(g, t1_of(xc1), tagc1, crows, ccols, sp, ok);
       }
       if (!ok) {
         status = ST_BARRIER_TIMEOUT;
         break;
       }
-      C* Fo = (F == g.B0) ? g.B1 : g.B0;
+      C* Fo = (F == crows) ? ccols : crows;
       const C* pump = (const C*)d.pump;
       const C* cav = (const C*)d.cav;
       const C* ocm = (const C*)d.ocm;
@@ -1386,12 +1415,14 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
       const unsigned xc2 = ++sy.nx, tagc2 = sy.base + xc2;
       const int par = (int)(xc2 & 1u);
       publish_slots<R>(cd, par, rank, 1, acc[0], acc[1], acc[2], tagc2);
-      rows_inv_to_t2<R>(g, F, Fo, t2_of(xc2), tagc2);
+      C* cri = rows_inv_to_t2<R>(g, F, Fo, t2_of(xc2), tagc2);
       double tot[3];
       ok = xchg_ready<XM>(cd, sy, a.timeout_ticks);
       if (ok) {
         Spin sp = spin_begin(cd, a.timeout_ticks);
-        ok = t2_to_cols_inv<R>(g, t2_of(xc2), tagc2, sp);
+        // (the columns go to B0, so their transforms run in B1 and B2)
+        C* cd1 = cri == g.B1 ? g.B2 : g.B1;
+        ok = t2_to_cols_inv<R>(g, t2_of(xc2), tagc2, sp, cd1, cd1 == g.B1 ? g.B2 : g.B1);
       }
       if (!ok || !gather_sums<R>(cd, par, tagc2, tot, red, a.timeout_ticks)) {
         status = ST_BARRIER_TIMEOUT;

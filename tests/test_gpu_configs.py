@@ -64,8 +64,10 @@ def test_stated_single_pass_goes_nonfinite_at_the_oracles_step(gpu_ctx, method):
     assert r["accepted"] == int(cnt[1])
     assert r["attempts"] == int(cnt[0]) + (1 if method == "heun" else 0)
     if method == "rk23":
-        # the field keeps the last accepted step (finite)
-        assert np.all(np.isfinite(r["field"])) and rel_l2(r["field"], Eo) < 1e-6
+        # the field keeps the last accepted step: finite, ~1e142 V/m after
+        # the runaway growth, which amplifies the last-digit differences of
+        # the two FP64 computations (measured 1.5e-4 relative L2)
+        assert np.all(np.isfinite(r["field"])) and rel_l2(r["field"], Eo) < 1e-3
 
 
 _BATCH = {}
@@ -147,3 +149,87 @@ def test_repeated_mex_calls_reuse_the_run(gpu_ctx):
         fresh.close()
         for x, y in zip(g, want):
             assert np.array_equal(x, y)
+
+
+def _xtal_pass_case(xt, n=4096, window=8e-12, seed=3):
+    from optifax_amd import physics as ph
+    sw = ph.SimWindow(1040e-9, n, window)
+    si = xt.solver_inputs(sw)
+    E = np.fft.fftshift(ph.sech_pump(sw, 200e-15, 2e-10, 20e-12) + ph.quantum_noise(sw, 20e-12, seed))
+    args = (si["alpha_w"], si["G33"], si["w0"], si["bdiffw0"], si["h"], si["nSteps"], si["dt"],
+            si["beta_op"], 5e-2, 2.5e-3, float(np.ceil(si["nSteps"] / 4)))
+    return si, E, args
+
+
+def _crystals():
+    from optifax_amd import physics as ph
+    return {
+        "KTP, chirped 40->36 um": ph.NonlinearCrystal(
+            lambda z: ph.chirped_grating(z, 40e-6, 36e-6, 1.0, 0.1e-6), 0.4e-3,
+            chi2=np.pi * 14e-12, material="KTP"),
+        "OP-GaP": ph.NonlinearCrystal(60e-6, 0.3e-3, chi2=np.pi * 14e-12, material="OP-GaP"),
+        "PPLN fan-out 29-31 um at y=0.5": ph.NonlinearCrystal(
+            [29e-6, 31e-6], 0.4e-3, chi2=np.pi * 14e-12, vertical_position=0.5),
+        "PPLN stepped, grating 3": ph.NonlinearCrystal(
+            [28e-6, 29e-6, 30.5e-6, 32e-6], 0.4e-3, chi2=np.pi * 14e-12, vertical_position=2.5),
+        "PPLN 30 um, 1 um domain uncertainty": ph.NonlinearCrystal(
+            30e-6, 0.4e-3, chi2=np.pi * 14e-12, uncertainty=1e-6, seed=7),
+    }
+
+
+@pytest.mark.parametrize("name", list(_crystals()))
+def test_pass_other_crystals_fp64_and_fp32(gpu_ctx, name):
+    """Solver inputs of the other materials and grating forms (sellmeier.m,
+    NonlinearCrystal.m:84-139, chirpedgrating.m, QPMcrystal.m) through the
+    HIP kernel at 2^12 points: FP64 fields reproduce the oracle's step
+    sequence and field; FP32 follows the same steps."""
+    si, E, args = _xtal_pass_case(_crystals()[name])
+    Eo, apo, smo, cnt = oracle.nee_heun(E, *args, n_cols=4)
+    Eg, apg, smg = OPOmexBatch(E, *args, np.zeros((E.size, 4), complex), np.zeros(args[5]),
+                               ctx=gpu_ctx, precision="double")
+    assert np.array_equal(smg, smo)
+    assert rel_l2(Eg, Eo) < 1e-9 and rel_l2(apg, apo) < 1e-9
+    Ef, _, smf = OPOmexBatch(E, *args, np.zeros((E.size, 4), complex), np.zeros(args[5]),
+                             ctx=gpu_ctx)
+    print(f"{name}: {int(cnt[0])} attempts, FP32 rel L2 {rel_l2(Ef, Eo):.2e}")
+    assert np.array_equal(smf, smo) and rel_l2(Ef, Eo) < 2e-4
+
+
+def test_round_trips_with_an_optic_chain_fp64(gpu_ctx):
+    """A cavity with an optic chain (Cavity.m:50-87): a 1 mm undoped LN
+    window with a measured -150 fs^2 coating on both faces and 10 % loss
+    before the coupler, a 99 % mirror after it, folded into the device
+    operators, against the literal OpticalSim.run sequence with the same
+    optics in the coupler masks; 2^13 points, 0.4 mm PPLN, three trips."""
+    from optifax_amd import physics as ph
+    cfg = ph.OPOConfig(n=2 ** 13, window=6e-12, length=0.4e-3, pump_energy=2e-10,
+                       area=20e-12, delay=20e-15, noise_seed=5)
+    b = cfg.build()
+    si, sw = b["solver"], b["simwin"]
+    lam = np.linspace(500.0, 2500.0, 81)
+    gdd = (lam, np.full(lam.size, -150.0))
+    t_w, _, phi_w = ph.optic_response(sw, "T", 0.9 ** 0.5, "PPLN_undoped", 1e-3, gdd=gdd)
+    window = ("T", (t_w, None, np.nan_to_num(phi_w)))
+    mirror = ("R", ph.optic_response(sw, "R", 0.01, "PPLN_undoped", 1e-3))
+    T_b, phi_b = ph.cavity_response([window])
+    T_a, phi_a = ph.cavity_response([mirror])
+    ops = ph.cavity_operators(sw, si["n_xtal"], b["oc"], b["pump"], b["noise"],
+                              chain_before_oc=(T_b, phi_b), chain_after_oc=(T_a, phi_a))
+    R, T, phi = b["oc"].masks(sw)
+    trips = 3
+    # literal order: the optics before the coupler act on both coupler
+    # outputs, the mirror after it on the fed-back field only (a mirror in
+    # reflection adds no phase, so one coupler phase serves both outputs)
+    assert not np.any(phi_a)
+    o = oracle.opo_run(trips, b["noise"], b["pump"], np.ones(sw.n), si["n_xtal"], R * T_b * T_a,
+                       T * T_b, phi + phi_b, cfg.delay, sw.rel_omegas, si)
+    run = OPOBatch([dict(solver=si, field0=ops["field0"], pump=ops["pump"], cav=ops["cav"],
+                         ocm=ops["ocm"], delay=cfg.delay)], precision="double", trip_cap=trips,
+                   ctx=gpu_ctx)
+    run.launch(trips)
+    run.sync()
+    r = run.read(0)
+    run.close()
+    assert np.array_equal(r["trip_log"][:, 2:].astype(int), o["steps"])
+    assert rel_l2(r["field"], oracle_next_input(b, o["E_cav"])) < 1e-9
+    assert rel_l2(r["oc_spec"], SimWindow.fft_order(SimWindow.t2k(o["E_oc"]))) < 1e-9
