@@ -110,6 +110,33 @@ int ofx_crystal_pass_host(ofx_context* ctx, const ofx_pass_params* p, int batch,
                           const float* beta_op, float* ApFT, float* stepMods, int* status,
                           const ofx_tuning* tune);
 
+/*
+ * The same pass with no allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.
+ *
+ * Every array is a DEVICE pointer (float complex64 fields and float tables,
+ * laid out as for ofx_crystal_pass_host) and the pass is computed in place:
+ * Et, ApFT and stepMods are overwritten where the pass writes them, status
+ * (or NULL) receives OFX_STATUS_* per instance.  The inputs must be complete
+ * on the device before the call; the call runs on the context's stream and
+ * returns when the pass is done, with the same return codes as
+ * ofx_crystal_pass_host.  Default launch geometry (no tuning argument).
+ *
+ * Exchange buffers, descriptors and tables live in one device array of
+ * OFX_NOALLOC_SCRATCH_BYTES that the library holds on every device it runs
+ * on until the process ends; the call allocates nothing (device or host).
+ * A configuration whose buffers do not fit the array (roughly N*batch up to
+ * 2^17 with Heun, less with RK23 or long crystals), or batch > 16, is
+ * rejected with OFX_E_ARG and a message.  Calls in one process are
+ * serialized by the library (they share the array, which keeps the last
+ * call's contents; every call clears what it needs cleared).
+ */
+#define OFX_NOALLOC_SCRATCH_BYTES (16u << 20)
+int ofx_crystal_pass_noalloc(ofx_context* ctx, const ofx_pass_params* p, int batch, float* Et,
+                             const float* alpha_w, const float* G, int g_batch,
+                             const float* beta_op, float* ApFT, float* stepMods, int* status);
+
 /* Same, FP64 fields (complex128) and FP64 tables; used for exact parity. */
 int ofx_crystal_pass_host_f64(ofx_context* ctx, const ofx_pass_params* p, int batch, double* Et,
                               const double* alpha_w, const double* G, int g_batch,

@@ -44,7 +44,7 @@ _ERRNAMES = {
 
 EXPORTED = [
     "ofx_create", "ofx_destroy", "ofx_last_error", "ofx_device_info",
-    "ofx_crystal_pass_host", "ofx_crystal_pass_host_f64",
+    "ofx_crystal_pass_host", "ofx_crystal_pass_host_f64", "ofx_crystal_pass_noalloc",
     "ofx_opo_create", "ofx_opo_destroy", "ofx_opo_launch", "ofx_opo_sync",
     "ofx_opo_graph_replay", "ofx_opo_last_ms", "ofx_opo_read", "ofx_opo_write_field",
     "ofx_opo_geometry", "ofx_opo_xcc_ids", "ofx_opo_set_wait_limit",
@@ -111,6 +111,9 @@ def load(path: str | None = None):
     L.ofx_crystal_pass_host_f64.argtypes = [vp, ctypes.POINTER(PassParams), ctypes.c_int, _DP, _DP,
                                             _DP, ctypes.c_int, _DP, _DP, _DP, _IP,
                                             ctypes.POINTER(Tuning)]
+    if hasattr(L, "ofx_crystal_pass_noalloc"):   # device pointers (plain addresses)
+        L.ofx_crystal_pass_noalloc.argtypes = [vp, ctypes.POINTER(PassParams), ctypes.c_int, vp,
+                                               vp, vp, ctypes.c_int, vp, vp, vp, vp]
     L.ofx_opo_create.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OpoSpec),
                                  ctypes.c_int, ctypes.POINTER(Tuning), ctypes.POINTER(vp)]
     L.ofx_opo_destroy.argtypes = [vp]

@@ -40,5 +40,14 @@ int be_launch(Backend* b, const LaunchArgs& a, int precision, int xmode, int gri
               unsigned* epochs, int n_epochs, const void* key, int replays);
 int be_last_ms(Backend* b, double* ms);
 void be_forget_graph(Backend* b, const void* key);
+// The library's fixed device array for the no-allocation entry point: its
+// device address on this backend's device (looked up once per device when
+// the first backend of that device is created) and its size in bytes.
+int be_scratch(Backend* b, unsigned char** ptr, size_t* bytes);
+// stream-ordered: zero `bytes` (a multiple of 4) at device address p with a
+// fill kernel; dst[i] = (double)src[i] for device arrays; device-to-device copy
+int be_fill_zero(Backend* b, void* p, size_t bytes);
+int be_widen(Backend* b, double* dst, const float* src, size_t n);
+int be_d2d(Backend* b, void* dst, const void* src, size_t bytes);
 
 }  // namespace ofx

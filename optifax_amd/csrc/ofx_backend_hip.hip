@@ -5,7 +5,9 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
+#include <mutex>
 #include <string>
 
 #include "../../include/optifax_hip.h"
@@ -49,8 +51,33 @@ __global__ void zero_kernel(ZeroList z) {
   }
 }
 
+// Device array of the no-allocation entry point (ofx_crystal_pass_noalloc):
+// exchange buffers, descriptors and tables of its launches.  Each device
+// holds its own copy until the process ends.  Host code never uses this
+// name as an address: it is a host-side stand-in for the device symbol, and
+// the per-device address from hipGetSymbolAddress (g_scratch) is used
+// instead.
+static __device__ __align__(256) unsigned char scratch[OFX_NOALLOC_SCRATCH_BYTES];
+
+#define OFX_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch[OFX_MAX_DEVICES];  // device address per device, or null
+
+__global__ void fill_zero_kernel(unsigned* p, size_t words) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+__global__ void widen_kernel(double* dst, const float* src, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = (double)src[i];
+}
+
 struct Backend {
   int device = 0;
+  unsigned char* scratch = nullptr;  // this device's address of `scratch`
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool timed = false;
@@ -120,7 +147,62 @@ int be_create(int device, Backend** out, DeviceInfo* info) {
   }
   snprintf(info->name + strlen(info->name), sizeof info->name - strlen(info->name), " lds=%d",
            info->lds_max);
+  // The no-allocation entry point's array: its address on this device, once
+  // per device (with the device current), and one clearing of its contents.
+  if (device >= OFX_MAX_DEVICES) {
+    be_destroy(b);
+    return OFX_E_UNSUPPORTED;
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[device]) {
+      void* p = nullptr;
+      hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e == hipSuccess) {
+        hipLaunchKernelGGL(fill_zero_kernel, dim3(1024), dim3(256), 0, b->stream, (unsigned*)p,
+                           (size_t)OFX_NOALLOC_SCRATCH_BYTES / 4);
+        e = hipGetLastError();
+      }
+      if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+      if (e != hipSuccess) {
+        be_destroy(b);
+        return OFX_E_HIP;
+      }
+      g_scratch[device] = (unsigned char*)p;
+    }
+    b->scratch = g_scratch[device];
+  }
   *out = b;
+  return OFX_OK;
+}
+
+int be_scratch(Backend* b, unsigned char** ptr, size_t* bytes) {
+  if (!b || !b->scratch) return OFX_E_UNSUPPORTED;
+  *ptr = b->scratch;
+  *bytes = (size_t)OFX_NOALLOC_SCRATCH_BYTES;
+  return OFX_OK;
+}
+
+int be_fill_zero(Backend* b, void* p, size_t bytes) {
+  const size_t words = bytes / 4;
+  if (!words) return OFX_OK;
+  const size_t blocks = std::min<size_t>((words + 255) / 256, 2048);
+  hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)blocks), dim3(256), 0, b->stream,
+                     (unsigned*)p, words);
+  HIPCHK(b, hipGetLastError());
+  return OFX_OK;
+}
+
+int be_widen(Backend* b, double* dst, const float* src, size_t n) {
+  if (!n) return OFX_OK;
+  const size_t blocks = std::min<size_t>((n + 255) / 256, 2048);
+  hipLaunchKernelGGL(widen_kernel, dim3((unsigned)blocks), dim3(256), 0, b->stream, dst, src, n);
+  HIPCHK(b, hipGetLastError());
+  return OFX_OK;
+}
+
+int be_d2d(Backend* b, void* dst, const void* src, size_t bytes) {
+  HIPCHK(b, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, b->stream));
   return OFX_OK;
 }
 

@@ -1095,3 +1095,239 @@ int ofx_crystal_pass_host_f64(ofx_context* ctx, const ofx_pass_params* p, int ba
 
 }  // extern "C"
 
+// ---------------- single pass without allocation ----------------
+// The library's fixed device array (be_scratch) holds everything the pass
+// needs besides the caller's arrays.  Layout per call, each block 256-byte
+// aligned: the tag epoch pair (always first, so the tag base only grows
+// from call to call), the per-instance state words, the descriptors, the
+// clusters' barrier words and exchange buffers (cleared every call, so no
+// word left by an earlier call of another shape can carry an awaited tag),
+// then the tables.  Host staging lives in file-scope arrays; a mutex
+// serializes the calls, which share both.
+namespace {
+
+constexpr int NA_MAX_BATCH = 16;
+constexpr int NA_MAX_N = 65536;
+constexpr int NA_MAX_GRID = 4096;
+constexpr int NA_MAX_XCD = 64;
+
+std::mutex g_na_mu;
+double na_tw64[2 * NA_MAX_N], na_wt64[2 * NA_MAX_N], na_ab64[NA_MAX_N];
+float na_tw32[2 * NA_MAX_N], na_wt32[2 * NA_MAX_N], na_ab32[NA_MAX_N];
+InstDesc na_inst[NA_MAX_BATCH];
+ClusterDesc na_clus[NA_MAX_BATCH];
+int na_order[NA_MAX_BATCH], na_status[NA_MAX_BATCH];
+int na_wgc[NA_MAX_GRID], na_wgr[NA_MAX_GRID];
+
+size_t na_take(size_t& off, size_t bytes) {
+  const size_t o = off;
+  off = (off + bytes + 255) & ~(size_t)255;
+  return o;
+}
+
+}  // namespace
+
+extern "C" int ofx_crystal_pass_noalloc(ofx_context* ctx, const ofx_pass_params* p, int batch,
+                                        float* Et, const float* alpha_w, const float* G,
+                                        int g_batch, const float* beta_op, float* ApFT,
+                                        float* stepMods, int* status) {
+  if (!ctx) return OFX_E_ARG;
+  int st = check_pass(ctx, p);
+  if (st) return st;
+  if (batch == 0) return OFX_OK;
+  if (batch < 0 || !Et || !alpha_w || !G || !beta_op)
+    return fail(ctx, OFX_E_ARG, "Et, alpha_w, G, beta_op are required and batch >= 0");
+  if (g_batch != 1 && g_batch != batch)
+    return fail(ctx, OFX_E_ARG, "G must hold NSteps or NSteps*batch values (g_batch=%d)", g_batch);
+  if (batch > NA_MAX_BATCH)
+    return fail(ctx, OFX_E_ARG, "the no-allocation pass takes at most %d instances (got %d)",
+                NA_MAX_BATCH, batch);
+  std::lock_guard<std::mutex> lock(g_na_mu);
+  unsigned char* base = nullptr;
+  size_t cap = 0;
+  if (be_scratch(ctx->be, &base, &cap))
+    return fail(ctx, OFX_E_UNSUPPORTED, "no device array for the no-allocation pass");
+  const int n = p->n, nsteps = p->nsteps, method = p->method;
+  const int n_cols = ApFT ? p->n_cols : 0;
+
+  // geometry: as ofx_opo_create plans a batch of equal instances (one
+  // cluster per instance, all resident at once)
+  int C = 0, n1 = 0, block = 0;
+  st = plan_instance(ctx, *p, batch, nullptr, 0, 0, 8, &C, &n1);
+  if (!st) st = plan_block(ctx, nullptr, C, &block);
+  if (st) return st;
+  const int n2 = n / n1;
+  const int xmode = exchange_mode(0, C);
+  const size_t lds = lds_bytes_for(n1, n2, C, method, 8);
+  const int bpc = be_blocks_per_cu(ctx->be, OFX_F32, xmode, block, lds);
+  if (bpc < 1)
+    return fail(ctx, OFX_E_ARG, "kernel cannot launch with block=%d and %zu bytes of LDS", block, lds);
+  const int slots = ctx->info.cus * bpc;
+  if ((long long)batch * C > slots || slots > NA_MAX_GRID)
+    return fail(ctx, OFX_E_ARG, "%d instances of %d workgroups do not fit the device at once", batch,
+                C);
+  // XCD-aware placement: workgroup id g runs on XCD g % nx; a cluster goes
+  // to the XCD with most free slots, spread only when none has room
+  const int nx = std::max(1, std::min(ctx->info.nxcd, NA_MAX_XCD));
+  const int per_x = slots / nx;
+  int freex[NA_MAX_XCD];
+  for (int x = 0; x < nx; x++) freex[x] = per_x;
+  for (int g = 0; g < slots; g++) na_wgc[g] = -1, na_wgr[g] = 0;
+  int maxid = -1;
+  auto fullest = [&]() {
+    int best = 0;
+    for (int x = 1; x < nx; x++)
+      if (freex[x] > freex[best]) best = x;
+    return best;
+  };
+  for (int c = 0; c < batch; c++) {
+    int x = fullest();
+    const bool one = freex[x] >= C;
+    for (int q = 0; q < C; q++) {
+      if (!one) x = fullest();
+      const int id = x + nx * (per_x - freex[x]);
+      freex[x]--;
+      na_wgc[id] = c;
+      na_wgr[id] = q;
+      maxid = std::max(maxid, id);
+    }
+  }
+  const int grid = maxid + 1;
+
+  // layout of the device array
+  size_t off = 0;
+  const size_t o_epoch = na_take(off, 8);
+  const size_t o_clear = off;  // from here to o_tables: cleared every call
+  const size_t o_state = na_take(off, 32 * (size_t)batch);
+  const size_t o_inst = na_take(off, sizeof(InstDesc) * batch);
+  const size_t o_clus = na_take(off, sizeof(ClusterDesc) * batch);
+  const size_t o_order = na_take(off, 4 * (size_t)batch);
+  const size_t o_wgc = na_take(off, 4 * (size_t)grid);
+  const size_t o_wgr = na_take(off, 4 * (size_t)grid);
+  const size_t o_xid = na_take(off, 4 * (size_t)grid);
+  const size_t o_sync = na_take(off, 128 * (size_t)batch);
+  size_t o_t1[NA_MAX_BATCH], o_t2[NA_MAX_BATCH], o_slots[NA_MAX_BATCH], o_xcc[NA_MAX_BATCH],
+      o_kx[NA_MAX_BATCH];
+  const size_t tsz = 16;  // one tagged complex64 sample
+  for (int c = 0; c < batch; c++) {
+    o_t1[c] = na_take(off, 2 * (size_t)n * tsz);
+    o_t2[c] = na_take(off, 2 * (size_t)n * tsz);
+    o_slots[c] = na_take(off, 2 * 8 * 4 * (size_t)C * tsz);
+    o_xcc[c] = na_take(off, 4 * (size_t)C);
+    o_kx[c] = method == OFX_METHOD_RK23 ? na_take(off, 2 * (size_t)C * 6 * (size_t)n2 * tsz) : 0;
+  }
+  const size_t o_tables = off;
+  const size_t o_tw = na_take(off, 8 * (size_t)n);
+  const size_t o_wt = na_take(off, 8 * (size_t)n);
+  const size_t o_ab = na_take(off, 4 * (size_t)n);
+  const size_t o_beta = na_take(off, 8 * (size_t)n);
+  const size_t o_alpha = na_take(off, 8 * (size_t)n);
+  const size_t o_G = na_take(off, 8 * (size_t)nsteps * (size_t)g_batch);
+  if (off > cap)
+    return fail(ctx, OFX_E_ARG,
+                "N=%d, NSteps=%d, batch=%d needs %zu bytes of device scratch; the no-allocation "
+                "pass holds %zu",
+                n, nsteps, batch, off, cap);
+
+  // tables on the host (the same arithmetic as ofx_opo_create), FP32
+  fill_twiddle(n, na_tw64);
+  time_tables(n, p->dt, p->w0, na_wt64, na_ab64);
+  for (int j = 0; j < 2 * n; j++) {
+    na_tw32[j] = (float)na_tw64[j];
+    na_wt32[j] = (float)na_wt64[j];
+  }
+  for (int j = 0; j < n; j++) na_ab32[j] = (float)na_ab64[j];
+
+  unsigned char* const S = base;
+  int* const st_d = (int*)(S + o_state);
+  for (int b = 0; b < batch; b++) {
+    InstDesc& d = na_inst[b];
+    memset(&d, 0, sizeof d);
+    d.n = n;
+    d.n1 = n1;
+    d.n2 = n2;
+    d.lg1 = ilog2i(n1);
+    d.lg2 = ilog2i(n2);
+    d.nsteps = nsteps;
+    d.n_cols = n_cols;
+    d.method = method;
+    d.w0 = p->w0;
+    d.bdiffw0 = p->bdiffw0;
+    d.h = p->h;
+    d.dt = p->dt;
+    d.max_err = p->max_err;
+    d.min_err = p->min_err;
+    d.sel = p->sel;
+    d.beta = (const double*)(S + o_beta);
+    d.alpha = (const double*)(S + o_alpha);
+    d.G = (const double*)(S + o_G) + (g_batch == 1 ? 0 : (size_t)b * nsteps);
+    d.wt = S + o_wt;
+    d.absorb = S + o_ab;
+    d.field = Et + 2 * (size_t)n * b;
+    d.apft = n_cols ? (void*)(ApFT + 2 * (size_t)n * n_cols * b) : nullptr;
+    d.stepmods = stepMods ? stepMods + (size_t)nsteps * b : nullptr;
+    d.status = st_d + b;
+    d.trips_done = st_d + batch + b;
+    d.fresh = st_d + 2 * batch + b;
+    d.counters = (long long*)(S + o_state + 16 * (size_t)batch) + 2 * b;
+    ClusterDesc& cd = na_clus[b];
+    memset(&cd, 0, sizeof cd);
+    cd.C = C;
+    cd.first_inst = b;
+    cd.n_inst = 1;
+    cd.halo = halo_of(method);
+    cd.t1 = S + o_t1[b];
+    cd.t2 = S + o_t2[b];
+    cd.slots = S + o_slots[b];
+    cd.bar = (unsigned*)(S + o_sync + 128 * (size_t)b);
+    cd.abort_flag = (int*)(S + o_sync + 128 * (size_t)b + 64);
+    cd.xcc = (int*)(S + o_xcc[b]);
+    cd.kx = method == OFX_METHOD_RK23 ? S + o_kx[b] : nullptr;
+    cd.epoch = (unsigned*)(S + o_epoch);  // one tag base for every cluster
+    na_order[b] = b;
+  }
+
+  Backend* be = ctx->be;
+  auto hip_err = [&](const char* what) {
+    return fail(ctx, OFX_E_HIP, "%s failed: %s", what, be_error(be));
+  };
+  if (be_fill_zero(be, S + o_clear, o_tables - o_clear)) return hip_err("clearing the scratch");
+  if (be_h2d(be, S + o_inst, na_inst, sizeof(InstDesc) * batch) ||
+      be_h2d(be, S + o_clus, na_clus, sizeof(ClusterDesc) * batch) ||
+      be_h2d(be, S + o_order, na_order, 4 * (size_t)batch) ||
+      be_h2d(be, S + o_wgc, na_wgc, 4 * (size_t)grid) ||
+      be_h2d(be, S + o_wgr, na_wgr, 4 * (size_t)grid) ||
+      be_h2d(be, S + o_tw, na_tw32, 8 * (size_t)n) ||
+      be_h2d(be, S + o_wt, na_wt32, 8 * (size_t)n) ||
+      be_h2d(be, S + o_ab, na_ab32, 4 * (size_t)n))
+    return hip_err("upload");
+  if (be_widen(be, (double*)(S + o_beta), beta_op, (size_t)n) ||
+      be_widen(be, (double*)(S + o_alpha), alpha_w, (size_t)n) ||
+      be_widen(be, (double*)(S + o_G), G, (size_t)nsteps * g_batch))
+    return hip_err("table conversion");
+  LaunchArgs a;
+  memset(&a, 0, sizeof a);
+  a.inst = (const InstDesc*)(S + o_inst);
+  a.clusters = (const ClusterDesc*)(S + o_clus);
+  a.inst_order = (const int*)(S + o_order);
+  a.wg_cluster = (const int*)(S + o_wgc);
+  a.wg_rank = (const int*)(S + o_wgr);
+  a.twiddle = S + o_tw;
+  a.nmax = n;
+  a.trips = 0;
+  a.conv_tol = 0.0;
+  a.timeout_ticks = (unsigned long long)((double)ctx->info.wallclock_khz * 1000.0 * 10.0);
+  a.xcc_ids = (int*)(S + o_xid);
+  if (be_launch(be, a, OFX_F32, xmode, grid, block, lds, nullptr, nullptr, 0,
+                (unsigned*)(S + o_epoch), 1, nullptr, 0))
+    return hip_err("launch");
+  if (status && be_d2d(be, status, st_d, 4 * (size_t)batch)) return hip_err("status copy");
+  if (be_d2h(be, na_status, st_d, 4 * (size_t)batch) || be_sync(be)) return hip_err("pass");
+  for (int b = 0; b < batch; b++)
+    if (na_status[b] == ST_BARRIER_TIMEOUT)
+      return fail(ctx, OFX_E_TIMEOUT, "instance %d: a cluster member timed out waiting for exchange data", b);
+  for (int b = 0; b < batch; b++)
+    if (na_status[b] == ST_NONFINITE)
+      return fail(ctx, OFX_E_NONFINITE, "instance %d: field became non-finite", b);
+  return OFX_OK;
+}

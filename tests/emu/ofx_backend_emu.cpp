@@ -116,4 +116,18 @@ int be_launch(Backend* b, const LaunchArgs& a, int precision, int xmode, int gri
 int be_last_ms(Backend* b, double* ms) { *ms = b->last_ms; return OFX_OK; }
 void be_forget_graph(Backend*, const void*) {}
 
+// the no-allocation entry point's array, in host memory here
+alignas(256) static unsigned char g_emu_scratch[OFX_NOALLOC_SCRATCH_BYTES];
+int be_scratch(Backend*, unsigned char** ptr, size_t* bytes) {
+  *ptr = g_emu_scratch;
+  *bytes = sizeof g_emu_scratch;
+  return OFX_OK;
+}
+int be_fill_zero(Backend*, void* p, size_t bytes) { memset(p, 0, bytes); return OFX_OK; }
+int be_widen(Backend*, double* dst, const float* src, size_t n) {
+  for (size_t i = 0; i < n; i++) dst[i] = (double)src[i];
+  return OFX_OK;
+}
+int be_d2d(Backend*, void* dst, const void* src, size_t bytes) { memcpy(dst, src, bytes); return OFX_OK; }
+
 }  // namespace ofx
