@@ -415,6 +415,64 @@ def main_pass(args):
     print(json.dumps(out))
 
 
+def main_mex(args):
+    """The MEX drop-in as OpticalSim.run drives it: one OPOmexBatch call per
+    round trip (host arrays in and out, OpticalSim.m:237-252) on the ring
+    workload's crystal pass (N=2^15, 5 mm, FP32, ApFT with 4 columns,
+    stepMods).  The context keeps the device run between calls with the same
+    shapes, so only the first call plans and allocates.  Reported: the first
+    call's wall time, the later calls' mean wall time, the device time of
+    the same pass (HIP events around one launch of the kept run) and their
+    difference, the per-call host overhead (uploads, readbacks, conversions)."""
+    from optifax_amd.solver import Context, OPOBatch, OPOmexBatch
+    cfg, b = workload()
+    si = b["solver"]
+    n = si["beta_op"].size
+    E = b["ops"]["field0"]
+    args_ = (si["alpha_w"], si["G33"], si["w0"], si["bdiffw0"], si["h"], si["nSteps"], si["dt"],
+             si["beta_op"], si["maxErr"], si["minErr"], si["sel"])
+    ctx = Context(0)
+    walls = []
+    for it in range(args.warmup + args.steps + 1):
+        t0 = time.perf_counter()
+        E, _, _ = OPOmexBatch(E, *args_, np.zeros((n, 4), complex), np.zeros(si["nSteps"]),
+                              ctx=ctx)
+        walls.append(time.perf_counter() - t0)
+    # device time of the same pass: one launch of a run kept on the device
+    run = OPOBatch([dict(solver=dict(si, n_cols=4), field0=E)], ctx=ctx)
+    run.launch(0)
+    run.sync()
+    dev = []
+    for _ in range(3):
+        run.write_field(0, E)
+        run.launch(0)
+        run.sync()
+        dev.append(run.last_ms())
+    run.close()
+    ctx.close()
+    warm = walls[1 + args.warmup:]
+    out = {
+        "metric": "OPOmexBatch calls/s (one call per round trip)",
+        "value": 1.0 / float(np.mean(warm)),
+        "unit": "calls/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * float(np.mean(warm)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (the ring workload's noise start, each call's output the next input)",
+        "config": {"workload": "OPOmexBatch crystal pass of the ring workload (N=2^15, 5 mm PPLN, "
+                               "ApFT 4 columns, stepMods), host arrays",
+                   "first_call_ms": 1e3 * walls[0],
+                   "device_pass_ms": float(np.mean(dev)),
+                   "host_overhead_ms_per_call": 1e3 * float(np.mean(warm)) - float(np.mean(dev))},
+    }
+    print(json.dumps(out))
+
+
 def main_surrogate(args):
     """The surrogate's shape: 2050 -> 2048 x3 -> 2048 MLP, batch 512, Adam, on a
     synthetic device-resident set of 8192 pairs; steps = Adam steps."""
@@ -585,7 +643,8 @@ def main():
     ap.add_argument("--esd-trips", type=int, default=300,
                     help="round trips of each run of the esd workload")
     ap.add_argument("--workload", default="ring",
-                    choices=["ring", "batch", "warmstart", "surrogate", "steady", "esd", "pass"],
+                    choices=["ring", "batch", "warmstart", "surrogate", "steady", "esd", "pass",
+                             "mex"],
                     help="ring: one synchronously pumped ring OPO (the headline line); batch: 256 uneven "
                          "instances in one persistent launch")
     ap.add_argument("--tune", default="heuristic", choices=["heuristic", "auto"])
@@ -619,6 +678,8 @@ def main():
         return main_esd(args)
     if args.workload == "pass":
         return main_pass(args)
+    if args.workload == "mex":
+        return main_mex(args)
     from optifax_amd.solver import Context, OPOBatch
     cfg, b = workload()
     si, ops = b["solver"], b["ops"]
