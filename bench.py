@@ -211,10 +211,12 @@ def main_warmstart(args):
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
-def mlp_step_flops(batch, n_in_padded, width, n_out):
+def mlp_step_flops(batch, n_in, width, n_out):
     """Algorithmic flops of one training step: forward, weight gradients, and
-    the data gradients of layers 1-3 (the input needs none)."""
-    fwd = 2.0 * batch * (n_in_padded * width + 2 * width * width + width * n_out)
+    the data gradients of layers 1-3 (the input needs none).  n_in is the
+    network's input width (the device pads it to 128 columns; the padding is
+    not counted)."""
+    fwd = 2.0 * batch * (n_in * width + 2 * width * width + width * n_out)
     dgrad = 2.0 * batch * (2 * width * width + width * n_out)
     return 2.0 * fwd + dgrad
 
@@ -488,7 +490,7 @@ def main_surrogate(args):
     t0 = time.perf_counter()
     losses, ms = net.train(args.steps, batch=batch)
     wall = time.perf_counter() - t0
-    flops = mlp_step_flops(batch, (n_in + 63) // 64 * 64, width, n_out)
+    flops = mlp_step_flops(batch, n_in, width, n_out)
     tflops = flops / (ms / args.steps * 1e-3) / 1e12
     out = {
         "metric": METRIC,

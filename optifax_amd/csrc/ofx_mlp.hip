@@ -24,6 +24,11 @@
 // the slices two steps ahead in flight during the current slice's MFMAs.
 // For the 512-row batch and width 2048, a layer is 8 x 32 = 256 workgroups:
 // one per CU.
+//
+// Training step: the weight-gradient GEMM's epilogue applies Adam to the
+// FP32 master weights and writes both BF16 copies, so the gradient itself
+// never reaches memory; the bias gradients and their Adam step are one small
+// kernel per layer.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -41,7 +46,7 @@ typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-enum { EPI_GELU = 0, EPI_BIAS = 1, EPI_DGELU = 2, EPI_F32 = 3 };
+enum { EPI_GELU = 0, EPI_BIAS = 1, EPI_DGELU = 2, EPI_F32 = 3, EPI_ADAM = 4 };
 
 constexpr int TM = 64, TN = 64, TK = 64;
 constexpr int LDK = TK + 8;  // LDS row stride (bf16): 144 B, staggers the 16-B reads over banks
@@ -63,6 +68,20 @@ __device__ __forceinline__ void* uniform_ptr(const void* p) {
   return (void*)(((uint64_t)hi << 32) | lo);
 }
 
+// Adam (Kingma & Ba) hyper-parameters of one step; c1, c2 = bias corrections
+struct AdamHP {
+  float lr, b1, b2, eps, c1, c2;
+};
+
+// one Adam update of parameter p with gradient g (moments m, v updated in place)
+__device__ __forceinline__ float adam_one(float p, float g, float& m, float& v, const AdamHP& h) {
+  const float mi = h.b1 * m + (1.f - h.b1) * g;
+  const float vi = h.b2 * v + (1.f - h.b2) * g * g;
+  m = mi;
+  v = vi;
+  return p - h.lr * (mi / h.c1) / (sqrtf(vi / h.c2) + h.eps);
+}
+
 struct GemmArgs {
   const bf16* A;  // [M][lda], k-contiguous
   const bf16* B;  // [N][ldb], k-contiguous
@@ -76,23 +95,129 @@ struct GemmArgs {
   int ldct;
   const float* Zin;   // pre-activation [M][ldz] (DGELU)
   int ldz;
+  // ADAM: the product is the gradient of the FP32 master weights Cf; their
+  // moments am, av share Cf's layout; Cb and Ct receive the BF16 copies
+  float* am;
+  float* av;
+  AdamHP hp;
 };
+
+// Epilogue of a 64 x 64 output tile (all 512 threads of gemm_nt).  The MFMA
+// layout scatters a lane's outputs over rows, so the tile goes through LDS
+// and the epilogue runs on rows: thread t owns row t / 8, columns
+// 8 (t % 8) .. + 7, and moves FP32 data as 32-byte and BF16 data as 16-byte
+// pieces (a row of the tile is 256 contiguous bytes of an FP32 matrix).  A
+// transposed BF16 output goes back through LDS once more and is stored as
+// 16-byte column pieces of 8 rows.
+//   GELU:  Cf = Z = acc + bias, Cb = H = gelu(Z), Ct = H^T
+//   BIAS:  Cf = acc + bias
+//   DGELU: Cb = dZ = acc * gelu'(Zin), Ct = dZ^T
+//   F32:   Cf = acc
+//   ADAM:  acc is the gradient of the FP32 master weights Cf (moments am, av):
+//          one Adam step, then Cb = BF16 copy, Ct = its transpose
+//   xch: half 1's accumulators (written before the call, 16 KB);
+//   T:   64 x 68 floats, then 64 x 72 BF16 (the B stages, free after the K loop)
+template <int EPI>
+__device__ __forceinline__ void tile_epilogue(const GemmArgs& g, f32x16 acc, const float* xch,
+                                              float* T, int half, int th, int m0, int n0) {
+  constexpr int LT = 68;
+  const int lane = th & 63, w = th >> 6, wr = w >> 1, wc = w & 1;
+  if (half == 0) {
+#pragma unroll
+    for (int i = 0; i < 16; i++) acc[i] += xch[i * 256 + th];
+    // accumulator i of lane l holds row 8 (i / 4) + 4 (l / 32) + i % 4, column l % 32
+    const int col = 32 * wc + (lane & 31), rb = 32 * wr + 4 * (lane >> 5);
+#pragma unroll
+    for (int i = 0; i < 16; i++) T[(rb + 8 * (i >> 2) + (i & 3)) * LT + col] = acc[i];
+  }
+  __syncthreads();
+  const int t = th + 256 * half;
+  const int r = t >> 3, c8 = (t & 7) * 8;
+  const int row = m0 + r, col = n0 + c8;
+  float v[8];
+  {
+    const float4* gp = reinterpret_cast<const float4*>(T + r * LT + c8);
+    const float4 a = gp[0], b = gp[1];
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+  }
+  auto ld8 = [](const float* q, float (&o)[8]) {
+    const float4 a = reinterpret_cast<const float4*>(q)[0], b = reinterpret_cast<const float4*>(q)[1];
+    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+    o[4] = b.x; o[5] = b.y; o[6] = b.z; o[7] = b.w;
+  };
+  auto st8 = [](float* q, const float (&o)[8]) {
+    reinterpret_cast<float4*>(q)[0] = make_float4(o[0], o[1], o[2], o[3]);
+    reinterpret_cast<float4*>(q)[1] = make_float4(o[4], o[5], o[6], o[7]);
+  };
+  bf16x8 ob;
+  const size_t ic = (size_t)row * g.ldc + col;
+  if (EPI == EPI_GELU || EPI == EPI_BIAS) {
+    float bv[8];
+    ld8(g.bias + col, bv);
+#pragma unroll
+    for (int e = 0; e < 8; e++) v[e] += bv[e];
+    st8(g.Cf + ic, v);
+    if (EPI == EPI_BIAS) return;
+#pragma unroll
+    for (int e = 0; e < 8; e++) ob[e] = (bf16)gelu(v[e]);
+  } else if (EPI == EPI_DGELU) {
+    float z[8];
+    ld8(g.Zin + (size_t)row * g.ldz + col, z);
+#pragma unroll
+    for (int e = 0; e < 8; e++) ob[e] = (bf16)(v[e] * gelu_d(z[e]));
+  } else if (EPI == EPI_ADAM) {
+    float pv[8], mv[8], vv[8];
+    ld8(g.Cf + ic, pv);
+    ld8(g.am + ic, mv);
+    ld8(g.av + ic, vv);
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      pv[e] = adam_one(pv[e], v[e], mv[e], vv[e], g.hp);
+      ob[e] = (bf16)pv[e];
+    }
+    st8(g.Cf + ic, pv);
+    st8(g.am + ic, mv);
+    st8(g.av + ic, vv);
+  } else {
+    st8(g.Cf + ic, v);
+    return;
+  }
+  *reinterpret_cast<bf16x8*>(g.Cb + (size_t)row * g.ldcb + col) = ob;
+  if (!g.Ct) return;  // (uniform: no barrier is skipped by part of the workgroup)
+  bf16* Wb = reinterpret_cast<bf16*>(T);  // 64 x 72 BF16 over the (consumed) FP32 tile
+  __syncthreads();
+  *reinterpret_cast<bf16x8*>(Wb + r * 72 + c8) = ob;
+  __syncthreads();
+  // transposed copy: thread t writes column t / 8 of the tile, rows 8 (t % 8) .. + 7
+  const int cc = t >> 3, r8 = (t & 7) * 8;
+  bf16x8 tv;
+#pragma unroll
+  for (int e = 0; e < 8; e++) tv[e] = Wb[(r8 + e) * 72 + cc];
+  *reinterpret_cast<bf16x8*>(g.Ct + (size_t)(n0 + cc) * g.ldct + m0 + r8) = tv;
+}
 
 // 512 threads = two halves of 4 waves.  Half h accumulates the K slices
 // kt = 2i + h of the same 64 x 64 tile (its own double-buffered LDS), so a CU
 // holds two waves per SIMD with independent load streams; at the end half 1
-// hands its accumulators to half 0 through LDS, which runs the epilogue.
+// hands its accumulators to half 0 through LDS, which assembles the tile for
+// the epilogue that both halves run.
+struct GemmSmem {
+  bf16 As[2][2][TM * LDK];  // [half][stage]
+  bf16 Bs[2][2][TN * LDK];
+};
+
 template <int EPI>
-__global__ void __launch_bounds__(512) gemm_nt(GemmArgs g) {
-  __shared__ __align__(16) bf16 As[2][2][TM * LDK];  // [half][stage]
-  __shared__ __align__(16) bf16 Bs[2][2][TN * LDK];
+__device__ __forceinline__ void gemm_tile(const GemmArgs& g, int bx, int by, GemmSmem& sm) {
+  auto& As = sm.As;
+  auto& Bs = sm.Bs;
   // (the half index is the same across a wave; readfirstlane lets the compiler
   // keep what depends on it, such as the slice offsets, in scalar registers)
   const int tid = threadIdx.x, th = tid & 255;
   const int half = __builtin_amdgcn_readfirstlane(tid >> 8);
   const int lane = th & 63, w = th >> 6;
   const int wr = w >> 1, wc = w & 1;
-  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+  const int m0 = by * TM, n0 = bx * TN;
   const int r = th >> 3, c = (th & 7) * 8;
   // Tile loads through buffer descriptors of the 64-row A and B panels:
   // loop-invariant per-thread byte offsets plus the slice offset in an
@@ -171,59 +296,53 @@ __global__ void __launch_bounds__(512) gemm_nt(GemmArgs g) {
     for (int i = 0; i < 16; i++) xch[i * 256 + th] = acc[i];
   }
   __syncthreads();
-  if (half == 1) return;
-#pragma unroll
-  for (int i = 0; i < 16; i++) acc[i] += xch[i * 256 + th];
-  // accumulator j of lane l holds row 8 (j / 4) + 4 (l / 32) + j % 4, column l % 32
-  const int col = n0 + 32 * wc + (lane & 31);
-  const int rbase = m0 + 32 * wr + 4 * (lane >> 5);
-  float bcol = 0.0f;
-  if (EPI == EPI_GELU || EPI == EPI_BIAS) bcol = g.bias[col];
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    const int row0 = rbase + 8 * q;
-    bf16x4 tv;
-#pragma unroll
-    for (int e = 0; e < 4; e++) {
-      const int row = row0 + e;
-      const float v = acc[4 * q + e];
-      if (EPI == EPI_GELU) {
-        const float z = v + bcol;
-        g.Cf[(size_t)row * g.ldc + col] = z;
-        const bf16 hb = (bf16)gelu(z);
-        g.Cb[(size_t)row * g.ldcb + col] = hb;
-        tv[e] = hb;
-      } else if (EPI == EPI_BIAS) {
-        g.Cf[(size_t)row * g.ldc + col] = v + bcol;
-      } else if (EPI == EPI_DGELU) {
-        const bf16 db = (bf16)(v * gelu_d(g.Zin[(size_t)row * g.ldz + col]));
-        g.Cb[(size_t)row * g.ldcb + col] = db;
-        tv[e] = db;
-      } else {
-        g.Cf[(size_t)row * g.ldc + col] = v;
-      }
-    }
-    if ((EPI == EPI_GELU || EPI == EPI_DGELU) && g.Ct)
-      *reinterpret_cast<bf16x4*>(&g.Ct[(size_t)col * g.ldct + row0]) = tv;
+  tile_epilogue<EPI>(g, acc, xch, reinterpret_cast<float*>(&Bs[0][0][0]), half, th, m0, n0);
+}
+
+template <int EPI>
+__global__ void __launch_bounds__(512) gemm_nt(GemmArgs g) {
+  __shared__ __align__(16) GemmSmem sm;
+  gemm_tile<EPI>(g, blockIdx.x, blockIdx.y, sm);
+}
+
+// Two independent products in one launch (1-D grid): workgroups below n1 take
+// the tiles of g1, the others those of g2.  Pairs a data-gradient product
+// (bound by operand traffic) with a weight-gradient product whose Adam
+// epilogue is bound by HBM, so the CUs work on both at once.
+template <int E1, int E2>
+__global__ void __launch_bounds__(512) gemm_nt2(GemmArgs g1, GemmArgs g2, int n1) {
+  __shared__ __align__(16) GemmSmem sm;
+  const int b = blockIdx.x;
+  if (b < n1) {
+    const int nx = g1.N / TN;
+    gemm_tile<E1>(g1, b % nx, b / nx, sm);
+  } else {
+    const int nx = g2.N / TN;
+    gemm_tile<E2>(g2, (b - n1) % nx, (b - n1) / nx, sm);
   }
 }
 
-// Relative-L2 loss of each row; dL/dy = (y - t) / (B ||y - t|| ||t||) as BF16,
-// row-major and transposed.  One workgroup (256 threads) per row.
+// Relative-L2 loss of each row; dL/dy = (y - t) / (B ||y - t|| ||t||) as BF16
+// (row-major; the transpose is a separate tiled pass); n a multiple of 64
+// (16-byte aligned rows, 4 values per thread and load).  Target row r is
+// T[idx[r]] (a device dataset) or T[r] when idx is null.  One workgroup
+// (256 threads) per row.
 __global__ void __launch_bounds__(256) rel_l2_loss(const float* __restrict__ y,
-                                                   const float* __restrict__ t, int n, float inv_b,
-                                                   int batch, bf16* __restrict__ dy,
-                                                   bf16* __restrict__ dyT,
+                                                   const float* __restrict__ T,
+                                                   const int* __restrict__ idx, int n, float inv_b,
+                                                   bf16* __restrict__ dy,
                                                    float* __restrict__ row_loss) {
   __shared__ float red[2][4];
   const int r = blockIdx.x;
   const float* yr = y + (size_t)r * n;
-  const float* tr = t + (size_t)r * n;
+  const float* tr = T + (size_t)(idx ? idx[r] : r) * n;
   float se = 0.f, st = 0.f;
-  for (int i = threadIdx.x; i < n; i += 256) {
-    const float d = yr[i] - tr[i];
-    se += d * d;
-    st += tr[i] * tr[i];
+  for (int i = 4 * threadIdx.x; i < n; i += 1024) {
+    const float4 a = *reinterpret_cast<const float4*>(yr + i);
+    const float4 b = *reinterpret_cast<const float4*>(tr + i);
+    const float d0 = a.x - b.x, d1 = a.y - b.y, d2 = a.z - b.z, d3 = a.w - b.w;
+    se += (d0 * d0 + d1 * d1) + (d2 * d2 + d3 * d3);
+    st += (b.x * b.x + b.y * b.y) + (b.z * b.z + b.w * b.w);
   }
   for (int o = 32; o > 0; o >>= 1) {
     se += __shfl_xor(se, o);
@@ -235,50 +354,83 @@ __global__ void __launch_bounds__(256) rel_l2_loss(const float* __restrict__ y,
   }
   __syncthreads();
   const float E = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
-  const float T = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
-  const float ne = sqrtf(E), nt = T > 0.f ? sqrtf(T) : 1.f;
+  const float Tn = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+  const float ne = sqrtf(E), nt = Tn > 0.f ? sqrtf(Tn) : 1.f;
   if (threadIdx.x == 0) row_loss[r] = ne / nt;
-  if (!dy) return;
   const float sc = ne > 0.f ? inv_b / (ne * nt) : 0.f;
-  for (int i = threadIdx.x; i < n; i += 256) {
-    const bf16 v = (bf16)(sc * (yr[i] - tr[i]));
-    dy[(size_t)r * n + i] = v;
-    dyT[(size_t)i * batch + r] = v;
+  for (int i = 4 * threadIdx.x; i < n; i += 1024) {
+    const float4 a = *reinterpret_cast<const float4*>(yr + i);
+    const float4 b = *reinterpret_cast<const float4*>(tr + i);
+    bf16x4 o;
+    o[0] = (bf16)(sc * (a.x - b.x));
+    o[1] = (bf16)(sc * (a.y - b.y));
+    o[2] = (bf16)(sc * (a.z - b.z));
+    o[3] = (bf16)(sc * (a.w - b.w));
+    *reinterpret_cast<bf16x4*>(dy + (size_t)r * n + i) = o;
   }
 }
 
-// mean of the row losses into hist[slot] (one wave, fixed order)
-__global__ void loss_mean(const float* __restrict__ row_loss, int batch, float* __restrict__ hist,
-                          int slot) {
-  float s = 0.f;
-  for (int i = threadIdx.x; i < batch; i += 64) s += row_loss[i];
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-  if (threadIdx.x == 0) hist[slot] = s / (float)batch;
+// dst [cols][rows] = src [rows][cols]^T (BF16), 64 x 64 tiles through LDS
+__global__ void __launch_bounds__(256) transpose_bf16(const bf16* __restrict__ src, int rows,
+                                                      int cols, bf16* __restrict__ dst) {
+  __shared__ bf16 tile[64][66];
+  const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  for (int i = threadIdx.x; i < 4096; i += 256)
+    tile[i >> 6][i & 63] = src[(size_t)(r0 + (i >> 6)) * cols + c0 + (i & 63)];
+  __syncthreads();
+  for (int i = threadIdx.x; i < 4096; i += 256)
+    dst[(size_t)(c0 + (i >> 6)) * rows + r0 + (i & 63)] = tile[i & 63][i >> 6];
 }
 
-// bias gradient: db[n] = sum_b dZT[n][b], one wave per n
-__global__ void __launch_bounds__(256) row_sum(const bf16* __restrict__ dT, int rows, int cols,
-                                               float* __restrict__ out) {
-  const int n = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (n >= rows) return;
-  float s = 0.f;
-  for (int b = lane; b < cols; b += 64) s += (float)dT[(size_t)n * cols + b];
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-  if (lane == 0) out[n] = s;
-}
+// Bias gradients of all four layers, db[n] = sum_b dZT[n][b] (one wave per n,
+// 8 BF16 of the row per lane and load; cols a multiple of 8), and the mean of
+// the row losses (the last workgroup, one wave, fixed order) into hist[slot].
+// adam != 0: the biases take their Adam step with moments am, av; else db
+// goes to out.
+struct BiasJobs {
+  const bf16* dT[4];
+  float* out[4];
+  float* p[4];
+  float* am[4];
+  float* av[4];
+  int rows[4];
+  int blk0[5];  // first workgroup of each layer; blk0[4] = the loss-mean workgroup
+  int cols, adam, slot;
+  AdamHP hp;
+  const float* row_loss;
+  float* hist;
+};
 
-// Adam (Kingma & Ba) on the FP32 masters
-__global__ void adam_step(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
-                          float* __restrict__ v, size_t n, float lr, float b1, float b2, float eps,
-                          float c1, float c2) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const float gi = g[i];
-    const float mi = b1 * m[i] + (1.f - b1) * gi;
-    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
-    m[i] = mi;
-    v[i] = vi;
-    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+__global__ void __launch_bounds__(256) bias_and_loss(BiasJobs j) {
+  const int b = blockIdx.x, lane = threadIdx.x & 63;
+  if (b == j.blk0[4]) {
+    if (threadIdx.x >= 64) return;
+    float s = 0.f;
+    for (int i = lane; i < j.cols; i += 64) s += j.row_loss[i];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) j.hist[j.slot] = s / (float)j.cols;
+    return;
+  }
+  int l = 0;
+  while (l < 3 && b >= j.blk0[l + 1]) l++;
+  const int n = (b - j.blk0[l]) * 4 + (threadIdx.x >> 6);
+  if (n >= j.rows[l]) return;
+  const bf16* dT = j.dT[l];
+  float s = 0.f;
+  for (int c = 8 * lane; c < j.cols; c += 512) {
+    const bf16x8 v = *reinterpret_cast<const bf16x8*>(dT + (size_t)n * j.cols + c);
+#pragma unroll
+    for (int e = 0; e < 8; e++) s += (float)v[e];
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (lane) return;
+  if (j.adam) {
+    float mm = j.am[l][n], vv = j.av[l][n];
+    j.p[l][n] = adam_one(j.p[l][n], s, mm, vv, j.hp);
+    j.am[l][n] = mm;
+    j.av[l][n] = vv;
+  } else {
+    j.out[l][n] = s;
   }
 }
 
@@ -303,27 +455,34 @@ __global__ void __launch_bounds__(256) weight_copies(const float* __restrict__ W
   }
 }
 
-// input rows (FP32, n_in wide) -> BF16 [batch][ld_in] and [ld_in][batch], zero padded;
-// rows come from a device dataset through idx (or in order when idx is null)
+// input rows (FP32, n_in wide) -> BF16 [batch][ld_in] and [ld_in][batch], zero
+// padded; rows come from a device dataset through idx (or in order when idx
+// is null).  64 x 64 tiles, the transposed copy through LDS.
 __global__ void __launch_bounds__(256) load_inputs(const float* __restrict__ X,
                                                    const int* __restrict__ idx, int batch, int n_in,
                                                    int ld_in, bf16* __restrict__ h0,
                                                    bf16* __restrict__ h0T) {
-  const int r = blockIdx.x;
-  const size_t src = idx ? (size_t)idx[r] : (size_t)r;
-  for (int c = threadIdx.x; c < ld_in; c += 256) {
-    const bf16 v = (bf16)(c < n_in ? X[src * n_in + c] : 0.f);
-    h0[(size_t)r * ld_in + c] = v;
-    if (h0T) h0T[(size_t)c * batch + r] = v;
+  __shared__ bf16 tile[64][66];
+  const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  // thread t: column c0 + t % 64 of rows r0 + t / 64 + 4k (16 independent loads in flight)
+  const int cc = threadIdx.x & 63, c = c0 + cc, rq = threadIdx.x >> 6;
+  float v[16];
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    const int rr = rq + 4 * k;
+    const size_t src = idx ? (size_t)idx[r0 + rr] : (size_t)(r0 + rr);
+    v[k] = c < n_in ? X[src * n_in + c] : 0.f;
   }
-}
-
-__global__ void __launch_bounds__(256) load_targets(const float* __restrict__ T,
-                                                    const int* __restrict__ idx, int n_out,
-                                                    float* __restrict__ t) {
-  const int r = blockIdx.x;
-  const size_t src = idx ? (size_t)idx[r] : (size_t)r;
-  for (int c = threadIdx.x; c < n_out; c += 256) t[(size_t)r * n_out + c] = T[src * n_out + c];
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    const int rr = rq + 4 * k;
+    const bf16 b = (bf16)v[k];
+    h0[(size_t)(r0 + rr) * ld_in + c] = b;
+    tile[rr][cc] = b;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 4096; i += 256)
+    h0T[(size_t)(c0 + (i >> 6)) * batch + r0 + (i & 63)] = tile[i & 63][i >> 6];
 }
 
 }  // namespace ofxm
@@ -384,6 +543,17 @@ static int gemm(ofx_mlp* m, const GemmArgs& a) {
   return OFX_OK;
 }
 
+template <int E1, int E2>
+static int gemm2(ofx_mlp* m, const GemmArgs& a, const GemmArgs& b) {
+  for (const GemmArgs* x : {&a, &b})
+    if (x->M % TM || x->N % TN || x->K % TK || x->M <= 0 || x->N <= 0 || x->K <= 0)
+      return arg_err(m, "GEMM dimensions are not multiples of 64");
+  const int n1 = (a.N / TN) * (a.M / TM), n2 = (b.N / TN) * (b.M / TM);
+  hipLaunchKernelGGL((gemm_nt2<E1, E2>), dim3(n1 + n2), dim3(512), 0, m->stream, a, b, n1);
+  MLAUNCH(m);
+  return OFX_OK;
+}
+
 static int refresh_weights(ofx_mlp* m) {
   for (int l = 0; l < 4; l++) {
     hipLaunchKernelGGL(weight_copies, dim3(m->kin[l] / 64, m->kout[l] / 64), dim3(256), 0,
@@ -423,16 +593,41 @@ static int forward(ofx_mlp* m, int B) {
   return OFX_OK;
 }
 
-// loss, gradients into m->g (device layout), mean loss into hist[slot]
-static int backward(ofx_mlp* m, int B, int slot) {
-  hipLaunchKernelGGL(rel_l2_loss, dim3(B), dim3(256), 0, m->stream, m->y, m->t, m->n_out,
-                     1.0f / (float)B, B, m->dz[4], m->dzT[4], m->row_loss);
+// Loss (mean into hist[slot]) and gradients.  hp null: the gradients go to
+// m->g (device layout).  hp set: each layer's weights and biases take their
+// Adam step as soon as their gradients exist (in the weight-gradient GEMM's
+// epilogue, which also writes the BF16 copies), so m->g is not touched; the
+// layer's data gradient is computed first, from the weights before the step.
+// Targets: row r is T[idx[r]] (idx null: T[r]).
+static int backward(ofx_mlp* m, int B, int slot, const AdamHP* hp, const float* T,
+                    const int* idx) {
+  hipLaunchKernelGGL(rel_l2_loss, dim3(B), dim3(256), 0, m->stream, m->y, T, idx, m->n_out,
+                     1.0f / (float)B, m->dz[4], m->row_loss);
   MLAUNCH(m);
-  hipLaunchKernelGGL(loss_mean, dim3(1), dim3(64), 0, m->stream, m->row_loss, B, m->hist, slot);
+  hipLaunchKernelGGL(transpose_bf16, dim3(m->n_out / 64, B / 64), dim3(256), 0, m->stream,
+                     m->dz[4], B, m->n_out, m->dzT[4]);
   MLAUNCH(m);
+  GemmArgs dh[4] = {}, dw[4] = {};
   for (int l = 3; l >= 0; l--) {
+    if (l > 0) {
+      // dZ_l = (dZ_{l+1} . Wn_l^T) * gelu'(z_l) : A = dz[l+1] [B][out], B = wn[l] [in][out]
+      GemmArgs& d = dh[l];
+      d.A = m->dz[l + 1];
+      d.lda = m->kout[l];
+      d.B = m->wn[l];
+      d.ldb = m->kout[l];
+      d.M = B;
+      d.N = m->kin[l];
+      d.K = m->kout[l];
+      d.Zin = m->z[l];
+      d.ldz = m->width;
+      d.Cb = m->dz[l];
+      d.ldcb = m->width;
+      d.Ct = m->dzT[l];
+      d.ldct = B;
+    }
     // dWt_l [out][in] = dZ^T . H : A = dzT[l+1] [out][B], B = hT[l] [in][B]
-    GemmArgs a{};
+    GemmArgs& a = dw[l];
     a.A = m->dzT[l + 1];
     a.lda = B;
     a.B = m->hT[l];
@@ -440,51 +635,82 @@ static int backward(ofx_mlp* m, int B, int slot) {
     a.M = m->kout[l];
     a.N = m->kin[l];
     a.K = B;
-    a.Cf = m->g + m->off_w[l];
     a.ldc = m->kin[l];
-    int st = gemm<EPI_F32>(m, a);
-    if (st) return st;
-    hipLaunchKernelGGL(row_sum, dim3((m->kout[l] + 3) / 4), dim3(256), 0, m->stream, m->dzT[l + 1],
-                       m->kout[l], B, m->g + m->off_b[l]);
-    MLAUNCH(m);
-    if (l == 0) break;
-    // dZ_l = (dZ_{l+1} . Wn_l^T) * gelu'(z_l) : A = dz[l+1] [B][out], B = wn[l] [in][out]
-    GemmArgs d{};
-    d.A = m->dz[l + 1];
-    d.lda = m->kout[l];
-    d.B = m->wn[l];
-    d.ldb = m->kout[l];
-    d.M = B;
-    d.N = m->kin[l];
-    d.K = m->kout[l];
-    d.Zin = m->z[l];
-    d.ldz = m->width;
-    d.Cb = m->dz[l];
-    d.ldcb = m->width;
-    d.Ct = m->dzT[l];
-    d.ldct = B;
-    st = gemm<EPI_DGELU>(m, d);
-    if (st) return st;
+    if (hp) {
+      a.Cf = m->p + m->off_w[l];
+      a.am = m->am + m->off_w[l];
+      a.av = m->av + m->off_w[l];
+      a.hp = *hp;
+      a.Cb = m->wt[l];
+      a.ldcb = m->kin[l];
+      a.Ct = m->wn[l];  // null for layer 0
+      a.ldct = m->kout[l];
+    } else {
+      a.Cf = m->g + m->off_w[l];
+    }
   }
+  // Launch order: dZ_3 alone; then each layer's weight gradient (and Adam
+  // step) paired with the next data gradient, which reads other weights; the
+  // last launch pairs layers 1 and 0.  dZ_l is always computed before layer
+  // l's weights change.
+  int st = gemm<EPI_DGELU>(m, dh[3]);
+  if (st) return st;
+  if (hp) {
+    st = gemm2<EPI_DGELU, EPI_ADAM>(m, dh[2], dw[3]);
+    if (!st) st = gemm2<EPI_DGELU, EPI_ADAM>(m, dh[1], dw[2]);
+    if (!st) st = gemm2<EPI_ADAM, EPI_ADAM>(m, dw[1], dw[0]);
+  } else {
+    st = gemm2<EPI_DGELU, EPI_F32>(m, dh[2], dw[3]);
+    if (!st) st = gemm2<EPI_DGELU, EPI_F32>(m, dh[1], dw[2]);
+    if (!st) st = gemm2<EPI_F32, EPI_F32>(m, dw[1], dw[0]);
+  }
+  if (st) return st;
+  // biases of every layer (their gradients do not feed any other) and the mean loss
+  BiasJobs j{};
+  int blk = 0;
+  for (int l = 0; l < 4; l++) {
+    const size_t ob = m->off_b[l];
+    j.dT[l] = m->dzT[l + 1];
+    j.out[l] = m->g + ob;
+    j.p[l] = m->p + ob;
+    j.am[l] = m->am + ob;
+    j.av[l] = m->av + ob;
+    j.rows[l] = m->kout[l];
+    j.blk0[l] = blk;
+    blk += (m->kout[l] + 3) / 4;
+  }
+  j.blk0[4] = blk;
+  j.cols = B;
+  j.adam = hp ? 1 : 0;
+  if (hp) j.hp = *hp;
+  j.slot = slot;
+  j.row_loss = m->row_loss;
+  j.hist = m->hist;
+  hipLaunchKernelGGL(bias_and_loss, dim3(blk + 1), dim3(256), 0, m->stream, j);
+  MLAUNCH(m);
   return OFX_OK;
 }
 
-static int adam(ofx_mlp* m, float lr) {
+// hyper-parameters of the next Adam step (advances the step count)
+static AdamHP next_adam(ofx_mlp* m, float lr) {
   m->adam_t++;
   const double b1 = 0.9, b2 = 0.999;
-  const float c1 = (float)(1.0 - pow(b1, (double)m->adam_t));
-  const float c2 = (float)(1.0 - pow(b2, (double)m->adam_t));
-  hipLaunchKernelGGL(adam_step, dim3(2048), dim3(256), 0, m->stream, m->p, m->g, m->am, m->av,
-                     m->np_dev, lr, (float)b1, (float)b2, 1e-8f, c1, c2);
-  MLAUNCH(m);
-  return refresh_weights(m);
+  AdamHP h;
+  h.lr = lr;
+  h.b1 = (float)b1;
+  h.b2 = (float)b2;
+  h.eps = 1e-8f;
+  h.c1 = (float)(1.0 - pow(b1, (double)m->adam_t));
+  h.c2 = (float)(1.0 - pow(b2, (double)m->adam_t));
+  return h;
 }
 
 // host batch -> device inputs (and targets)
 static int stage_batch(ofx_mlp* m, int B, const float* x, const float* t) {
   MCHK(m, hipMemcpyAsync(m->xin, x, sizeof(float) * (size_t)B * m->n_in, hipMemcpyHostToDevice,
                          m->stream));
-  hipLaunchKernelGGL(load_inputs, dim3(B), dim3(256), 0, m->stream, m->xin, (const int*)nullptr, B,
+  hipLaunchKernelGGL(load_inputs, dim3(m->ld_in / 64, B / 64), dim3(256), 0, m->stream, m->xin,
+                     (const int*)nullptr, B,
                      m->n_in, m->ld_in, m->h[0], m->hT[0]);
   MLAUNCH(m);
   if (t)
@@ -671,7 +897,7 @@ int ofx_mlp_gradients(ofx_mlp* m, int batch, const float* x, const float* t, flo
   MCHK(m, hipSetDevice(m->device));
   st = stage_batch(m, batch, x, t);
   if (!st) st = forward(m, batch);
-  if (!st) st = backward(m, batch, 0);
+  if (!st) st = backward(m, batch, 0, nullptr, m->t, nullptr);
   if (st) return st;
   if (loss) MCHK(m, hipMemcpyAsync(loss, m->hist, sizeof(float), hipMemcpyDeviceToHost, m->stream));
   MCHK(m, hipStreamSynchronize(m->stream));
@@ -686,8 +912,10 @@ int ofx_mlp_step(ofx_mlp* m, int batch, const float* x, const float* t, float lr
   MCHK(m, hipSetDevice(m->device));
   st = stage_batch(m, batch, x, t);
   if (!st) st = forward(m, batch);
-  if (!st) st = backward(m, batch, 0);
-  if (!st) st = adam(m, lr);
+  if (!st) {
+    const AdamHP hp = next_adam(m, lr);
+    st = backward(m, batch, 0, &hp, m->t, nullptr);
+  }
   if (st) return st;
   if (loss) MCHK(m, hipMemcpyAsync(loss, m->hist, sizeof(float), hipMemcpyDeviceToHost, m->stream));
   MCHK(m, hipStreamSynchronize(m->stream));
@@ -741,15 +969,14 @@ int ofx_mlp_train(ofx_mlp* m, int batch, int steps, const int* idx, float lr, fl
   MCHK(m, hipEventRecord(m->e0, m->stream));
   for (int s = 0; s < steps; s++) {
     const int* ix = m->idx + (size_t)s * batch;
-    hipLaunchKernelGGL(load_inputs, dim3(batch), dim3(256), 0, m->stream, m->dsX, ix, batch,
-                       m->n_in, m->ld_in, m->h[0], m->hT[0]);
-    MLAUNCH(m);
-    hipLaunchKernelGGL(load_targets, dim3(batch), dim3(256), 0, m->stream, m->dsT, ix, m->n_out,
-                       m->t);
+    hipLaunchKernelGGL(load_inputs, dim3(m->ld_in / 64, batch / 64), dim3(256), 0, m->stream,
+                       m->dsX, ix, batch, m->n_in, m->ld_in, m->h[0], m->hT[0]);
     MLAUNCH(m);
     st = forward(m, batch);
-    if (!st) st = backward(m, batch, s);
-    if (!st) st = adam(m, lr);
+    if (!st) {
+      const AdamHP hp = next_adam(m, lr);
+      st = backward(m, batch, s, &hp, m->dsT, ix);
+    }
     if (st) return st;
   }
   MCHK(m, hipEventRecord(m->e1, m->stream));
