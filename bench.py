@@ -339,6 +339,97 @@ def main_esd(args):
     print(json.dumps(out))
 
 
+def config5_configs(n_det=8, fixed_step=None, pump_fwhm=200e-15):
+    """Config 5 as specified: 2^16 points, 16 ps, 8 mm fan-out PPLN at the
+    local period 30 um, 4 W at 1 GHz (4 nJ sech^2 of pump_fwhm; config 5
+    does not state the length, the ring workload's is 200 fs), 5 % coupler on the 1.2-1.8 um band, n_det of the 64 detunings
+    linspace(-50, 50) fs (every 64/n_det-th, ends included by linspace over
+    n_det), adaptive Heun with the default error bounds (fixed_step: error
+    control off and base step fixed_step metres; the pass then keeps its
+    initial step multiplier of 4, NEE_CPU.m:10, so it takes steps of
+    4 fixed_step)."""
+    from optifax_amd import physics as ph
+    kw = {}
+    if fixed_step:
+        kw = dict(step=float(fixed_step), error_bounds=(0.0, 1e30))
+    return [ph.OPOConfig(n=2 ** 16, window=16e-12, length=8e-3, period=30e-6,
+                         pump_energy=4e-9, pump_fwhm=float(pump_fwhm), delay=float(d),
+                         delay_mode="sync", noise_seed=1000 + i, **kw)
+            for i, d in enumerate(np.linspace(-50e-15, 50e-15, n_det))]
+
+
+def main_config5(args):
+    """Whether config 5's cavities reach a steady state: FP64 plain round
+    trips from one-photon noise for n detunings at once (one persistent
+    launch per 10 trips), convergence test as specified (change of the
+    output-coupled spectrum between trips < 1e-4) until all converged, the
+    trip budget (--c5-trips) or --ws-max-seconds.  value = detunings
+    converged; per detuning: the change between trips (median over the last
+    50 trips), the output energy's rms fluctuation over the last 50 trips
+    relative to its mean, and the adaptive attempts per trip."""
+    from optifax_amd import warmstart as ws
+    from optifax_amd.solver import OPOBatch
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    cfgs = config5_configs(args.c5_detunings, args.c5_fixed_step, args.c5_pump_fwhm)
+    insts = [ws.opo_instance(c)[0] for c in cfgs]
+    budget = args.c5_trips
+    run = OPOBatch(insts, trip_cap=budget, precision=args.precision)
+    t0 = time.perf_counter()
+    done = 0
+    while done < budget:
+        k = min(10, budget - done)
+        run.launch(k, 1e-4)
+        run.sync(allow_nonfinite=True)
+        done += k
+        st = [run.read(i)["status"] for i in range(len(insts))]
+        now = time.perf_counter() - t0
+        log(f"config5: {done} trips, {sum(x == 2 for x in st)}/{len(st)} converged, {now:.0f} s")
+        if all(x in (2, 3) for x in st) or now > args.ws_max_seconds:
+            break
+    wall = time.perf_counter() - t0
+    rs = [run.read(i) for i in range(len(insts))]
+    run.close()
+    per = []
+    for c, r in zip(cfgs, rs):
+        n = r["trips_done"]
+        tl = r["trip_log"][:n]
+        w = tl[-50:]
+        en = w[:, 1]
+        per.append({"detuning_fs": round(c.delay * 1e15, 2), "status": r["status"],
+                    "trips": int(n),
+                    "change_median_last50": float(np.median(w[:, 0])),
+                    "oc_energy_rms_rel_last50": float(en.std() / en.mean()) if en.mean() > 0 else None,
+                    "oc_energy_mean_last50": float(en.mean()),
+                    "attempts_per_trip_median": float(np.median(tl[:, 2]))})
+    out = {
+        "metric": METRIC,
+        "value": int(sum(p["status"] == 2 for p in per)),
+        "unit": f"of {len(per)} detunings at steady state (change between trips < 1e-4)",
+        "n_gpus": 1,
+        "steps": done,
+        "warmup": 0,
+        "ms_per_step": 1e3 * wall / max(done, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "double" else "f32",
+        "data": "synthetic (sech^2 pump + seeded one-photon-per-mode noise)",
+        "config": {
+            "workload": f"config 5 cavities: 2^16 points, 16 ps, 8 mm PPLN at 30 um, 4 nJ "
+                        f"{args.c5_pump_fwhm * 1e15:g} fs sech^2 at 1 GHz, 5% OC on 1.2-1.8 um, "
+                        f"{len(per)} detunings "
+                        "linspace(-50, 50) fs, "
+                        + (f"error control off, fixed {4 * args.c5_fixed_step * 1e6:g} um steps"
+                           if args.c5_fixed_step
+                           else "adaptive Heun (default error bounds)")
+                        + f", plain trips from noise, budget {budget}",
+            "seconds": wall,
+            "per_detuning": per,
+        },
+    }
+    print(json.dumps(out))
+
+
 def main_pass(args):
     """A single SHG/OPG crystal pass: 2^13 points, 10 ps window, 1040 nm;
     sech^2 200 fs pump in 5 um^2 with seeded one-photon noise; 30 um PPLN,
@@ -646,9 +737,15 @@ def main():
                     help="round trips of each run of the esd workload")
     ap.add_argument("--workload", default="ring",
                     choices=["ring", "batch", "warmstart", "surrogate", "steady", "esd", "pass",
-                             "mex"],
+                             "mex", "config5"],
                     help="ring: one synchronously pumped ring OPO (the headline line); batch: 256 uneven "
                          "instances in one persistent launch")
+    ap.add_argument("--c5-detunings", type=int, default=8)
+    ap.add_argument("--c5-trips", type=int, default=200,
+                    help="trip budget of the config5 workload (specified: 1000)")
+    ap.add_argument("--c5-pump-fwhm", type=float, default=200e-15)
+    ap.add_argument("--c5-fixed-step", type=float, default=0.0,
+                    help="config5 with error control off and this fixed step (m); 0 = adaptive")
     ap.add_argument("--tune", default="heuristic", choices=["heuristic", "auto"])
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 256 MiB three times (the known-size "
@@ -682,6 +779,8 @@ def main():
         return main_pass(args)
     if args.workload == "mex":
         return main_mex(args)
+    if args.workload == "config5":
+        return main_config5(args)
     from optifax_amd.solver import Context, OPOBatch
     cfg, b = workload()
     si, ops = b["solver"], b["ops"]

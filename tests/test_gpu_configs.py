@@ -233,3 +233,40 @@ def test_round_trips_with_an_optic_chain_fp64(gpu_ctx):
     assert np.array_equal(r["trip_log"][:, 2:].astype(int), o["steps"])
     assert rel_l2(r["field"], oracle_next_input(b, o["E_cav"])) < 1e-9
     assert rel_l2(r["oc_spec"], SimWindow.fft_order(SimWindow.t2k(o["E_oc"]))) < 1e-9
+
+
+def test_config5_cavity_from_a_saturated_field_fp64(gpu_ctx):
+    """Config 5's cavity at its grid (2^16 points, 16 ps), band coupler (5 %
+    on 1.2-1.8 um), 4 nJ 200 fs pump and -50 fs detuning from synchronism,
+    PPLN at the fan-out's local period of 30 um, adaptive Heun with the
+    default error bounds, from a saturated start (pump plus a signal pulse at
+    30 % of the pump's peak field): two FP64 round trips against the literal
+    OpticalSim.run sequence of the oracle.  The crystal is shortened to
+    0.5 mm so that the oracle finishes in seconds; the 8 mm crystal changes
+    only the number of steps."""
+    from optifax_amd import physics as ph
+    from optifax_amd import warmstart as ws
+    from cases import oracle_next_input
+    cfg = ph.OPOConfig(n=2 ** 16, window=16e-12, length=0.5e-3, period=30e-6, pump_energy=4e-9,
+                       delay=-50e-15, delay_mode="sync", noise_seed=1000)
+    b = cfg.build()
+    si, sw = b["solver"], b["simwin"]
+    seed = np.fft.ifftshift(ws.smooth_seed(sw, cfg, 0.3))      # natural time order
+    E0 = b["noise"] + seed
+    R, T, phi = b["oc"].masks(sw)
+    trips = 2
+    o = oracle.opo_run(trips, E0, b["pump"], np.ones(sw.n), si["n_xtal"], R, T, phi, b["delay"],
+                       sw.rel_omegas, si)
+    ops = b["ops"]
+    run = OPOBatch([dict(solver=si, field0=oracle_next_input(b, E0), pump=ops["pump"],
+                         cav=ops["cav"], ocm=ops["ocm"], delay=b["delay"])],
+                   precision="double", trip_cap=trips, ctx=gpu_ctx)
+    run.launch(trips)
+    run.sync()
+    r = run.read(0)
+    run.close()
+    print(f"config 5 cavity, 0.5 mm: steps per trip {o['steps'].tolist()}, "
+          f"field rel L2 {rel_l2(r['field'], oracle_next_input(b, o['E_cav'])):.1e}")
+    assert np.array_equal(r["trip_log"][:, 2:].astype(int), o["steps"])
+    assert rel_l2(r["field"], oracle_next_input(b, o["E_cav"])) < 1e-9
+    assert rel_l2(r["oc_spec"], SimWindow.fft_order(SimWindow.t2k(o["E_oc"]))) < 1e-9
