@@ -789,7 +789,8 @@ def main():
     trip_cap = args.warmup + args.steps + 1
     inst = dict(solver=si, field0=ops["field0"], pump=ops["pump"], cav=ops["cav"],
                 ocm=ops["ocm"], delay=b["delay"], method=args.method)
-    run = OPOBatch([inst], precision=args.precision, trip_cap=trip_cap, ctx=ctx)
+    run = OPOBatch([inst], precision=args.precision, trip_cap=trip_cap, ctx=ctx,
+                   tune="auto" if args.tune == "auto" else None)
     geo = run.geometry(0)
     # warm-up trips (also builds the one-trip graph)
     if args.warmup > 0:
@@ -855,6 +856,8 @@ def main():
             "rejected_per_trip": rej / args.steps,
             "cluster_workgroups": geo["cluster"],
             "block": geo["block"],
+            "n1": geo["n1"],
+            "tune": args.tune,
         },
         "fp32_round_trips_per_s": rates["single"],
         "fp64_round_trips_per_s": rates["double"],

@@ -94,8 +94,9 @@ def _truncate(inst, steps):
                 method=inst.get("method", s.get("method", "heun")))
 
 
-def measure(inst, cand, k, precision="single", steps=1500, ctx=None, repeats=2):
-    """Seconds per adaptive step of one pass with k concurrent copies, or None
+def _time_pass(inst, cand, k, precision, steps, ctx, repeats):
+    """(best seconds of one launch, adaptive attempts per copy in it) for k
+    concurrent copies of the pass truncated to `steps` minimal steps, or None
     when the candidate cannot launch."""
     from .solver import OPOBatch
     C, n1, block = cand
@@ -109,6 +110,9 @@ def measure(inst, cand, k, precision="single", steps=1500, ctx=None, repeats=2):
         best = None
         before = 0
         for _ in range(repeats):
+            run.write_field(0, proto["field0"])
+            for i in range(1, k):
+                run.write_field(i, proto["field0"])
             run.launch(0)
             run.sync(allow_nonfinite=True)
             ms = run.last_ms()
@@ -116,20 +120,40 @@ def measure(inst, cand, k, precision="single", steps=1500, ctx=None, repeats=2):
             att, before = total - before, total
             if att < 1:
                 return None
-            t = ms * 1e-3 / att
-            best = t if best is None else min(best, t)
+            best = (ms * 1e-3, att) if best is None or ms * 1e-3 < best[0] else best
         return best
     finally:
         run.close()
 
 
-def tune(inst, batch=1, precision="single", steps=1500, ctx=None, use_cache=True, verbose=False):
-    """Best (cluster, n1, block) for `batch` instances shaped like `inst`."""
+def measure(inst, cand, k, precision="single", steps=1500, ctx=None, repeats=2):
+    """Seconds per adaptive step of one pass with k concurrent copies, or None
+    when the candidate cannot launch.  The cost per step is the marginal one
+    between passes truncated to `steps` and 3 `steps` minimal steps, so the
+    launch's fixed costs (table loads into LDS, the first exchange) do not
+    count: a long crystal pass amortises them, a short timing run does not."""
+    a = _time_pass(inst, cand, k, precision, steps, ctx, repeats)
+    if a is None:
+        return None
+    b = _time_pass(inst, cand, k, precision, 3 * steps, ctx, repeats)
+    if b is None:
+        return None
+    (t1, n1), (t2, n2) = a, b
+    if n2 > n1 and t2 > t1:
+        return (t2 - t1) / (n2 - n1)
+    return t2 / n2
+
+
+def tune(inst, batch=1, precision="single", steps=1500, ctx=None, use_cache=True, verbose=False,
+         slots=None):
+    """Best (cluster, n1, block) for `batch` instances shaped like `inst` on
+    `slots` workgroup slots (default: one per CU of the device)."""
     from .solver import default_context
     ctx = ctx or default_context()
     info = ctx.info()
     n = int(np.asarray(inst["solver"]["beta_op"]).size)
-    key = f"{info['name']}|n={n}|{precision}|B={int(batch)}"
+    slots = int(slots) if slots else int(info["cus"])
+    key = f"{info['name']}|n={n}|{precision}|B={int(batch)}|slots={slots}"
     if use_cache:
         if key in _MEM:
             return dict(_MEM[key])
@@ -137,7 +161,6 @@ def tune(inst, batch=1, precision="single", steps=1500, ctx=None, use_cache=True
         if key in disk:
             _MEM[key] = disk[key]
             return dict(disk[key])
-    slots = int(info["cus"])
     results = []
     t0 = time.time()
     for cand in candidates(n):
@@ -167,22 +190,34 @@ def tune(inst, batch=1, precision="single", steps=1500, ctx=None, use_cache=True
 def tune_batch(instances, precision="single", steps=1500, ctx=None, verbose=False):
     """Per-instance geometry for an uneven batch: instances are grouped by N,
     each group tuned for its own size and count (first instance of the group
-    as the prototype).  Returns (instances with cluster/n1 set, block)."""
+    as the prototype) on its share of the CUs.  The groups run side by side
+    in one launch, so a group is given the share of the workgroup slots that
+    its work takes (sum over its instances of N log2 N times the minimal
+    steps): tuned against the whole device, a group would pick clusters that
+    are fastest alone but crowd out the others.  Returns (instances with
+    cluster/n1 set, block)."""
+    from .solver import default_context
     groups: dict = {}
     for i, inst in enumerate(instances):
         n = int(np.asarray(inst["solver"]["beta_op"]).size)
         groups.setdefault(n, []).append(i)
+    cus = int((ctx or default_context()).info()["cus"])
+    work = {n: sum(n * math.log2(n) * float(instances[i]["solver"]["nSteps"]) for i in ids)
+            for n, ids in groups.items()}
+    total = sum(work.values())
+    share = {n: max(1, int(round(cus * w / total))) for n, w in work.items()}
     out = [dict(inst) for inst in instances]
     blocks = {}
     for n, ids in groups.items():
         best = tune(instances[ids[0]], batch=len(ids), precision=precision, steps=steps, ctx=ctx,
-                    verbose=verbose)
+                    verbose=verbose, slots=share[n])
         for i in ids:
             out[i]["cluster"] = best["cluster"]
             out[i]["n1"] = best["n1"]
         blocks[n] = (len(ids), best["block"])
     # one block size per launch: the one chosen for the most instances
     block = max(blocks.values())[1]
-    tune_batch.last_report = {n: tune(instances[ids[0]], batch=len(ids), precision=precision,
-                                      steps=steps, ctx=ctx) for n, ids in groups.items()}
+    tune_batch.last_report = {n: dict(tune(instances[ids[0]], batch=len(ids), precision=precision,
+                                           steps=steps, ctx=ctx, slots=share[n]), slots=share[n])
+                              for n, ids in groups.items()}
     return out, block
