@@ -430,6 +430,93 @@ def main_config5(args):
     print(json.dumps(out))
 
 
+def fibre_example(n=2 ** 13):
+    """The supercontinuum example distributed with gnlsefib.m (Dudley, Genty &
+    Coen's test_Dudley, in ps and m): 835 nm, 10 kW 28.4 fs sech in 15 cm of
+    PCF (gamma 0.11 /W/m, betas to 10th order), fused-silica Raman (fr 0.18,
+    12.2 / 32 fs) and self-steepening, 12.5 ps window on n points, 200 saves."""
+    from optifax_amd import fibre
+    c_nm_ps = 299792458 * 1e9 / 1e12
+    T = np.linspace(-12.5 / 2, 12.5 / 2, n)
+    betas = [-11.830e-3, 8.1038e-5, -9.5205e-8, 2.0737e-10, -5.3943e-13, 1.3486e-15,
+             -2.5495e-18, 3.0524e-21, -1.7140e-24]
+    return dict(T=T, A=np.sqrt(10000.0) / np.cosh(T / 0.0284), w0=2 * np.pi * c_nm_ps / 835.0,
+                gamma=0.11, betas=betas, loss=0.0, fr=0.18,
+                RT=fibre.raman_response(T, 0.0122, 0.032), flength=0.15, nsaves=200)
+
+
+def main_fibre(args):
+    """Fibre GNLSE (gnlsefib.m) on the device: the example pulse at 2^13
+    points with FP32 fields, one fibre (latency) and a batch of fibres with
+    peak powers 5-15 kW in one launch, one workgroup each (throughput;
+    value = fibre solves/s of the batch); FP64 fields at 2^12 beside it.
+    Parity: the single FP32 solve's final temporal field against the FP64
+    oracle (oracle/gnlse.py, numpy, one core), whose wall time is the CPU
+    baseline."""
+    from optifax_amd import fibre
+    c = fibre_example()
+    op = fibre.operators(c["T"], c["w0"], c["gamma"], c["betas"], c["loss"], c["fr"], c["RT"])
+    run = lambda fields, ops, prec, T=c["T"]: fibre.gnlse_batch(T, fields, ops, c["flength"],
+                                                                 c["nsaves"], prec)
+    run([c["A"]], [op], "single")                       # warm-up (module load, first launch)
+    t0 = time.perf_counter()
+    Z, AWz, st1, ms1 = run([c["A"]], [op], "single")
+    wall1 = time.perf_counter() - t0
+    AT1, _ = fibre.lab_frame(Z, AWz[0], op, c["T"].size, op["dT"])
+    B = args.fibre_batch
+    powers = np.linspace(5000.0, 15000.0, B)
+    fields = [np.sqrt(p) / np.cosh(c["T"] / 0.0284) for p in powers]
+    t0 = time.perf_counter()
+    _, _, stB, msB = run(fields, [op] * B, "single")
+    wallB = time.perf_counter() - t0
+    c64 = fibre_example(2 ** 12)
+    op64 = fibre.operators(c64["T"], c64["w0"], c64["gamma"], c64["betas"], 0.0, c64["fr"],
+                           c64["RT"])
+    _, _, st64, ms64 = fibre.gnlse_batch(c64["T"], [c64["A"]], [op64], c64["flength"],
+                                         c64["nsaves"], "double")
+    out = {
+        "metric": METRIC,
+        "value": B / (msB * 1e-3),
+        "unit": f"fibre solves/s (batch of {B}, device time)",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": msB,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 fields, f64 step control",
+        "data": "synthetic (the gnlsefib.m example pulse; the batch scans its peak power)",
+        "config": {
+            "workload": "fibre GNLSE (gnlsefib.m): 2^13 points, 12.5 ps, 835 nm 28.4 fs sech, "
+                        "15 cm PCF, gamma 0.11, 10 betas, Raman fr 0.18, self-steepening, "
+                        "ode45 (RelTol 1e-5, NormControl), 200 saves",
+            "single_device_ms": ms1, "single_wall_s": wall1,
+            "single_steps": [int(st1[0, 0]), int(st1[0, 1])],
+            "batch": B, "batch_device_ms": msB, "batch_wall_s": wallB,
+            "batch_steps_median": float(np.median(stB[:, 0])),
+            "batch_status_ok": int((stB[:, 3] == 0).sum()),
+            "fp64_2e12_device_ms": ms64, "fp64_2e12_steps": [int(st64[0, 0]), int(st64[0, 1])],
+        },
+    }
+    if not args.no_cpu_baseline:
+        import oracle.gnlse as og
+        t0 = time.perf_counter()
+        Zo, ATo, _, _, _, sto = og.gnlsefib(c["T"], c["A"], c["w0"], c["gamma"], c["betas"],
+                                            c["loss"], c["fr"], c["RT"], c["flength"],
+                                            c["nsaves"])
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": 1.0 / dt, "unit": "fibre solves/s", "cores": 1, "kind": "port",
+            "sample": f"the single example solve in the FP64 oracle (numpy FFTs, one core): "
+                      f"{dt:.1f} s, {sto['accepted']} steps"}
+        out["parity"] = {
+            "against": "FP64 oracle, final temporal field",
+            "rel_l2": float(np.linalg.norm(AT1[-1] - ATo[-1]) / np.linalg.norm(ATo[-1])),
+            "oracle_steps": [sto["accepted"], sto["rejected"]]}
+    print(json.dumps(out))
+
+
 def main_pass(args):
     """A single SHG/OPG crystal pass: 2^13 points, 10 ps window, 1040 nm;
     sech^2 200 fs pump in 5 um^2 with seeded one-photon noise; 30 um PPLN,
@@ -737,9 +824,10 @@ def main():
                     help="round trips of each run of the esd workload")
     ap.add_argument("--workload", default="ring",
                     choices=["ring", "batch", "warmstart", "surrogate", "steady", "esd", "pass",
-                             "mex", "config5"],
+                             "mex", "config5", "fibre"],
                     help="ring: one synchronously pumped ring OPO (the headline line); batch: 256 uneven "
                          "instances in one persistent launch")
+    ap.add_argument("--fibre-batch", type=int, default=256)
     ap.add_argument("--c5-detunings", type=int, default=8)
     ap.add_argument("--c5-trips", type=int, default=200,
                     help="trip budget of the config5 workload (specified: 1000)")
@@ -781,6 +869,8 @@ def main():
         return main_mex(args)
     if args.workload == "config5":
         return main_config5(args)
+    if args.workload == "fibre":
+        return main_fibre(args)
     from optifax_amd.solver import Context, OPOBatch
     cfg, b = workload()
     si, ops = b["solver"], b["ops"]

@@ -276,6 +276,30 @@ int ofx_qpm_zsum(int device, int n_out, const double* dk, int n_run, const doubl
 int ofx_sinc_gain(int device, int n_out, const double* dk0, int n_g, const double* kc, int n_m,
                   const double* m, double length, double* gain, double* ms);
 
+/* ---- fibre GNLSE (gnlsefib.m) ------------------------------------------
+ * ofx_gnlse_run replaces gnlsefib.m's integration (gnlsefib.m:49-76: the
+ * interaction-picture GNLSE right-hand side under ode45 with RelTol / AbsTol
+ * and NormControl on, solutions at the requested points).  The host forms
+ * the operators as gnlsefib.m:22-45 does; the device integrates.
+ * n_inst independent fibres of n points (a power of two, n <= 2^13 with
+ * OFX_F32 fields, 2^12 with OFX_F64), one workgroup each, synchronous on
+ * `device`.  Arrays (host, FP64; complex interleaved; FFT order):
+ *   zs [nsaves >= 3]   requested points, strictly monotone, zs[0] = start
+ *   y0 [n_inst][n]     AW at zs[0] (ifft of the temporal field)
+ *   L  [n_inst][n]     linear operator i B - alpha / 2
+ *   W  [n_inst][n]     real: V + w0 with self-steepening, else 1
+ *   RW [n_inst][n]     n ifft(fftshift(RT)), or NULL: no Raman term
+ *   gamma, fr, dT [n_inst]   (gamma already divided by w0 with self-steepening)
+ *   out [n_inst][nsaves][n]  AW at each requested point (interaction picture)
+ *   stats [n_inst][4]  accepted steps, rejected steps, evaluations, status
+ *                      (0 ok, 1 step below 16 eps(z), 2 max_steps reached,
+ *                      3 non-finite)
+ * *ms (may be NULL) receives the kernel's device time. */
+int ofx_gnlse_run(int device, int precision, int n_inst, int n, int nsaves, const double* zs,
+                  const double* y0, const double* L, const double* W, const double* RW,
+                  const double* gamma, const double* fr, const double* dT, double rtol,
+                  double atol, long long max_steps, double* out, long long* stats, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -54,6 +54,7 @@ EXPORTED = [
     "ofx_mlp_set_params", "ofx_mlp_get_params", "ofx_mlp_forward", "ofx_mlp_gradients",
     "ofx_mlp_step", "ofx_mlp_upload_dataset", "ofx_mlp_train", "ofx_mlp_last_ms",
     "ofx_qpm_zsum", "ofx_sinc_gain",
+    "ofx_gnlse_run",
 ]
 
 
@@ -153,6 +154,11 @@ def load(path: str | None = None):
                                    ctypes.c_double, _DP, _DP]
         L.ofx_sinc_gain.argtypes = [ctypes.c_int, ctypes.c_int, _DP, ctypes.c_int, _DP,
                                     ctypes.c_int, _DP, ctypes.c_double, _DP, _DP]
+    if hasattr(L, "ofx_gnlse_run"):
+        L.ofx_gnlse_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_longlong, _DP,
+                                    ctypes.POINTER(ctypes.c_longlong), _DP]
     if path is None:
         _lib = L
     return L

@@ -21,7 +21,8 @@ HIP_FLAGS = [
     "-Wno-unused-result", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-LIB_SOURCES = ["ofx_backend_hip.hip", "ofx_runtime.cpp", "ofx_mlp.hip", "ofx_qpm.hip"]
+LIB_SOURCES = ["ofx_backend_hip.hip", "ofx_runtime.cpp", "ofx_mlp.hip", "ofx_qpm.hip",
+               "ofx_gnlse.hip"]
 
 
 def _run(cmd):

@@ -1,0 +1,484 @@
+// Body of the fibre GNLSE solver: one workgroup integrates one fibre
+// instance over z (gnlsefib.m:1-89), the field held in LDS for the FFTs.
+//
+// The ODE is gnlsefib.m's interaction-picture form, dAW/dz = rhs(z, AW)
+// (gnlsefib.m:49-61), with MATLAB's fft / ifft conventions:
+//   AT = fft(AW exp(L z)),  IT = |AT|^2,
+//   RS = dT fr fft(ifft(IT) RW)                        (Raman, when given)
+//   M  = ifft(AT ((1 - fr) IT + RS))   or  ifft(AT IT)  (no Raman)
+//   rhs = i gamma W M exp(-L z)
+// four (two without Raman) length-n FFTs per evaluation, each a radix-4
+// Stockham transform in LDS (fft_q) over the workgroup's two LDS buffers,
+// twiddles from a quarter table in LDS.
+//
+// Integration: the Dormand-Prince 5(4) pair with the step control of
+// MATLAB's ode45 at NormControl on (Shampine & Reichelt 1997), outputs at
+// the requested points through the pair's quartic continuous extension.
+// Every thread runs the same scalar control from the same FP64 block sums
+// (fixed summation order), so all take the same decisions without a
+// broadcast.  Stage vectors f_1..f_7, y, ynew and the stage input live in
+// global memory (per instance 11 n complex values, L2-resident for the
+// sizes an LDS field allows).
+//
+// Compiles as HIP device code (ofx_gnlse.hip) and, with OFX_EMU, as plain
+// C++ where a workgroup is one CPU thread (tests only).
+#pragma once
+#include "ofx_pass_impl.h"
+
+namespace ofx {
+
+struct GnlseArgs {
+  int n, lg, nsaves, n_inst;
+  const double* zs;      // [nsaves] requested points, zs[0] = start
+  const double* L;       // [n_inst][n] complex (re, im), FFT order
+  const double* W;       // [n_inst][n] real, FFT order
+  const double* RW;      // [n_inst][n] complex, or null: no Raman
+  const double* gamma;   // [n_inst] (gamma / w0 when self-steepening is on)
+  const double* fr;      // [n_inst] Raman fraction
+  const double* dT;      // [n_inst] time step
+  double rtol, atol;
+  long long max_steps;   // accepted + rejected steps before the instance gives up
+  int lossless;          // every Re(L) is 0: exp(L z) is a pure phase
+  void* ws;              // [n_inst][11 n] cpx<R> work vectors
+  const void* tw;        // [n / 4] cpx<R>: exp(-2 pi i m / n), m < n / 4
+  const double* y0;      // [n_inst][n] complex: AW at zs[0]
+  double* out;           // [n_inst][nsaves][n] complex: AW at each requested point
+  long long* stats;      // [n_inst][4]: accepted, rejected, evaluations, status
+};
+
+enum { GNLSE_OK = 0, GNLSE_HMIN = 1, GNLSE_CAP = 2, GNLSE_NONFINITE = 3 };
+
+// Dormand-Prince (1980) 5(4) pair; column k of DP_B gives stage k + 2, the
+// last column the 5th-order solution; DP_E = b5 - b4; DP_BI the continuous
+// extension (Shampine 1986): y(t + s h) = y + h sum_j f_j sum_m DP_BI[j][m] s^(m+1).
+// (Constant memory on the device: the control indexes them at run time.)
+#ifdef OFX_EMU
+#define OFX_CONST_TABLE static const
+#else
+#define OFX_CONST_TABLE static __constant__ const
+#endif
+OFX_CONST_TABLE double DP_A[6] = {1.0 / 5, 3.0 / 10, 4.0 / 5, 8.0 / 9, 1.0, 1.0};
+OFX_CONST_TABLE double DP_B[7][6] = {
+    {1.0 / 5, 3.0 / 40, 44.0 / 45, 19372.0 / 6561, 9017.0 / 3168, 35.0 / 384},
+    {0, 9.0 / 40, -56.0 / 15, -25360.0 / 2187, -355.0 / 33, 0},
+    {0, 0, 32.0 / 9, 64448.0 / 6561, 46732.0 / 5247, 500.0 / 1113},
+    {0, 0, 0, -212.0 / 729, 49.0 / 176, 125.0 / 192},
+    {0, 0, 0, 0, -5103.0 / 18656, -2187.0 / 6784},
+    {0, 0, 0, 0, 0, 11.0 / 84},
+    {0, 0, 0, 0, 0, 0}};
+OFX_CONST_TABLE double DP_E[7] = {71.0 / 57600, 0, -71.0 / 16695, 71.0 / 1920,
+                                  -17253.0 / 339200, 22.0 / 525, -1.0 / 40};
+OFX_CONST_TABLE double DP_BI[7][4] = {{1, -183.0 / 64, 37.0 / 12, -145.0 / 128},
+                                      {0, 0, 0, 0},
+                                      {0, 1500.0 / 371, -1000.0 / 159, 1000.0 / 371},
+                                      {0, -125.0 / 32, 125.0 / 12, -375.0 / 64},
+                                      {0, 9477.0 / 3392, -729.0 / 106, 25515.0 / 6784},
+                                      {0, -11.0 / 7, 11.0 / 3, -55.0 / 28},
+                                      {0, 3.0 / 2, -4, 5.0 / 2}};
+
+// Sum of one value per thread over the workgroup in FP64, the same result in
+// every thread (xor butterfly within a wave, then the wave partials in wave
+// order).  red: one double per wave.
+OFX_DEV double wg_sum(double v, double* red) {
+#ifdef OFX_EMU
+  (void)red;
+  return v;
+#else
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int nw = (int)(blockDim.x >> 6);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int i = 0; i < nw; i++) s += red[i];
+  return s;
+#endif
+}
+
+// MATLAB's eps(t): the spacing of doubles at |t|
+OFX_DEV double eps_of(double t) {
+  t = fabs(t);
+  if (t < 2.2250738585072014e-308) return 4.9406564584124654e-324;
+  int e;
+  (void)frexp(t, &e);
+  return ldexp(1.0, e - 53);
+}
+
+// exp(-2 pi i m / n) for 0 <= m < n from the quarter table q[r] = exp(-2 pi i r / n),
+// r < n / 4: m = k n/4 + r gives q[r] (-i)^k.
+template <typename R>
+OFX_DEV cpx<R> tw_quarter(const cpx<R>* q, int m, int lg) {
+  const int k = m >> (lg - 2), r = m & ((1 << (lg - 2)) - 1);
+  const cpx<R> v = q[r];
+  switch (k & 3) {
+    case 0: return v;
+    case 1: return mkc<R>(v.y, -v.x);    // -i v
+    case 2: return mkc<R>(-v.x, -v.y);   // -v
+    default: return mkc<R>(-v.y, v.x);   // i v
+  }
+}
+
+// One length-2^lg Stockham FFT in LDS (radix-4 passes, one radix-2 pass for
+// odd lg), ping-ponging between `in` and `out`; returns the buffer holding
+// the result.  SIGN = -1 forward, +1 inverse, both unnormalised.  Twiddles
+// from the quarter table in LDS (lg >= 2).
+template <typename R, int SIGN>
+OFX_DEV cpx<R>* fft_q(cpx<R>* in, cpx<R>* out, int lg, const cpx<R>* q) {
+  typedef cpx<R> C;
+  const int L = 1 << lg;
+  const int tid = OFX_TID, nt = OFX_NT;
+  int Ns = 1;
+  if (lg & 1) {
+    const int h = L >> 1;
+#pragma unroll 4
+    for (int j = tid; j < h; j += nt) {
+      const C v0 = in[j], v1 = in[j + h];
+      out[2 * j] = cadd(v0, v1);
+      out[2 * j + 1] = csub(v0, v1);
+    }
+    OFX_SYNC();
+    C* t = in;
+    in = out;
+    out = t;
+    Ns = 2;
+  }
+  const int q4 = L >> 2;
+  for (; Ns < L; Ns <<= 2) {
+    const int span = L / (4 * Ns);
+#pragma unroll 4
+    for (int j = tid; j < q4; j += nt) {
+      const int k = j & (Ns - 1);
+      C v0 = in[j], v1 = in[j + q4], v2 = in[j + 2 * q4], v3 = in[j + 3 * q4];
+      if (Ns > 1) {
+        const int m = k * span;
+        const C w1 = tw_quarter(q, m, lg), w2 = tw_quarter(q, 2 * m, lg),
+                w3 = tw_quarter(q, 3 * m, lg);
+        if (SIGN > 0) {
+          v1 = cmulc(v1, w1);
+          v2 = cmulc(v2, w2);
+          v3 = cmulc(v3, w3);
+        } else {
+          v1 = cmul(v1, w1);
+          v2 = cmul(v2, w2);
+          v3 = cmul(v3, w3);
+        }
+      }
+      const C s02 = cadd(v0, v2), d02 = csub(v0, v2), s13 = cadd(v1, v3), d13 = csub(v1, v3);
+      const C rd = (SIGN < 0) ? mkc<R>(d13.y, -d13.x) : mkc<R>(-d13.y, d13.x);
+      const int base = ((j - k) << 2) + k;
+      out[base] = cadd(s02, s13);
+      out[base + Ns] = cadd(d02, rd);
+      out[base + 2 * Ns] = csub(s02, s13);
+      out[base + 3 * Ns] = csub(d02, rd);
+    }
+    OFX_SYNC();
+    C* t = in;
+    in = out;
+    out = t;
+  }
+  return in;
+}
+
+template <typename R>
+struct GnlseInst {
+  const GnlseArgs* a;
+  int n, lg;
+  cpx<R>* lds0;
+  cpx<R>* lds1;
+  double* red;
+  const double* L;
+  const double* W;
+  const double* RW;
+  const cpx<R>* tw;  // quarter twiddle table (LDS)
+  int lossless;
+  double gamma, fr, dT;
+  cpx<R>* at;  // AT of the current evaluation
+};
+
+// exp(L_i z) as (growth, phasor): FP64 exponent and phase, the phase reduced in FP64
+template <typename R>
+OFX_DEV cpx<R> lin_factor(const double* L, int i, double z, bool inverse, int lossless) {
+  const double li = L[2 * i + 1];
+  const cpx<R> e = expi((inverse ? -li : li) * z, R(0));
+  if (lossless) return e;
+  const double lr = L[2 * i];
+  const double g = exp((inverse ? -lr : lr) * z);
+  return mkc<R>((R)(g * (double)e.x), (R)(g * (double)e.y));
+}
+
+// f = rhs(z, y) (gnlsefib.m:49-61); y, f in global memory
+template <typename R>
+OFX_DEV void gnlse_rhs(const GnlseInst<R>& c, double z, const cpx<R>* y, cpx<R>* f) {
+  typedef cpx<R> C;
+  const int n = c.n, tid = OFX_TID, nt = OFX_NT;
+  const R inv_n = (R)(1.0 / (double)n);
+#pragma unroll 4
+  for (int i = tid; i < n; i += nt) c.lds0[i] = cmul(y[i], lin_factor<R>(c.L, i, z, false, c.lossless));
+  OFX_SYNC();
+  C* P = fft_q<R, -1>(c.lds0, c.lds1, c.lg, c.tw);   // AT = fft(.)
+  C* Q = P == c.lds0 ? c.lds1 : c.lds0;
+  for (int i = tid; i < n; i += nt) {
+    const C v = P[i];
+    c.at[i] = v;
+    Q[i] = mkc<R>(v.x * v.x + v.y * v.y, R(0));
+  }
+  OFX_SYNC();
+  C* M;
+  if (c.RW) {
+    C* X = fft_q<R, 1>(Q, P, c.lg, c.tw);              // n ifft(IT)
+#pragma unroll 4
+    for (int i = tid; i < n; i += nt) {
+      const C rw = mkc<R>((R)c.RW[2 * i], (R)c.RW[2 * i + 1]);
+      X[i] = cscale(cmul(X[i], rw), inv_n);
+    }
+    OFX_SYNC();
+    C* Xo = X == c.lds0 ? c.lds1 : c.lds0;
+    C* Y = fft_q<R, -1>(X, Xo, c.lg, c.tw);             // fft(ifft(IT) RW)
+    const R dfr = (R)(c.dT * c.fr), one_fr = (R)(1.0 - c.fr);
+#pragma unroll 4
+    for (int i = tid; i < n; i += nt) {
+      const C a = c.at[i];
+      const R it = a.x * a.x + a.y * a.y;
+      const C rs = cscale(Y[i], dfr);
+      Y[i] = cmul(a, mkc<R>(one_fr * it + rs.x, rs.y));
+    }
+    OFX_SYNC();
+    M = fft_q<R, 1>(Y, Y == c.lds0 ? c.lds1 : c.lds0, c.lg, c.tw);
+  } else {
+#pragma unroll 4
+    for (int i = tid; i < n; i += nt) {
+      const C a = c.at[i];
+      Q[i] = cscale(a, a.x * a.x + a.y * a.y);
+    }
+    OFX_SYNC();
+    M = fft_q<R, 1>(Q, P, c.lg, c.tw);
+  }
+  // rhs = i gamma W (M / n) exp(-L z)
+#pragma unroll 4
+  for (int i = tid; i < n; i += nt) {
+    const C m = cmul(cscale(M[i], inv_n), lin_factor<R>(c.L, i, z, true, c.lossless));
+    const R s = (R)(c.gamma * c.W[i]);
+    f[i] = mkc<R>(-s * m.y, s * m.x);
+  }
+  OFX_SYNC();
+}
+
+// dst = y + sum_j cf[j] f_j (in stage order, zero coefficients skipped)
+template <typename R>
+OFX_DEV void stage_comb(int n, const cpx<R>* y, cpx<R>* const* f, const double* cf, int nj,
+                        cpx<R>* dst) {
+#pragma unroll 2
+  for (int i = OFX_TID; i < n; i += OFX_NT) {
+    R ax = R(0), ay = R(0);
+#pragma unroll
+    for (int j = 0; j < 7; j++) {
+      if (j >= nj || cf[j] == 0.0) continue;
+      const R c = (R)cf[j];
+      ax = ax + c * f[j][i].x;
+      ay = ay + c * f[j][i].y;
+    }
+    dst[i] = mkc<R>(y[i].x + ax, y[i].y + ay);
+  }
+}
+
+template <typename R>
+OFX_DEV double norm2(int n, const cpx<R>* v, double* red) {
+  double s = 0.0;
+#pragma unroll 4
+  for (int i = OFX_TID; i < n; i += OFX_NT) {
+    const double x = (double)v[i].x, y = (double)v[i].y;
+    s += x * x + y * y;
+  }
+  return sqrt(wg_sum(s, red));
+}
+
+// || sum_j E_j f_j ||, the error weights applied per point in stage order
+template <typename R>
+OFX_DEV double err_norm(int n, cpx<R>* const* f, const double* E, double* red) {
+  double s = 0.0;
+#pragma unroll 2
+  for (int i = OFX_TID; i < n; i += OFX_NT) {
+    double ax = 0.0, ay = 0.0;
+#pragma unroll
+    for (int j = 0; j < 7; j++) {
+      if (E[j] == 0.0) continue;
+      ax = ax + E[j] * (double)f[j][i].x;
+      ay = ay + E[j] * (double)f[j][i].y;
+    }
+    s += ax * ax + ay * ay;
+  }
+  return sqrt(wg_sum(s, red));
+}
+
+template <typename R>
+OFX_DEV void store_out(int n, const cpx<R>* v, double* o) {
+  for (int i = OFX_TID; i < n; i += OFX_NT) {
+    o[2 * i] = (double)v[i].x;
+    o[2 * i + 1] = (double)v[i].y;
+  }
+}
+
+template <typename R>
+OFX_DEV bool all_finite(int n, const cpx<R>* v, double* red) {
+  double bad = 0.0;
+  for (int i = OFX_TID; i < n; i += OFX_NT)
+    if (!(fabs((double)v[i].x) <= 1.7976931348623157e308) ||
+        !(fabs((double)v[i].y) <= 1.7976931348623157e308))
+      bad = 1.0;
+  return wg_sum(bad, red) == 0.0;
+}
+
+// Integrate instance `inst` over zs[0] .. zs[nsaves - 1] (ode45, NormControl on).
+template <typename R>
+OFX_DEV void gnlse_instance(const GnlseArgs& a, int inst, cpx<R>* lds0, cpx<R>* lds1,
+                            cpx<R>* ldstw, double* red) {
+  typedef cpx<R> C;
+  const int n = a.n;
+  GnlseInst<R> c;
+  c.a = &a;
+  c.n = n;
+  c.lg = a.lg;
+  c.lds0 = lds0;
+  c.lds1 = lds1;
+  c.red = red;
+  c.L = a.L + (size_t)inst * 2 * n;
+  c.W = a.W + (size_t)inst * n;
+  c.RW = a.RW ? a.RW + (size_t)inst * 2 * n : nullptr;
+  for (int i = OFX_TID; i < (n >> 2); i += OFX_NT) ldstw[i] = ((const C*)a.tw)[i];
+  c.tw = ldstw;
+  c.lossless = a.lossless;
+  c.gamma = a.gamma[inst];
+  c.fr = a.fr[inst];
+  c.dT = a.dT[inst];
+  C* ws = (C*)a.ws + (size_t)inst * 11 * n;
+  C* f[7];
+  for (int j = 0; j < 7; j++) f[j] = ws + (size_t)j * n;
+  C* y = ws + (size_t)7 * n;
+  C* ynew = ws + (size_t)8 * n;
+  C* ys = ws + (size_t)9 * n;
+  c.at = ws + (size_t)10 * n;
+  double* out = a.out + (size_t)inst * a.nsaves * 2 * n;
+  long long* st = a.stats + 4 * (size_t)inst;
+  for (int i = OFX_TID; i < n; i += OFX_NT) {
+    const double* p = a.y0 + ((size_t)inst * n + i) * 2;
+    y[i] = mkc<R>((R)p[0], (R)p[1]);
+  }
+  OFX_SYNC();
+  store_out(n, y, out);
+  const double t0 = a.zs[0], tfinal = a.zs[a.nsaves - 1];
+  const double tdir = tfinal >= t0 ? 1.0 : -1.0;
+  const double rtol = a.rtol, pw = 1.0 / 5.0;
+  const double threshold = a.atol / rtol;
+  const double hmax = 0.1 * fabs(tfinal - t0);
+  double t = t0;
+  gnlse_rhs(c, t, y, f[0]);
+  long long nfev = 1, acc = 0, rej = 0;
+  int status = GNLSE_OK;
+  double normy = norm2(n, y, red);
+  double hmin = 16.0 * eps_of(t);
+  double absh = fmin(hmax, fabs(a.zs[1] - a.zs[0]));
+  const double rh = (norm2(n, f[0], red) / fmax(normy, threshold)) / (0.8 * pow(rtol, pw));
+  if (absh * rh > 1.0) absh = 1.0 / rh;
+  absh = fmax(absh, hmin);
+  int nxt = 1;
+  bool done = false;
+  while (!done) {
+    hmin = 16.0 * eps_of(t);
+    absh = fmin(hmax, fmax(hmin, absh));
+    double h = tdir * absh;
+    if (1.1 * absh >= fabs(tfinal - t)) {
+      h = tfinal - t;
+      absh = fabs(h);
+      done = true;
+    }
+    bool nofailed = true;
+    double err = 0.0, normynew = 0.0, tnew = t;
+    for (;;) {
+      double cf[6];
+      for (int k = 0; k < 5; k++) {
+        for (int j = 0; j <= k; j++) cf[j] = h * DP_B[j][k];
+        stage_comb(n, y, f, cf, k + 1, ys);
+        OFX_SYNC();
+        gnlse_rhs(c, t + h * DP_A[k], ys, f[k + 1]);
+      }
+      tnew = t + h * DP_A[5];
+      if (done) tnew = tfinal;
+      for (int j = 0; j < 6; j++) cf[j] = h * DP_B[j][5];
+      h = tnew - t;
+      stage_comb(n, y, f, cf, 6, ynew);
+      OFX_SYNC();
+      gnlse_rhs(c, tnew, ynew, f[6]);
+      nfev += 6;
+      normynew = norm2(n, ynew, red);
+      const double errwt = fmax(fmax(normy, normynew), threshold);
+      err = absh * (err_norm(n, f, DP_E, red) / errwt);
+      if (!(err == err)) {  // NaN: the field is not finite
+        status = GNLSE_NONFINITE;
+        break;
+      }
+      if (acc + rej + 1 > a.max_steps) {
+        status = GNLSE_CAP;
+        break;
+      }
+      if (err > rtol) {
+        rej++;
+        if (absh <= hmin) {
+          status = GNLSE_HMIN;
+          break;
+        }
+        if (nofailed) {
+          nofailed = false;
+          absh = fmax(hmin, absh * fmax(0.1, 0.8 * pow(rtol / err, pw)));
+        } else {
+          absh = fmax(hmin, 0.5 * absh);
+        }
+        h = tdir * absh;
+        done = false;
+      } else {
+        break;
+      }
+    }
+    if (status != GNLSE_OK) break;
+    acc++;
+    // requested points inside (t, tnew]
+    while (nxt < a.nsaves && tdir * (tnew - a.zs[nxt]) >= 0.0) {
+      double* o = out + (size_t)nxt * 2 * n;
+      if (a.zs[nxt] == tnew) {
+        store_out(n, ynew, o);
+      } else {
+        const double s = (a.zs[nxt] - t) / h;
+        const double sv[4] = {s, s * s, s * s * s, s * s * s * s};
+        double cs[7];
+        for (int j = 0; j < 7; j++)
+          cs[j] = h * (((DP_BI[j][0] * sv[0] + DP_BI[j][1] * sv[1]) + DP_BI[j][2] * sv[2]) +
+                       DP_BI[j][3] * sv[3]);
+        stage_comb(n, y, f, cs, 7, ys);
+        store_out(n, ys, o);
+      }
+      nxt++;
+    }
+    OFX_SYNC();
+    if (done) break;
+    if (nofailed) {
+      const double temp = 1.25 * pow(err / rtol, pw);
+      absh = temp > 0.2 ? absh / temp : 5.0 * absh;
+    }
+    t = tnew;
+    C* sw = y;
+    y = ynew;
+    ynew = sw;
+    normy = normynew;
+    sw = f[0];
+    f[0] = f[6];
+    f[6] = sw;
+  }
+  if (status == GNLSE_OK && !all_finite(n, y, red)) status = GNLSE_NONFINITE;
+  if (OFX_TID == 0) {
+    st[0] = acc;
+    st[1] = rej;
+    st[2] = nfev;
+    st[3] = status;
+  }
+}
+
+}  // namespace ofx
