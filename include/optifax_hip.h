@@ -286,7 +286,8 @@ int ofx_sinc_gain(int device, int n_out, const double* dk0, int n_g, const doubl
  * `device`.  Arrays (host, FP64; complex interleaved; FFT order):
  *   zs [nsaves >= 3]   requested points, strictly monotone, zs[0] = start
  *   y0 [n_inst][n]     AW at zs[0] (ifft of the temporal field)
- *   L  [n_inst][n]     linear operator i B - alpha / 2
+ *   L  [n_inst][n]     linear operator i B - alpha / 2 (its real part the
+ *                      same at every frequency, else OFX_E_ARG)
  *   W  [n_inst][n]     real: V + w0 with self-steepening, else 1
  *   RW [n_inst][n]     n ifft(fftshift(RT)), or NULL: no Raman term
  *   gamma, fr, dT [n_inst]   (gamma already divided by w0 with self-steepening)

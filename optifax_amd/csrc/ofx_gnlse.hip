@@ -60,12 +60,17 @@ int gnlse_run(int n_inst, int n, int lg, int nsaves, const double* zs, const dou
     tw[m].y = (R)sin(ph);
   }
   const size_t ni = (size_t)n_inst, nn = (size_t)n;
-  DBuf b_zs, b_y0, b_L, b_W, b_RW, b_g, b_fr, b_dT, b_tw, b_ws, b_out, b_st;
+  std::vector<double> hB, hlre;
+  std::vector<R> hW;
+  std::vector<C> hRW;
+  if (!ofx::gnlse_tables<R>(n_inst, n, L, W, RW, hB, hlre, hW, hRW)) return OFX_E_ARG;
+  DBuf b_zs, b_y0, b_B, b_lre, b_W, b_RW, b_g, b_fr, b_dT, b_tw, b_ws, b_out, b_st;
   int st = upload(b_zs, zs, (size_t)nsaves);
   if (!st) st = upload(b_y0, y0, ni * 2 * nn);
-  if (!st) st = upload(b_L, L, ni * 2 * nn);
-  if (!st) st = upload(b_W, W, ni * nn);
-  if (!st && RW) st = upload(b_RW, RW, ni * 2 * nn);
+  if (!st) st = upload(b_B, hB.data(), ni * nn);
+  if (!st) st = upload(b_lre, hlre.data(), ni);
+  if (!st) st = upload(b_W, hW.data(), ni * nn);
+  if (!st && RW) st = upload(b_RW, hRW.data(), ni * nn);
   if (!st) st = upload(b_g, gamma, ni);
   if (!st) st = upload(b_fr, fr, ni);
   if (!st) st = upload(b_dT, dT, ni);
@@ -80,18 +85,16 @@ int gnlse_run(int n_inst, int n, int lg, int nsaves, const double* zs, const dou
   a.nsaves = nsaves;
   a.n_inst = n_inst;
   a.zs = (const double*)b_zs.p;
-  a.L = (const double*)b_L.p;
-  a.W = (const double*)b_W.p;
-  a.RW = RW ? (const double*)b_RW.p : nullptr;
+  a.B = (const double*)b_B.p;
+  a.lre = (const double*)b_lre.p;
+  a.W = b_W.p;
+  a.RW = RW ? b_RW.p : nullptr;
   a.gamma = (const double*)b_g.p;
   a.fr = (const double*)b_fr.p;
   a.dT = (const double*)b_dT.p;
   a.rtol = rtol;
   a.atol = atol;
   a.max_steps = max_steps;
-  a.lossless = 1;
-  for (size_t i = 0; i < (size_t)n_inst * n; i++)
-    if (L[2 * i] != 0.0) a.lossless = 0;
   a.ws = b_ws.p;
   a.tw = b_tw.p;
   a.y0 = (const double*)b_y0.p;

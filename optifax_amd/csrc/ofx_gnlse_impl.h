@@ -6,7 +6,7 @@
 //   AT = fft(AW exp(L z)),  IT = |AT|^2,
 //   RS = dT fr fft(ifft(IT) RW)                        (Raman, when given)
 //   M  = ifft(AT ((1 - fr) IT + RS))   or  ifft(AT IT)  (no Raman)
-//   rhs = i gamma W M exp(-L z)
+//   rhs = i gamma W M exp(-L z),  L = i B - alpha / 2
 // four (two without Raman) length-n FFTs per evaluation, each a radix-4
 // Stockham transform in LDS (fft_q) over the workgroup's two LDS buffers,
 // twiddles from a quarter table in LDS.
@@ -23,6 +23,8 @@
 // Compiles as HIP device code (ofx_gnlse.hip) and, with OFX_EMU, as plain
 // C++ where a workgroup is one CPU thread (tests only).
 #pragma once
+#include <vector>
+
 #include "ofx_pass_impl.h"
 
 namespace ofx {
@@ -30,15 +32,15 @@ namespace ofx {
 struct GnlseArgs {
   int n, lg, nsaves, n_inst;
   const double* zs;      // [nsaves] requested points, zs[0] = start
-  const double* L;       // [n_inst][n] complex (re, im), FFT order
-  const double* W;       // [n_inst][n] real, FFT order
-  const double* RW;      // [n_inst][n] complex, or null: no Raman
+  const double* B;       // [n_inst][n] Im L = the dispersion B(V), FFT order
+  const double* lre;     // [n_inst] Re L = -alpha / 2 (the same at every frequency)
+  const void* W;         // [n_inst][n] R: V + w0 or 1, FFT order
+  const void* RW;        // [n_inst][n] cpx<R>, or null: no Raman
   const double* gamma;   // [n_inst] (gamma / w0 when self-steepening is on)
   const double* fr;      // [n_inst] Raman fraction
   const double* dT;      // [n_inst] time step
   double rtol, atol;
   long long max_steps;   // accepted + rejected steps before the instance gives up
-  int lossless;          // every Re(L) is 0: exp(L z) is a pure phase
   void* ws;              // [n_inst][11 n] cpx<R> work vectors
   const void* tw;        // [n / 4] cpx<R>: exp(-2 pi i m / n), m < n / 4
   const double* y0;      // [n_inst][n] complex: AW at zs[0]
@@ -186,24 +188,19 @@ struct GnlseInst {
   cpx<R>* lds0;
   cpx<R>* lds1;
   double* red;
-  const double* L;
-  const double* W;
-  const double* RW;
+  const double* B;
+  double lre;
+  const R* W;
+  const cpx<R>* RW;
   const cpx<R>* tw;  // quarter twiddle table (LDS)
-  int lossless;
   double gamma, fr, dT;
   cpx<R>* at;  // AT of the current evaluation
 };
 
-// exp(L_i z) as (growth, phasor): FP64 exponent and phase, the phase reduced in FP64
+// exp(+-i B_i z): the phase formed and reduced in FP64
 template <typename R>
-OFX_DEV cpx<R> lin_factor(const double* L, int i, double z, bool inverse, int lossless) {
-  const double li = L[2 * i + 1];
-  const cpx<R> e = expi((inverse ? -li : li) * z, R(0));
-  if (lossless) return e;
-  const double lr = L[2 * i];
-  const double g = exp((inverse ? -lr : lr) * z);
-  return mkc<R>((R)(g * (double)e.x), (R)(g * (double)e.y));
+OFX_DEV cpx<R> phase_factor(const double* B, int i, double z) {
+  return expi(B[i] * z, R(0));
 }
 
 // f = rhs(z, y) (gnlsefib.m:49-61); y, f in global memory
@@ -212,8 +209,10 @@ OFX_DEV void gnlse_rhs(const GnlseInst<R>& c, double z, const cpx<R>* y, cpx<R>*
   typedef cpx<R> C;
   const int n = c.n, tid = OFX_TID, nt = OFX_NT;
   const R inv_n = (R)(1.0 / (double)n);
+  // exp(L z) = exp(Re L z) exp(i B z); Re L is one number per fibre
+  const R gf = (R)exp(c.lre * z), gb = (R)(exp(-c.lre * z) / (double)n);
 #pragma unroll 4
-  for (int i = tid; i < n; i += nt) c.lds0[i] = cmul(y[i], lin_factor<R>(c.L, i, z, false, c.lossless));
+  for (int i = tid; i < n; i += nt) c.lds0[i] = cscale(cmul(y[i], phase_factor<R>(c.B, i, z)), gf);
   OFX_SYNC();
   C* P = fft_q<R, -1>(c.lds0, c.lds1, c.lg, c.tw);   // AT = fft(.)
   C* Q = P == c.lds0 ? c.lds1 : c.lds0;
@@ -228,8 +227,7 @@ OFX_DEV void gnlse_rhs(const GnlseInst<R>& c, double z, const cpx<R>* y, cpx<R>*
     C* X = fft_q<R, 1>(Q, P, c.lg, c.tw);              // n ifft(IT)
 #pragma unroll 4
     for (int i = tid; i < n; i += nt) {
-      const C rw = mkc<R>((R)c.RW[2 * i], (R)c.RW[2 * i + 1]);
-      X[i] = cscale(cmul(X[i], rw), inv_n);
+      X[i] = cscale(cmul(X[i], c.RW[i]), inv_n);
     }
     OFX_SYNC();
     C* Xo = X == c.lds0 ? c.lds1 : c.lds0;
@@ -254,10 +252,11 @@ OFX_DEV void gnlse_rhs(const GnlseInst<R>& c, double z, const cpx<R>* y, cpx<R>*
     M = fft_q<R, 1>(Q, P, c.lg, c.tw);
   }
   // rhs = i gamma W (M / n) exp(-L z)
+  const R gW = (R)c.gamma;
 #pragma unroll 4
   for (int i = tid; i < n; i += nt) {
-    const C m = cmul(cscale(M[i], inv_n), lin_factor<R>(c.L, i, z, true, c.lossless));
-    const R s = (R)(c.gamma * c.W[i]);
+    const C m = cscale(cmulc(M[i], phase_factor<R>(c.B, i, z)), gb);
+    const R s = gW * c.W[i];
     f[i] = mkc<R>(-s * m.y, s * m.x);
   }
   OFX_SYNC();
@@ -341,12 +340,13 @@ OFX_DEV void gnlse_instance(const GnlseArgs& a, int inst, cpx<R>* lds0, cpx<R>* 
   c.lds0 = lds0;
   c.lds1 = lds1;
   c.red = red;
-  c.L = a.L + (size_t)inst * 2 * n;
-  c.W = a.W + (size_t)inst * n;
-  c.RW = a.RW ? a.RW + (size_t)inst * 2 * n : nullptr;
+  c.B = a.B + (size_t)inst * n;
+  c.lre = a.lre[inst];
+  c.W = (const R*)a.W + (size_t)inst * n;
+  c.RW = a.RW ? (const C*)a.RW + (size_t)inst * n : nullptr;
   for (int i = OFX_TID; i < (n >> 2); i += OFX_NT) ldstw[i] = ((const C*)a.tw)[i];
   c.tw = ldstw;
-  c.lossless = a.lossless;
+
   c.gamma = a.gamma[inst];
   c.fr = a.fr[inst];
   c.dT = a.dT[inst];
@@ -479,6 +479,34 @@ OFX_DEV void gnlse_instance(const GnlseArgs& a, int inst, cpx<R>* lds0, cpx<R>* 
     st[2] = nfev;
     st[3] = status;
   }
+}
+
+// Host side: the device tables from the caller's operators.  L must be
+// i B - alpha / 2 (gnlsefib.m:31): its real part the same at every
+// frequency.  Returns false when it is not.
+template <typename R>
+bool gnlse_tables(int n_inst, int n, const double* L, const double* W, const double* RW,
+                  std::vector<double>& B, std::vector<double>& lre, std::vector<R>& Wr,
+                  std::vector<cpx<R>>& RWr) {
+  const size_t nn = (size_t)n;
+  B.resize((size_t)n_inst * nn);
+  lre.resize(n_inst);
+  Wr.resize((size_t)n_inst * nn);
+  RWr.resize(RW ? (size_t)n_inst * nn : 0);
+  for (int k = 0; k < n_inst; k++) {
+    const double* l = L + (size_t)k * 2 * nn;
+    lre[k] = l[0];
+    for (size_t i = 0; i < nn; i++) {
+      if (l[2 * i] != lre[k]) return false;
+      B[(size_t)k * nn + i] = l[2 * i + 1];
+      Wr[(size_t)k * nn + i] = (R)W[(size_t)k * nn + i];
+      if (RW) {
+        RWr[(size_t)k * nn + i].x = (R)RW[((size_t)k * nn + i) * 2];
+        RWr[(size_t)k * nn + i].y = (R)RW[((size_t)k * nn + i) * 2 + 1];
+      }
+    }
+  }
+  return true;
 }
 
 }  // namespace ofx

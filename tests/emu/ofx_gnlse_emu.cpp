@@ -24,6 +24,10 @@ int run(int n_inst, int n, int lg, int nsaves, const double* zs, const double* y
     tw[m].x = (R)cos(ph);
     tw[m].y = (R)sin(ph);
   }
+  std::vector<double> hB, hlre;
+  std::vector<R> hW;
+  std::vector<C> hRW;
+  if (!ofx::gnlse_tables<R>(n_inst, n, L, W, RW, hB, hlre, hW, hRW)) return OFX_E_ARG;
   std::vector<C> ws((size_t)n_inst * 11 * n);
   ofx::GnlseArgs a;
   a.n = n;
@@ -31,18 +35,16 @@ int run(int n_inst, int n, int lg, int nsaves, const double* zs, const double* y
   a.nsaves = nsaves;
   a.n_inst = n_inst;
   a.zs = zs;
-  a.L = L;
-  a.W = W;
-  a.RW = RW;
+  a.B = hB.data();
+  a.lre = hlre.data();
+  a.W = hW.data();
+  a.RW = RW ? (const void*)hRW.data() : nullptr;
   a.gamma = gamma;
   a.fr = fr;
   a.dT = dT;
   a.rtol = rtol;
   a.atol = atol;
   a.max_steps = max_steps;
-  a.lossless = 1;
-  for (size_t i = 0; i < (size_t)n_inst * n; i++)
-    if (L[2 * i] != 0.0) a.lossless = 0;
   a.ws = ws.data();
   a.tw = tw.data();
   a.y0 = y0;
