@@ -214,12 +214,35 @@ def phi_calc(L, T, simwin: SimWindow, material="PPLN", table=None):
     return phi - phi[ref], phi_abs
 
 
-def bulk_dispersion(simwin: SimWindow, material, path_length, table=None, T=20.0):
+def path_length(simwin: SimWindow, material, length, table=None, T=20.0):
+    """Dielectric.PathLength (Dielectric.m:94-101): a number, or for a length
+    given as a function of the refractive index (a prism) that function of
+    the material's index as Dielectric.simulate forms it (Dielectric.m:39-46:
+    sellmeier at the optic's temperature, real part, clamped at 1), one path
+    per wavelength."""
+    if not callable(length):
+        return length
+    n = np.real(sellmeier_n(simwin.wavelengths * 1e6, material, T, table))
+    return np.asarray(length(np.where(n < 1, 1.0, n)), dtype=np.float64)
+
+
+def prism_pb_path_length(n, L1=19.0):
+    """prismfn_PB.m:1-29, the reference's Pellin-Broca prism: the path
+    through the prism [m] for index n, apex angle a = 78 deg 26', Brewster
+    incidence at 54 deg: 2 L1 sin(a) / cos(asin(sin(54 deg) / n)) / 1000."""
+    a = math.radians(78 + 26 / 60)
+    th_i = np.real(np.arcsin(np.sin(math.radians(54.0)) / np.asarray(n, dtype=np.complex128)))
+    return 2 * L1 * math.sin(a) / np.cos(th_i) / 1000.0
+
+
+def bulk_dispersion(simwin: SimWindow, material, path_length_, table=None, T=20.0):
     """Dielectric.Dispersion (Dielectric.m:106-114): the relative spectral
-    phase of a bulk optic, phi_calc of its path length (NaN outside the
+    phase of a bulk optic, phi_calc of its path length (a number, or a
+    function of the index for a prism: see path_length; NaN outside the
     0.2-20 um knot band).  Its factor exp(-i phi) multiplies a cavity's
     spectral transfer where the optic sits in the chain."""
-    return phi_calc(path_length, T, simwin, material, table)[0]
+    L = path_length(simwin, material, path_length_, table, T)
+    return phi_calc(L, T, simwin, material, table)[0]
 
 
 def fresnel_t(n1, n2, theta_deg=0.0, exit=False):
@@ -242,12 +265,15 @@ def fresnel_t(n1, n2, theta_deg=0.0, exit=False):
 
 
 def surface_transmission(coating, simwin: SimWindow, material=None, theta_deg=0.0, order=1,
-                         table=None, T=20.0):
-    """OpticalSurface.Transmission (OpticalSurface.m:45-71) for the coating
-    forms that need no data file: a number (percent if > 1), a table of
-    [T, upper wavelength limit] rows, a callable of the wavelengths, "AR"
-    (1), or "None" (the bare surface: fresnel's p transmission from air into
-    the material, or out of it for the second surface, order 2)."""
+                         table=None, T=20.0, data_dir=None):
+    """OpticalSurface.Transmission (OpticalSurface.m:45-82): a number (percent
+    if > 1), a table of [T, upper wavelength limit] rows, a callable of the
+    wavelengths, "AR" (1), "None" (the bare surface: fresnel's p transmission
+    from air into the material, or out of it for the second surface, order
+    2), or the name of a transmission data file in `data_dir` (transmission(),
+    shifted towards shorter wavelengths at oblique incidence: the curve moves
+    by 0.2 lam_max sin^2(theta), lam_max the wavelength of peak transmission,
+    in whole grid steps rounded up, OpticalSurface.m:73-82)."""
     lam = simwin.wavelengths
     if isinstance(coating, str):
         if coating == "AR":
@@ -255,7 +281,16 @@ def surface_transmission(coating, simwin: SimWindow, material=None, theta_deg=0.
         if coating == "None":
             n = sellmeier_n(lam * 1e6, material, T, table)
             return fresnel_t(1.0, n, theta_deg, exit=(order == 2))[0]
-        raise ValueError(f"coating {coating!r} needs a transmission data file")
+        if data_dir is None:
+            raise ValueError(f"coating {coating!r} is a data file: give data_dir")
+        import os
+        t = transmission(os.path.join(data_dir, coating), lam)[0]
+        if theta_deg != 0:
+            i = int(np.argmax(t))
+            dlam = (lam[i + 1] - lam[i - 1]) / 2
+            shift = lam[i] * 0.2 * math.sin(math.radians(theta_deg)) ** 2
+            t = np.roll(t, -int(math.ceil(shift / dlam)))
+        return t
     if callable(coating):
         return np.asarray(coating(lam), dtype=np.float64)
     c = np.asarray(coating, dtype=np.float64)
@@ -268,6 +303,143 @@ def surface_transmission(coating, simwin: SimWindow, material=None, theta_deg=0.
         for k in range(1, len(vals)):
             t[(lam > lims[k - 1]) & (lam < lims[k])] = vals[k]
     return t / 100.0 if np.any(t > 1) else t
+
+
+def matlab_smooth(y, span):
+    """smooth(y, span), 'moving' method (import_interp.m:13 smooths data files
+    with span 0.01): span < 1 is a fraction of the points, rounded up; an even
+    span drops by one; each point is the mean of the `span` points centred on
+    it, and near the ends of the largest centred window that fits (1, 3,
+    5, ... points: yy(1) = y(1), yy(2) = mean(y(1:3)), ...)."""
+    y = np.asarray(y, dtype=np.float64)
+    n = y.size
+    if span < 1:
+        span = math.ceil(span * n)
+    span = min(int(span), n)
+    w = span - 1 + span % 2
+    if w <= 1:
+        return y.copy()
+    idx = np.arange(n)
+    half = np.minimum(np.minimum(idx, n - 1 - idx), w // 2)
+    c = np.concatenate([[0.0], np.cumsum(y)])
+    lo, hi = idx - half, idx + half + 1
+    return (c[hi] - c[lo]) / (hi - lo)
+
+
+def makima(x, y, xq, fill=0.0):
+    """interp1(x, y, xq, 'makima', fill): the modified Akima piecewise cubic
+    (Akima 1970 with the weights w = |d_{i+1} - d_i| + |d_{i+1} + d_i| / 2),
+    `fill` outside [x_1, x_n]."""
+    from scipy.interpolate import Akima1DInterpolator
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    order = np.argsort(x, kind="stable")
+    f = Akima1DInterpolator(x[order], y[order], method="makima")
+    out = f(np.asarray(xq, dtype=np.float64), extrapolate=False)
+    return np.where(np.isnan(out), fill, out)
+
+
+def read_data_table(path):
+    """The two numeric columns of a data file as readtable gives them
+    (header lines and non-numeric rows skipped; comma, tab or space separated)."""
+    xs, ys = [], []
+    with open(path) as f:
+        for line in f:
+            parts = line.replace(",", " ").split()
+            if len(parts) < 2:
+                continue
+            try:
+                a, b = float(parts[0]), float(parts[1])
+            except ValueError:
+                continue
+            xs.append(a)
+            ys.append(b)
+    if not xs:
+        raise ValueError(f"no numeric rows in {path}")
+    return np.array(xs), np.array(ys)
+
+
+def import_interp(path, x_sim, xfun=None, yfun=None):
+    """import_interp.m:1-23: a two-column data file onto x_sim: the y column
+    smoothed (smooth, span 0.01), mapped by yfun, padded with zeros at
+    min|x_sim| and max(x_sim) when x_sim covers the data on both sides, then
+    makima with zero outside."""
+    x_dat, y_dat = read_data_table(path)
+    if xfun is not None:
+        x_dat = xfun(x_dat)
+    y_dat = matlab_smooth(y_dat, 0.01)
+    if yfun is not None:
+        y_dat = yfun(y_dat)
+    x_sim = np.asarray(x_sim, dtype=np.float64)
+    if np.min(np.abs(x_sim)) < np.min(np.abs(x_dat)) and np.max(np.abs(x_sim)) > np.max(np.abs(x_dat)):
+        x_dat = np.concatenate([[np.min(np.abs(x_sim))], x_dat, [np.max(x_sim)]])
+        y_dat = np.concatenate([[0.0], y_dat, [0.0]])
+    return makima(x_dat, y_dat, x_sim, 0.0)
+
+
+def transmission(path, l_sim, nsteps=1, nmat=1.0, frnum=0, lim1=300e-9, lim2=None, table=None):
+    """transmission.m:1-44: a transmission data file (wavelength in nm, T in
+    percent) on the wavelengths l_sim [m] -> (T, T_step, a_step): clipped to
+    [0, 1]; divided by the Fresnel factor (4 n / (1 + n)^2)^frnum of the
+    material (nmat: indices, or a material name for sellmeier); zero outside
+    [lim1, lim2]; renormalised when above 1; per-step loss over nsteps."""
+    l_sim = np.asarray(l_sim, dtype=np.float64)
+    if lim2 is None:
+        lim2 = float(np.max(l_sim))
+    T = import_interp(path, l_sim * 1e9, yfun=lambda v: v / 100.0)
+    T = np.where(np.isfinite(T), T, 0.0)
+    T = np.clip(T, 0.0, 1.0)
+    if isinstance(nmat, str):
+        nmat = sellmeier_n(l_sim * 1e6, nmat, 20.0, table)
+    nmat = np.asarray(nmat)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        T = np.real(T / ((4.0 * nmat) / (1.0 + nmat) ** 2) ** frnum)
+    T = np.where(np.isfinite(T), T, 0.0)
+    T = np.where(T < 0, 0.0, T)
+    T = np.where((l_sim < lim1) | (l_sim > lim2), 0.0, T)
+    if np.max(T) > 1:
+        T = T / np.max(T)
+    with np.errstate(divide="ignore"):
+        a_step = -np.log(T) / nsteps / 2.0
+    T_step = np.exp(-2.0 * a_step)
+    return T, T_step, a_step
+
+
+def load_transmission_table(path):
+    """The transmission columns of a table in the layout of the reference's
+    Materials.csv: {material: (T_file, T_length_m, Fresnel_Reflections)} for
+    the rows that name a file (Dielectric.m:47-50,70-78)."""
+    import csv
+    out = {}
+    with open(path, newline="") as f:
+        for row in csv.DictReader(f):
+            name = (row.get("Material") or "").strip()
+            tf = (row.get("T_file") or "").strip()
+            if not name or not tf:
+                continue
+            try:
+                out[name] = (tf, float(row["T_length_m"]), float(row["Fresnel_Reflections"]))
+            except (TypeError, ValueError, KeyError):
+                continue
+    return out
+
+
+def bulk_transmission(simwin: "SimWindow", material, path_length_, transmission_table, data_dir,
+                      table=None, T=20.0):
+    """Dielectric.Transmission (Dielectric.m:68-92): the material's data file
+    (T_file, measured over T_length_m with Fresnel_Reflections surfaces, from
+    `transmission_table`, see load_transmission_table) through transmission()
+    with the material's real index (clamped at 1, Dielectric.m:39-46) within
+    the window's spectral limits, scaled to the path length (a number or a
+    function of the index, see path_length): T^(path / T_length_m)."""
+    import os
+    tfile, tlen, frnum = transmission_table[material]
+    lam = simwin.wavelengths
+    nr = np.real(sellmeier_n(lam * 1e6, material, T, table))
+    nr = np.where(nr < 1, 1.0, nr)
+    lo, hi = simwin.spectral_limits
+    Tm, _, _ = transmission(os.path.join(data_dir, tfile), lam, 1, nr, frnum, lo * 1e-9, hi * 1e-9)
+    return Tm ** (path_length(simwin, material, path_length_, table, T) / tlen)
 
 
 def _movmean(x, k):
@@ -344,11 +516,14 @@ def coating_phase(simwin: SimWindow, gdd, theta_deg=0.0):
 
 
 def optic_response(simwin: SimWindow, regime="T", s1="None", material="FS", length=1e-3,
-                   theta_deg=0.0, s2=None, table=None, T=20.0, gdd=None, gdd2="s1"):
-    """Optic.simulate (Optic.m:79-99) for a lossless bulk: (transmission,
-    reflection, dispersion phase).  Transmission = S1 * S2 surface
-    transmissions (surface_transmission; s2 defaults to s1, the second
-    surface crossed on the way out), reflection = 1 - S1's.  The phase is
+                   theta_deg=0.0, s2=None, table=None, T=20.0, gdd=None, gdd2="s1",
+                   transmission_table=None, data_dir=None):
+    """Optic.simulate (Optic.m:79-99): (transmission, reflection, dispersion
+    phase).  Transmission = S1 * bulk * S2 (surface_transmission; s2
+    defaults to s1, the second surface crossed on the way out; the bulk's
+    data-file transmission over `length` when `transmission_table` and
+    `data_dir` are given, see bulk_transmission, else a lossless bulk),
+    reflection = 1 - S1's.  The phase is
     S1's coating phase (coating_phase of a measured GDD curve `gdd` =
     (wavelength_nm, gdd_fs2), else 0) and, in transmission, also the bulk's
     relative phase over `length` and S2's coating phase.  S2 carries the
@@ -357,14 +532,17 @@ def optic_response(simwin: SimWindow, regime="T", s1="None", material="FS", leng
     included (Optic.m:57,72-75).  `T` is the bulk's temperature; the bare
     surfaces take fresnel's default 20 C index (fresnel.m:10-16)."""
     s2 = s1 if s2 is None else s2
-    t1 = surface_transmission(s1, simwin, material, theta_deg, 1, table, 20.0)
-    t2 = surface_transmission(s2, simwin, material, theta_deg, 2, table, 20.0)
+    t1 = surface_transmission(s1, simwin, material, theta_deg, 1, table, 20.0, data_dir)
+    t2 = surface_transmission(s2, simwin, material, theta_deg, 2, table, 20.0, data_dir)
+    tb = 1.0
+    if transmission_table is not None:
+        tb = bulk_transmission(simwin, material, length, transmission_table, data_dir, table, T)
     phi = coating_phase(simwin, gdd, theta_deg)
     if regime == "T":
         g2 = gdd if (isinstance(gdd2, str) and gdd2 == "s1") else gdd2
         phi = phi + bulk_dispersion(simwin, material, length, table, T) + \
             coating_phase(simwin, g2, theta_deg)
-    return t1 * t2, 1.0 - t1, phi
+    return t1 * tb * t2, 1.0 - t1, phi
 
 
 def cavity_response(optics):
