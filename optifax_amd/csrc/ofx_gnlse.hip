@@ -1,7 +1,7 @@
 // Fibre GNLSE solver on gfx950 (gnlsefib.m): one workgroup per fibre
 // instance, the whole field in LDS (two buffers of n complex samples for the
-// Stockham FFTs and a quarter twiddle table: n <= 2^13 in FP32, 2^12 in FP64
-// within the 160 KB of LDS),
+// Stockham FFTs and n / 3 twiddles: n <= 2^13 in FP32, 2^12 in FP64 within
+// the 160 KB of LDS),
 // Dormand-Prince 5(4) with ode45's step control, outputs at the requested
 // points.  The body is ofx_gnlse_impl.h.
 #include <hip/hip_runtime.h>
@@ -22,7 +22,7 @@ __global__ void __launch_bounds__(512) gnlse_kernel(GnlseArgs a) {
   cpx<R>* l0 = reinterpret_cast<cpx<R>*>(gnlse_smem);
   cpx<R>* l1 = l0 + a.n;
   cpx<R>* tw = l1 + a.n;
-  double* red = reinterpret_cast<double*>(tw + a.n / 4);
+  double* red = reinterpret_cast<double*>(tw + fft_table_size(a.lg));
   gnlse_instance<R>(a, blockIdx.x, l0, l1, tw, red);
 }
 
@@ -52,13 +52,8 @@ int gnlse_run(int n_inst, int n, int lg, int nsaves, const double* zs, const dou
               const double* fr, const double* dT, double rtol, double atol, long long max_steps,
               double* out, long long* stats, double* ms) {
   typedef ofx::cpx<R> C;
-  // twiddles exp(-2 pi i m / n), built in FP64 and rounded once
-  std::vector<C> tw(n / 4);   // quarter table exp(-2 pi i m / n), m < n / 4
-  for (int m = 0; m < n / 4; m++) {
-    const double ph = -2.0 * M_PI * (double)m / (double)n;
-    tw[m].x = (R)cos(ph);
-    tw[m].y = (R)sin(ph);
-  }
+  // the FFT passes' twiddles, built in FP64 and rounded once
+  const std::vector<C> tw = ofx::fft_stage_twiddles<R>(lg);
   const size_t ni = (size_t)n_inst, nn = (size_t)n;
   std::vector<double> hB, hlre;
   std::vector<R> hW;
@@ -74,7 +69,7 @@ int gnlse_run(int n_inst, int n, int lg, int nsaves, const double* zs, const dou
   if (!st) st = upload(b_g, gamma, ni);
   if (!st) st = upload(b_fr, fr, ni);
   if (!st) st = upload(b_dT, dT, ni);
-  if (!st) st = upload(b_tw, tw.data(), nn / 4);
+  if (!st) st = upload(b_tw, tw.data(), tw.size());
   if (!st) st = upload(b_ws, (const C*)nullptr, ni * 11 * nn);
   if (!st) st = upload(b_out, (const double*)nullptr, ni * (size_t)nsaves * 2 * nn);
   if (!st) st = upload(b_st, (const long long*)nullptr, ni * 4);
@@ -102,7 +97,7 @@ int gnlse_run(int n_inst, int n, int lg, int nsaves, const double* zs, const dou
   a.stats = (long long*)b_st.p;
   // threads: a quarter of the points (one radix-4 butterfly each), 64..512
   const int nt = std::min(512, std::max(64, n / 4));
-  const size_t lds = (2 * nn + nn / 4) * sizeof(C) + 8 * (size_t)(nt / 64);
+  const size_t lds = (2 * nn + tw.size()) * sizeof(C) + 8 * (size_t)(nt / 64);
   st = hip_ok(hipFuncSetAttribute((const void*)ofx::gnlse_kernel<R>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   if (st) return st;

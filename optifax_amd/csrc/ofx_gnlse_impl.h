@@ -8,8 +8,9 @@
 //   M  = ifft(AT ((1 - fr) IT + RS))   or  ifft(AT IT)  (no Raman)
 //   rhs = i gamma W M exp(-L z),  L = i B - alpha / 2
 // four (two without Raman) length-n FFTs per evaluation, each a radix-4
-// Stockham transform in LDS (fft_q) over the workgroup's two LDS buffers,
-// twiddles from a quarter table in LDS.
+// Stockham transform in LDS (fft_fused) over the workgroup's two LDS buffers
+// with the pointwise steps in its first and last passes, twiddles from
+// per-pass tables in LDS.
 //
 // Integration: the Dormand-Prince 5(4) pair with the step control of
 // MATLAB's ode45 at NormControl on (Shampine & Reichelt 1997), outputs at
@@ -42,7 +43,7 @@ struct GnlseArgs {
   double rtol, atol;
   long long max_steps;   // accepted + rejected steps before the instance gives up
   void* ws;              // [n_inst][11 n] cpx<R> work vectors
-  const void* tw;        // [n / 4] cpx<R>: exp(-2 pi i m / n), m < n / 4
+  const void* tw;        // [fft_table_size(lg)] cpx<R>: the FFT passes' twiddles
   const double* y0;      // [n_inst][n] complex: AW at zs[0]
   double* out;           // [n_inst][nsaves][n] complex: AW at each requested point
   long long* stats;      // [n_inst][4]: accepted, rejected, evaluations, status
@@ -106,79 +107,141 @@ OFX_DEV double eps_of(double t) {
   return ldexp(1.0, e - 53);
 }
 
-// exp(-2 pi i m / n) for 0 <= m < n from the quarter table q[r] = exp(-2 pi i r / n),
-// r < n / 4: m = k n/4 + r gives q[r] (-i)^k.
+// Twiddles of the radix-4 passes: for each pass with Ns > 1, in pass order,
+// Ns entries exp(-2 pi i k / (4 Ns)), k < Ns, at offset (Ns - Ns0) / 3, where
+// Ns0 is the first such pass's Ns (2 for odd lg, 4 for even lg); (n - Ns0) / 3
+// entries in all.  Consecutive butterflies read consecutive entries.
+#ifdef OFX_EMU
+#define OFX_HD inline
+#else
+#define OFX_HD __host__ __device__ inline
+#endif
+OFX_HD int fft_ns0(int lg) { return (lg & 1) ? 2 : 4; }
+OFX_HD int fft_table_size(int lg) { return ((1 << lg) - fft_ns0(lg)) / 3; }
+
 template <typename R>
-OFX_DEV cpx<R> tw_quarter(const cpx<R>* q, int m, int lg) {
-  const int k = m >> (lg - 2), r = m & ((1 << (lg - 2)) - 1);
-  const cpx<R> v = q[r];
-  switch (k & 3) {
-    case 0: return v;
-    case 1: return mkc<R>(v.y, -v.x);    // -i v
-    case 2: return mkc<R>(-v.x, -v.y);   // -v
-    default: return mkc<R>(-v.y, v.x);   // i v
-  }
+std::vector<cpx<R>> fft_stage_twiddles(int lg) {
+  const int L = 1 << lg, ns0 = fft_ns0(lg);
+  std::vector<cpx<R>> t((size_t)fft_table_size(lg));
+  for (int ns = ns0; ns <= L / 4; ns <<= 2)
+    for (int k = 0; k < ns; k++) {
+      const double ph = -2.0 * M_PI * (double)k / (4.0 * (double)ns);
+      cpx<R>& e = t[(size_t)((ns - ns0) / 3 + k)];
+      e.x = (R)cos(ph);
+      e.y = (R)sin(ph);
+    }
+  return t;
 }
 
-// One length-2^lg Stockham FFT in LDS (radix-4 passes, one radix-2 pass for
-// odd lg), ping-ponging between `in` and `out`; returns the buffer holding
-// the result.  SIGN = -1 forward, +1 inverse, both unnormalised.  Twiddles
-// from the quarter table in LDS (lg >= 2).
-template <typename R, int SIGN>
-OFX_DEV cpx<R>* fft_q(cpx<R>* in, cpx<R>* out, int lg, const cpx<R>* q) {
+// Radix-4 butterfly on (v0, v1, v2, v3) in place, the inputs v1..v3 first
+// multiplied by w1, w1^2, w1^3 (conjugated for the inverse) when TW.
+template <typename R, int SIGN, bool TW>
+OFX_DEV void bfly4(cpx<R>& v0, cpx<R>& v1, cpx<R>& v2, cpx<R>& v3, cpx<R> w1) {
   typedef cpx<R> C;
-  const int L = 1 << lg;
+  if (TW) {
+    const C w2 = cmul(w1, w1), w3 = cmul(w2, w1);
+    if (SIGN > 0) {
+      v1 = cmulc(v1, w1);
+      v2 = cmulc(v2, w2);
+      v3 = cmulc(v3, w3);
+    } else {
+      v1 = cmul(v1, w1);
+      v2 = cmul(v2, w2);
+      v3 = cmul(v3, w3);
+    }
+  }
+  const C s02 = cadd(v0, v2), d02 = csub(v0, v2), s13 = cadd(v1, v3), d13 = csub(v1, v3);
+  const C rd = (SIGN < 0) ? mkc<R>(d13.y, -d13.x) : mkc<R>(-d13.y, d13.x);
+  v0 = cadd(s02, s13);
+  v1 = cadd(d02, rd);
+  v2 = csub(s02, s13);
+  v3 = csub(d02, rd);
+}
+
+// One length-2^lg Stockham FFT in LDS (one radix-2 pass for odd lg, then
+// radix-4 passes), SIGN = -1 forward, +1 inverse, both unnormalised.  The
+// pointwise work on either side is fused into the first and last passes:
+// ld(i) gives input element i, st(o, i, v) takes output element i, with o
+// the LDS buffer the last pass may write (returned).  Pass p writes
+// b[p & 1]; ld may read LDS only in b1.  Every pass ends in a barrier.
+template <typename R, int SIGN, typename Ld, typename St>
+OFX_DEV cpx<R>* fft_fused(int lg, cpx<R>* b0, cpx<R>* b1, const cpx<R>* tw, Ld ld, St st) {
+  typedef cpx<R> C;
+  const int L = 1 << lg, q4 = L >> 2;
   const int tid = OFX_TID, nt = OFX_NT;
-  int Ns = 1;
-  if (lg & 1) {
-    const int h = L >> 1;
-#pragma unroll 4
-    for (int j = tid; j < h; j += nt) {
-      const C v0 = in[j], v1 = in[j + h];
-      out[2 * j] = cadd(v0, v1);
-      out[2 * j + 1] = csub(v0, v1);
+  const int np = (lg & 1) + (lg >> 1);
+  cpx<R>* const bufs[2] = {b0, b1};
+  C* const outbuf = bufs[(np - 1) & 1];
+  const C one = mkc<R>(R(1), R(0));
+  if (np == 1) {  // n = 4: one pass from ld to st
+    if (tid == 0) {
+      C v0 = ld(0), v1 = ld(1), v2 = ld(2), v3 = ld(3);
+      bfly4<R, SIGN, false>(v0, v1, v2, v3, one);
+      st(outbuf, 0, v0);
+      st(outbuf, 1, v1);
+      st(outbuf, 2, v2);
+      st(outbuf, 3, v3);
     }
     OFX_SYNC();
-    C* t = in;
-    in = out;
-    out = t;
-    Ns = 2;
+    return outbuf;
   }
-  const int q4 = L >> 2;
-  for (; Ns < L; Ns <<= 2) {
-    const int span = L / (4 * Ns);
-#pragma unroll 4
+  // first pass (Ns = 1): from ld into b0
+  if (lg & 1) {
+    const int h = L >> 1;
+#pragma unroll 1
+    for (int j = tid; j < h; j += nt) {
+      const C v0 = ld(j), v1 = ld(j + h);
+      b0[2 * j] = cadd(v0, v1);
+      b0[2 * j + 1] = csub(v0, v1);
+    }
+  } else {
+#pragma unroll 1
+    for (int j = tid; j < q4; j += nt) {
+      C v0 = ld(j), v1 = ld(j + q4), v2 = ld(j + 2 * q4), v3 = ld(j + 3 * q4);
+      bfly4<R, SIGN, false>(v0, v1, v2, v3, one);
+      b0[4 * j] = v0;
+      b0[4 * j + 1] = v1;
+      b0[4 * j + 2] = v2;
+      b0[4 * j + 3] = v3;
+    }
+  }
+  OFX_SYNC();
+  // middle passes, LDS to LDS
+  const int ns0 = fft_ns0(lg);
+  int p = 1, Ns = (lg & 1) ? 2 : 4;
+  for (; Ns < q4; Ns <<= 2, p++) {
+    const C* in = bufs[(p + 1) & 1];
+    C* o = bufs[p & 1];
+    const C* tws = tw + (Ns - ns0) / 3;
+#pragma unroll 2
     for (int j = tid; j < q4; j += nt) {
       const int k = j & (Ns - 1);
       C v0 = in[j], v1 = in[j + q4], v2 = in[j + 2 * q4], v3 = in[j + 3 * q4];
-      if (Ns > 1) {
-        const int m = k * span;
-        const C w1 = tw_quarter(q, m, lg), w2 = tw_quarter(q, 2 * m, lg),
-                w3 = tw_quarter(q, 3 * m, lg);
-        if (SIGN > 0) {
-          v1 = cmulc(v1, w1);
-          v2 = cmulc(v2, w2);
-          v3 = cmulc(v3, w3);
-        } else {
-          v1 = cmul(v1, w1);
-          v2 = cmul(v2, w2);
-          v3 = cmul(v3, w3);
-        }
-      }
-      const C s02 = cadd(v0, v2), d02 = csub(v0, v2), s13 = cadd(v1, v3), d13 = csub(v1, v3);
-      const C rd = (SIGN < 0) ? mkc<R>(d13.y, -d13.x) : mkc<R>(-d13.y, d13.x);
+      bfly4<R, SIGN, true>(v0, v1, v2, v3, tws[k]);   // tws[k] = exp(-2 pi i k / (4 Ns))
       const int base = ((j - k) << 2) + k;
-      out[base] = cadd(s02, s13);
-      out[base + Ns] = cadd(d02, rd);
-      out[base + 2 * Ns] = csub(s02, s13);
-      out[base + 3 * Ns] = csub(d02, rd);
+      o[base] = v0;
+      o[base + Ns] = v1;
+      o[base + 2 * Ns] = v2;
+      o[base + 3 * Ns] = v3;
     }
     OFX_SYNC();
-    C* t = in;
-    in = out;
-    out = t;
   }
-  return in;
+  // last pass (Ns = n / 4): element j + c n/4 of the result to st
+  {
+    const C* in = bufs[(np - 2) & 1];
+    const C* tws = tw + (q4 - ns0) / 3;
+#pragma unroll 1
+    for (int j = tid; j < q4; j += nt) {
+      C v0 = in[j], v1 = in[j + q4], v2 = in[j + 2 * q4], v3 = in[j + 3 * q4];
+      bfly4<R, SIGN, true>(v0, v1, v2, v3, tws[j]);
+      st(outbuf, j, v0);
+      st(outbuf, j + q4, v1);
+      st(outbuf, j + 2 * q4, v2);
+      st(outbuf, j + 3 * q4, v3);
+    }
+    OFX_SYNC();
+  }
+  return outbuf;
 }
 
 template <typename R>
@@ -192,7 +255,7 @@ struct GnlseInst {
   double lre;
   const R* W;
   const cpx<R>* RW;
-  const cpx<R>* tw;  // quarter twiddle table (LDS)
+  const cpx<R>* tw;  // pass twiddle tables (LDS)
   double gamma, fr, dT;
   cpx<R>* at;  // AT of the current evaluation
 };
@@ -203,63 +266,57 @@ OFX_DEV cpx<R> phase_factor(const double* B, int i, double z) {
   return expi(B[i] * z, R(0));
 }
 
-// f = rhs(z, y) (gnlsefib.m:49-61); y, f in global memory
+// f = rhs(z, y) (gnlsefib.m:49-61); y, f in global memory.  The pointwise
+// steps run inside the first and last passes of the FFTs around them.
 template <typename R>
 OFX_DEV void gnlse_rhs(const GnlseInst<R>& c, double z, const cpx<R>* y, cpx<R>* f) {
   typedef cpx<R> C;
-  const int n = c.n, tid = OFX_TID, nt = OFX_NT;
-  const R inv_n = (R)(1.0 / (double)n);
+  const int lg = c.lg;
+  const R inv_n = (R)(1.0 / (double)c.n);
   // exp(L z) = exp(Re L z) exp(i B z); Re L is one number per fibre
-  const R gf = (R)exp(c.lre * z), gb = (R)(exp(-c.lre * z) / (double)n);
-#pragma unroll 4
-  for (int i = tid; i < n; i += nt) c.lds0[i] = cscale(cmul(y[i], phase_factor<R>(c.B, i, z)), gf);
-  OFX_SYNC();
-  C* P = fft_q<R, -1>(c.lds0, c.lds1, c.lg, c.tw);   // AT = fft(.)
-  C* Q = P == c.lds0 ? c.lds1 : c.lds0;
-  for (int i = tid; i < n; i += nt) {
-    const C v = P[i];
-    c.at[i] = v;
-    Q[i] = mkc<R>(v.x * v.x + v.y * v.y, R(0));
-  }
-  OFX_SYNC();
-  C* M;
-  if (c.RW) {
-    C* X = fft_q<R, 1>(Q, P, c.lg, c.tw);              // n ifft(IT)
-#pragma unroll 4
-    for (int i = tid; i < n; i += nt) {
-      X[i] = cscale(cmul(X[i], c.RW[i]), inv_n);
-    }
-    OFX_SYNC();
-    C* Xo = X == c.lds0 ? c.lds1 : c.lds0;
-    C* Y = fft_q<R, -1>(X, Xo, c.lg, c.tw);             // fft(ifft(IT) RW)
-    const R dfr = (R)(c.dT * c.fr), one_fr = (R)(1.0 - c.fr);
-#pragma unroll 4
-    for (int i = tid; i < n; i += nt) {
-      const C a = c.at[i];
-      const R it = a.x * a.x + a.y * a.y;
-      const C rs = cscale(Y[i], dfr);
-      Y[i] = cmul(a, mkc<R>(one_fr * it + rs.x, rs.y));
-    }
-    OFX_SYNC();
-    M = fft_q<R, 1>(Y, Y == c.lds0 ? c.lds1 : c.lds0, c.lg, c.tw);
-  } else {
-#pragma unroll 4
-    for (int i = tid; i < n; i += nt) {
-      const C a = c.at[i];
-      Q[i] = cscale(a, a.x * a.x + a.y * a.y);
-    }
-    OFX_SYNC();
-    M = fft_q<R, 1>(Q, P, c.lg, c.tw);
-  }
-  // rhs = i gamma W (M / n) exp(-L z)
+  const R gf = (R)exp(c.lre * z), gb = (R)(exp(-c.lre * z) / (double)c.n);
+  const double* B = c.B;
+  C* const at = c.at;
+  // AT = fft(AW exp(L z))
+  auto ld_y = [=](int i) { return cscale(cmul(y[i], phase_factor<R>(B, i, z)), gf); };
+  // rhs = i gamma W (M / n) exp(-L z), M = n ifft(.)
   const R gW = (R)c.gamma;
-#pragma unroll 4
-  for (int i = tid; i < n; i += nt) {
-    const C m = cscale(cmulc(M[i], phase_factor<R>(c.B, i, z)), gb);
-    const R s = gW * c.W[i];
+  const R* W = c.W;
+  auto st_f = [=](C*, int i, C v) {
+    const C m = cscale(cmulc(v, phase_factor<R>(B, i, z)), gb);
+    const R s = gW * W[i];
     f[i] = mkc<R>(-s * m.y, s * m.x);
+  };
+  C* X;
+  if (c.RW) {
+    // IT = |AT|^2 (AT kept for the last product)
+    X = fft_fused<R, -1>(lg, c.lds0, c.lds1, c.tw, ld_y, [=](C* o, int i, C v) {
+      at[i] = v;
+      o[i] = mkc<R>(v.x * v.x + v.y * v.y, R(0));
+    });
+    // n ifft(IT) RW / n
+    const C* RW = c.RW;
+    auto ld_x = [=](int i) { return X[i]; };
+    C* Xn = X == c.lds0 ? c.lds1 : c.lds0;
+    C* Y = fft_fused<R, 1>(lg, Xn, X, c.tw, ld_x,
+                           [=](C* o, int i, C v) { o[i] = cscale(cmul(v, RW[i]), inv_n); });
+    // AT ((1 - fr) IT + dT fr fft(ifft(IT) RW))
+    const R dfr = (R)(c.dT * c.fr), one_fr = (R)(1.0 - c.fr);
+    auto ld_yv = [=](int i) { return Y[i]; };
+    C* Yn = Y == c.lds0 ? c.lds1 : c.lds0;
+    X = fft_fused<R, -1>(lg, Yn, Y, c.tw, ld_yv, [=](C* o, int i, C v) {
+      const C a = at[i];
+      const R it = a.x * a.x + a.y * a.y;
+      const C rs = cscale(v, dfr);
+      o[i] = cmul(a, mkc<R>(one_fr * it + rs.x, rs.y));
+    });
+  } else {
+    // AT |AT|^2
+    X = fft_fused<R, -1>(lg, c.lds0, c.lds1, c.tw, ld_y,
+                         [=](C* o, int i, C v) { o[i] = cscale(v, v.x * v.x + v.y * v.y); });
   }
-  OFX_SYNC();
+  auto ld_m = [=](int i) { return X[i]; };
+  (void)fft_fused<R, 1>(lg, X == c.lds0 ? c.lds1 : c.lds0, X, c.tw, ld_m, st_f);
 }
 
 // dst = y + sum_j cf[j] f_j (in stage order, zero coefficients skipped)
@@ -344,7 +401,7 @@ OFX_DEV void gnlse_instance(const GnlseArgs& a, int inst, cpx<R>* lds0, cpx<R>* 
   c.lre = a.lre[inst];
   c.W = (const R*)a.W + (size_t)inst * n;
   c.RW = a.RW ? (const C*)a.RW + (size_t)inst * n : nullptr;
-  for (int i = OFX_TID; i < (n >> 2); i += OFX_NT) ldstw[i] = ((const C*)a.tw)[i];
+  for (int i = OFX_TID; i < fft_table_size(a.lg); i += OFX_NT) ldstw[i] = ((const C*)a.tw)[i];
   c.tw = ldstw;
 
   c.gamma = a.gamma[inst];
@@ -394,20 +451,29 @@ OFX_DEV void gnlse_instance(const GnlseArgs& a, int inst, cpx<R>* lds0, cpx<R>* 
     bool nofailed = true;
     double err = 0.0, normynew = 0.0, tnew = t;
     for (;;) {
-      double cf[6];
-      for (int k = 0; k < 5; k++) {
-        for (int j = 0; j <= k; j++) cf[j] = h * DP_B[j][k];
-        stage_comb(n, y, f, cf, k + 1, ys);
+      // stages 2..7: one copy of the right-hand side in the code
+#pragma unroll 1
+      for (int k = 0; k < 6; k++) {
+        const double hk = h;
+        double tk;
+        C* dst;
+        if (k < 5) {
+          tk = t + h * DP_A[k];
+          dst = ys;
+        } else {
+          tnew = t + h * DP_A[5];
+          if (done) tnew = tfinal;
+          h = tnew - t;
+          tk = tnew;
+          dst = ynew;
+        }
+        double cf[6];
+#pragma unroll
+        for (int j = 0; j < 6; j++) cf[j] = j <= k ? hk * DP_B[j][k] : 0.0;
+        stage_comb(n, y, f, cf, k + 1, dst);
         OFX_SYNC();
-        gnlse_rhs(c, t + h * DP_A[k], ys, f[k + 1]);
+        gnlse_rhs(c, tk, dst, k < 5 ? ws + (size_t)(k + 1) * n : f[6]);
       }
-      tnew = t + h * DP_A[5];
-      if (done) tnew = tfinal;
-      for (int j = 0; j < 6; j++) cf[j] = h * DP_B[j][5];
-      h = tnew - t;
-      stage_comb(n, y, f, cf, 6, ynew);
-      OFX_SYNC();
-      gnlse_rhs(c, tnew, ynew, f[6]);
       nfev += 6;
       normynew = norm2(n, ynew, red);
       const double errwt = fmax(fmax(normy, normynew), threshold);

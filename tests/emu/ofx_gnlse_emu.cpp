@@ -18,12 +18,7 @@ int run(int n_inst, int n, int lg, int nsaves, const double* zs, const double* y
         const double* dT, double rtol, double atol, long long max_steps, double* out,
         long long* stats) {
   typedef ofx::cpx<R> C;
-  std::vector<C> tw(n / 4);   // quarter table exp(-2 pi i m / n), m < n / 4
-  for (int m = 0; m < n / 4; m++) {
-    const double ph = -2.0 * M_PI * (double)m / (double)n;
-    tw[m].x = (R)cos(ph);
-    tw[m].y = (R)sin(ph);
-  }
+  const std::vector<C> tw = ofx::fft_stage_twiddles<R>(lg);
   std::vector<double> hB, hlre;
   std::vector<R> hW;
   std::vector<C> hRW;
@@ -52,8 +47,8 @@ int run(int n_inst, int n, int lg, int nsaves, const double* zs, const double* y
   a.stats = stats;
   std::vector<std::thread> th;
   for (int i = 0; i < n_inst; i++)
-    th.emplace_back([&a, i, n]() {
-      std::vector<C> l0(n), l1(n), tq(n / 4);
+    th.emplace_back([&a, &tw, i, n]() {
+      std::vector<C> l0(n), l1(n), tq(tw.size() + 1);
       double red[16];
       ofx::gnlse_instance<R>(a, i, l0.data(), l1.data(), tq.data(), red);
     });
