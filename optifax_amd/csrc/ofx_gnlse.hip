@@ -1,7 +1,7 @@
 // Fibre GNLSE solver on gfx950 (gnlsefib.m): one workgroup per fibre
 // instance, the whole field in LDS (two buffers of n complex samples for the
-// Stockham FFTs and n / 3 twiddles: n <= 2^13 in FP32, 2^12 in FP64 within
-// the 160 KB of LDS),
+// Stockham FFTs and up to 0.42 n twiddles: n <= 2^13 in FP32, 2^12 in FP64
+// within the 160 KB of LDS),
 // Dormand-Prince 5(4) with ode45's step control, outputs at the requested
 // points.  The body is ofx_gnlse_impl.h.
 #include <hip/hip_runtime.h>
@@ -53,7 +53,7 @@ int gnlse_run(int n_inst, int n, int lg, int nsaves, const double* zs, const dou
               double* out, long long* stats, double* ms) {
   typedef ofx::cpx<R> C;
   // the FFT passes' twiddles, built in FP64 and rounded once
-  const std::vector<C> tw = ofx::fft_stage_twiddles<R>(lg);
+  const std::vector<C> tw = ofx::fft_twiddles<R>(lg);
   const size_t ni = (size_t)n_inst, nn = (size_t)n;
   std::vector<double> hB, hlre;
   std::vector<R> hW;

@@ -7,10 +7,11 @@
 //   RS = dT fr fft(ifft(IT) RW)                        (Raman, when given)
 //   M  = ifft(AT ((1 - fr) IT + RS))   or  ifft(AT IT)  (no Raman)
 //   rhs = i gamma W M exp(-L z),  L = i B - alpha / 2
-// four (two without Raman) length-n FFTs per evaluation, each a radix-4
-// Stockham transform in LDS (fft_fused) over the workgroup's two LDS buffers
-// with the pointwise steps in its first and last passes, twiddles from
-// per-pass tables in LDS.
+// per evaluation two length-n FFTs and, with Raman, two length-n/2 FFTs
+// (the real convolution as half-length transforms of packed data), each a
+// radix-4 Stockham transform in LDS (fft_fused) over the workgroup's two LDS
+// buffers with the pointwise steps in its first and last passes, twiddles
+// from per-pass tables in LDS.
 //
 // Integration: the Dormand-Prince 5(4) pair with the step control of
 // MATLAB's ode45 at NormControl on (Shampine & Reichelt 1997), outputs at
@@ -107,30 +108,50 @@ OFX_DEV double eps_of(double t) {
   return ldexp(1.0, e - 53);
 }
 
-// Twiddles of the radix-4 passes: for each pass with Ns > 1, in pass order,
-// Ns entries exp(-2 pi i k / (4 Ns)), k < Ns, at offset (Ns - Ns0) / 3, where
-// Ns0 is the first such pass's Ns (2 for odd lg, 4 for even lg); (n - Ns0) / 3
-// entries in all.  Consecutive butterflies read consecutive entries.
+// Twiddle tables in LDS.  The radix-4 pass with stride Ns > 1 reads Ns
+// entries exp(-2 pi i k / (4 Ns)), k < Ns, at offset (Ns - 4) / 3, for Ns =
+// 4, 16, ... up to the largest power of 4 not above n / 4 (consecutive
+// butterflies read consecutive entries).  The quarter table qt[m] =
+// exp(-2 pi i m / n), m < n / 4, serves the radix-2 pass of odd lengths and
+// the real-data step of the Raman convolution: for even lg >= 4 it is the
+// last radix-4 table, otherwise it follows the radix-4 tables.
 #ifdef OFX_EMU
 #define OFX_HD inline
 #else
 #define OFX_HD __host__ __device__ inline
 #endif
-OFX_HD int fft_ns0(int lg) { return (lg & 1) ? 2 : 4; }
-OFX_HD int fft_table_size(int lg) { return ((1 << lg) - fft_ns0(lg)) / 3; }
+OFX_HD int fft_r4_top(int lg) { return 1 << (2 * ((lg - 2) >> 1)); }
+OFX_HD bool fft_qt_shared(int lg) { return !(lg & 1) && lg >= 4; }
+OFX_HD int fft_qt_offset(int lg) {
+  const int top = fft_r4_top(lg);
+  return fft_qt_shared(lg) ? (top - 4) / 3 : (4 * top - 4) / 3;
+}
+OFX_HD int fft_table_size(int lg) {
+  return (4 * fft_r4_top(lg) - 4) / 3 + (fft_qt_shared(lg) ? 0 : (1 << lg) / 4);
+}
 
 template <typename R>
-std::vector<cpx<R>> fft_stage_twiddles(int lg) {
-  const int L = 1 << lg, ns0 = fft_ns0(lg);
+std::vector<cpx<R>> fft_twiddles(int lg) {
+  const int n = 1 << lg, top = fft_r4_top(lg);
   std::vector<cpx<R>> t((size_t)fft_table_size(lg));
-  for (int ns = ns0; ns <= L / 4; ns <<= 2)
-    for (int k = 0; k < ns; k++) {
-      const double ph = -2.0 * M_PI * (double)k / (4.0 * (double)ns);
-      cpx<R>& e = t[(size_t)((ns - ns0) / 3 + k)];
-      e.x = (R)cos(ph);
-      e.y = (R)sin(ph);
-    }
+  auto put = [&](int at, double ph) {
+    t[(size_t)at].x = (R)cos(ph);
+    t[(size_t)at].y = (R)sin(ph);
+  };
+  for (int ns = 4; ns <= top; ns <<= 2)
+    for (int k = 0; k < ns; k++) put((ns - 4) / 3 + k, -2.0 * M_PI * (double)k / (4.0 * ns));
+  if (!fft_qt_shared(lg))
+    for (int m = 0; m < n / 4; m++) put(fft_qt_offset(lg) + m, -2.0 * M_PI * (double)m / n);
   return t;
+}
+
+// exp(-2 pi i k / L) for k < L / 2 from the quarter table of the length n = L s
+template <typename R>
+OFX_DEV cpx<R> tw_half(const cpx<R>* qt, int n, int s, int k) {
+  const int m = k * s, q = n >> 2;
+  if (m < q) return qt[m];
+  const cpx<R> v = qt[m - q];
+  return mkc<R>(v.y, -v.x);   // -i v
 }
 
 // Radix-4 butterfly on (v0, v1, v2, v3) in place, the inputs v1..v3 first
@@ -158,61 +179,60 @@ OFX_DEV void bfly4(cpx<R>& v0, cpx<R>& v1, cpx<R>& v2, cpx<R>& v3, cpx<R> w1) {
   v3 = csub(d02, rd);
 }
 
-// One length-2^lg Stockham FFT in LDS (one radix-2 pass for odd lg, then
-// radix-4 passes), SIGN = -1 forward, +1 inverse, both unnormalised.  The
-// pointwise work on either side is fused into the first and last passes:
-// ld(i) gives input element i, st(o, i, v) takes output element i, with o
-// the LDS buffer the last pass may write (returned).  Pass p writes
-// b[p & 1]; ld may read LDS only in b1.  Every pass ends in a barrier.
+// One length-2^lg Stockham FFT in LDS: radix-4 passes (Ns = 1, 4, ...) and,
+// for odd lg, a last radix-2 pass (Ns = L / 2, twiddles from the quarter
+// table qt of the length L qs).  SIGN = -1 forward, +1 inverse, both
+// unnormalised.  The pointwise work on either side is fused into the first
+// and last passes: ld(i) gives input element i, st(o, i, v) takes output
+// element i, with o the LDS buffer the last pass may write (returned).  Pass
+// p writes b[p & 1]; ld may read LDS only in b1.  Every pass ends in a
+// barrier.
 template <typename R, int SIGN, typename Ld, typename St>
-OFX_DEV cpx<R>* fft_fused(int lg, cpx<R>* b0, cpx<R>* b1, const cpx<R>* tw, Ld ld, St st) {
+OFX_DEV cpx<R>* fft_fused(int lg, cpx<R>* b0, cpx<R>* b1, const cpx<R>* tw, const cpx<R>* qt,
+                          int qs, Ld ld, St st) {
   typedef cpx<R> C;
-  const int L = 1 << lg, q4 = L >> 2;
+  const int L = 1 << lg, q4 = L >> 2, h = L >> 1;
   const int tid = OFX_TID, nt = OFX_NT;
-  const int np = (lg & 1) + (lg >> 1);
+  const int n4 = lg >> 1, np = n4 + (lg & 1);
   cpx<R>* const bufs[2] = {b0, b1};
   C* const outbuf = bufs[(np - 1) & 1];
   const C one = mkc<R>(R(1), R(0));
-  if (np == 1) {  // n = 4: one pass from ld to st
+  if (np == 1) {  // L = 2 or 4: one pass from ld to st
     if (tid == 0) {
-      C v0 = ld(0), v1 = ld(1), v2 = ld(2), v3 = ld(3);
-      bfly4<R, SIGN, false>(v0, v1, v2, v3, one);
-      st(outbuf, 0, v0);
-      st(outbuf, 1, v1);
-      st(outbuf, 2, v2);
-      st(outbuf, 3, v3);
+      if (lg == 1) {
+        const C v0 = ld(0), v1 = ld(1);
+        st(outbuf, 0, cadd(v0, v1));
+        st(outbuf, 1, csub(v0, v1));
+      } else {
+        C v0 = ld(0), v1 = ld(1), v2 = ld(2), v3 = ld(3);
+        bfly4<R, SIGN, false>(v0, v1, v2, v3, one);
+        st(outbuf, 0, v0);
+        st(outbuf, 1, v1);
+        st(outbuf, 2, v2);
+        st(outbuf, 3, v3);
+      }
     }
     OFX_SYNC();
     return outbuf;
   }
-  // first pass (Ns = 1): from ld into b0
-  if (lg & 1) {
-    const int h = L >> 1;
+  // first pass (radix 4, Ns = 1): from ld into b0
 #pragma unroll 1
-    for (int j = tid; j < h; j += nt) {
-      const C v0 = ld(j), v1 = ld(j + h);
-      b0[2 * j] = cadd(v0, v1);
-      b0[2 * j + 1] = csub(v0, v1);
-    }
-  } else {
-#pragma unroll 1
-    for (int j = tid; j < q4; j += nt) {
-      C v0 = ld(j), v1 = ld(j + q4), v2 = ld(j + 2 * q4), v3 = ld(j + 3 * q4);
-      bfly4<R, SIGN, false>(v0, v1, v2, v3, one);
-      b0[4 * j] = v0;
-      b0[4 * j + 1] = v1;
-      b0[4 * j + 2] = v2;
-      b0[4 * j + 3] = v3;
-    }
+  for (int j = tid; j < q4; j += nt) {
+    C v0 = ld(j), v1 = ld(j + q4), v2 = ld(j + 2 * q4), v3 = ld(j + 3 * q4);
+    bfly4<R, SIGN, false>(v0, v1, v2, v3, one);
+    b0[4 * j] = v0;
+    b0[4 * j + 1] = v1;
+    b0[4 * j + 2] = v2;
+    b0[4 * j + 3] = v3;
   }
   OFX_SYNC();
-  // middle passes, LDS to LDS
-  const int ns0 = fft_ns0(lg);
-  int p = 1, Ns = (lg & 1) ? 2 : 4;
-  for (; Ns < q4; Ns <<= 2, p++) {
+  // radix-4 passes LDS to LDS (all the others for odd lg, all but the last for even)
+  const int mid_end = (lg & 1) ? n4 : n4 - 1;
+  int Ns = 4;
+  for (int p = 1; p < mid_end; p++, Ns <<= 2) {
     const C* in = bufs[(p + 1) & 1];
     C* o = bufs[p & 1];
-    const C* tws = tw + (Ns - ns0) / 3;
+    const C* tws = tw + (Ns - 4) / 3;
 #pragma unroll 2
     for (int j = tid; j < q4; j += nt) {
       const int k = j & (Ns - 1);
@@ -226,10 +246,18 @@ OFX_DEV cpx<R>* fft_fused(int lg, cpx<R>* b0, cpx<R>* b1, const cpx<R>* tw, Ld l
     }
     OFX_SYNC();
   }
-  // last pass (Ns = n / 4): element j + c n/4 of the result to st
-  {
-    const C* in = bufs[(np - 2) & 1];
-    const C* tws = tw + (q4 - ns0) / 3;
+  // last pass: element i of the result to st
+  const C* in = bufs[(np - 2) & 1];
+  if (lg & 1) {  // radix 2, Ns = L / 2
+#pragma unroll 1
+    for (int j = tid; j < h; j += nt) {
+      const C w = tw_half(qt, L * qs, qs, j);   // exp(-2 pi i j / L)
+      const C v0 = in[j], v1 = SIGN > 0 ? cmulc(in[j + h], w) : cmul(in[j + h], w);
+      st(outbuf, j, cadd(v0, v1));
+      st(outbuf, j + h, csub(v0, v1));
+    }
+  } else {  // radix 4, Ns = L / 4
+    const C* tws = tw + (q4 - 4) / 3;
 #pragma unroll 1
     for (int j = tid; j < q4; j += nt) {
       C v0 = in[j], v1 = in[j + q4], v2 = in[j + 2 * q4], v3 = in[j + 3 * q4];
@@ -239,8 +267,8 @@ OFX_DEV cpx<R>* fft_fused(int lg, cpx<R>* b0, cpx<R>* b1, const cpx<R>* tw, Ld l
       st(outbuf, j + 2 * q4, v2);
       st(outbuf, j + 3 * q4, v3);
     }
-    OFX_SYNC();
   }
+  OFX_SYNC();
   return outbuf;
 }
 
@@ -255,7 +283,8 @@ struct GnlseInst {
   double lre;
   const R* W;
   const cpx<R>* RW;
-  const cpx<R>* tw;  // pass twiddle tables (LDS)
+  const cpx<R>* tw;  // radix-4 twiddle tables (LDS)
+  const cpx<R>* qt;  // quarter table exp(-2 pi i m / n), m < n / 4 (LDS)
   double gamma, fr, dT;
   cpx<R>* at;  // AT of the current evaluation
 };
@@ -268,15 +297,28 @@ OFX_DEV cpx<R> phase_factor(const double* B, int i, double z) {
 
 // f = rhs(z, y) (gnlsefib.m:49-61); y, f in global memory.  The pointwise
 // steps run inside the first and last passes of the FFTs around them.
+//
+// The Raman term fft(ifft(IT) RW) is a circular convolution of two real
+// sequences (IT = |AT|^2, and RW the spectrum of the real response), so its
+// result is real, and both of its transforms run at half length on the
+// packed sequence z[m] = IT[2m] + i IT[2m+1]:
+//   Zi = sum_m z[m] e^{+2 pi i m k / h}, h = n / 2 (one inverse transform)
+//   E = (Zi[k] + conj Zi[h-k]) / 2,  O = (Zi[k] - conj Zi[h-k]) / 2i
+//   U[k] = E + e^{2 pi i k / n} O,  U[k+h] = E - e^{2 pi i k / n} O   (= n ifft(IT))
+//   X = U RW / n;  P = X[k] + X[k+h],  Q = (X[k] - X[k+h]) e^{-2 pi i k / n}
+//   sum_k (P + i Q) e^{-2 pi i m k / h} = RS'[2m] + i RS'[2m+1]  (one forward transform)
+// with RS' = fft(X).  The real part is the whole result: the imaginary part
+// a full-length complex evaluation carries is rounding.
 template <typename R>
 OFX_DEV void gnlse_rhs(const GnlseInst<R>& c, double z, const cpx<R>* y, cpx<R>* f) {
   typedef cpx<R> C;
-  const int lg = c.lg;
-  const R inv_n = (R)(1.0 / (double)c.n);
+  const int lg = c.lg, n = c.n, h = n >> 1;
+  const R inv_n = (R)(1.0 / (double)n);
   // exp(L z) = exp(Re L z) exp(i B z); Re L is one number per fibre
-  const R gf = (R)exp(c.lre * z), gb = (R)(exp(-c.lre * z) / (double)c.n);
+  const R gf = (R)exp(c.lre * z), gb = (R)(exp(-c.lre * z) / (double)n);
   const double* B = c.B;
   C* const at = c.at;
+  const C* qt = c.qt;
   // AT = fft(AW exp(L z))
   auto ld_y = [=](int i) { return cscale(cmul(y[i], phase_factor<R>(B, i, z)), gf); };
   // rhs = i gamma W (M / n) exp(-L z), M = n ifft(.)
@@ -289,34 +331,51 @@ OFX_DEV void gnlse_rhs(const GnlseInst<R>& c, double z, const cpx<R>* y, cpx<R>*
   };
   C* X;
   if (c.RW) {
-    // IT = |AT|^2 (AT kept for the last product)
-    X = fft_fused<R, -1>(lg, c.lds0, c.lds1, c.tw, ld_y, [=](C* o, int i, C v) {
+    // AT kept for the last product; IT = |AT|^2 as n reals in the low half of X
+    X = fft_fused<R, -1>(lg, c.lds0, c.lds1, c.tw, qt, 1, ld_y, [=](C* o, int i, C v) {
       at[i] = v;
-      o[i] = mkc<R>(v.x * v.x + v.y * v.y, R(0));
+      reinterpret_cast<R*>(o)[i] = v.x * v.x + v.y * v.y;
     });
-    // n ifft(IT) RW / n
+    C* const Y = X == c.lds0 ? c.lds1 : c.lds0;
+    // Zi: half-length inverse transform of z within X
+    C* const Xz = X;
+    C* Zi = fft_fused<R, 1>(lg - 1, X + h, X, c.tw, qt, 2, [=](int m) { return Xz[m]; },
+                            [](C* o, int m, C v) { o[m] = v; });
+    C* const Zp = Zi == X ? X + h : X;
     const C* RW = c.RW;
-    auto ld_x = [=](int i) { return X[i]; };
-    C* Xn = X == c.lds0 ? c.lds1 : c.lds0;
-    C* Y = fft_fused<R, 1>(lg, Xn, X, c.tw, ld_x,
-                           [=](C* o, int i, C v) { o[i] = cscale(cmul(v, RW[i]), inv_n); });
-    // AT ((1 - fr) IT + dT fr fft(ifft(IT) RW))
+    const int tid = OFX_TID, nt = OFX_NT;
+    for (int k = tid; k < h; k += nt) {
+      const C a = Zi[k], b = Zi[k == 0 ? 0 : h - k];
+      const C bc = mkc<R>(b.x, -b.y);
+      const C E = cscale(cadd(a, bc), R(0.5));
+      const C d = csub(a, bc);
+      const C O = mkc<R>(R(0.5) * d.y, R(-0.5) * d.x);   // d / 2i
+      const C w = tw_half(qt, n, 1, k);                    // exp(-2 pi i k / n)
+      const C wO = cmulc(O, w);
+      const C x0 = cscale(cmul(cadd(E, wO), RW[k]), inv_n);
+      const C x1 = cscale(cmul(csub(E, wO), RW[k + h]), inv_n);
+      const C P = cadd(x0, x1), Q = cmul(csub(x0, x1), w);
+      Zp[k] = mkc<R>(P.x - Q.y, P.y + Q.x);   // P + i Q
+    }
+    OFX_SYNC();
+    // RS' by the half-length forward transform; into Y: AT ((1 - fr) IT + dT fr RS')
     const R dfr = (R)(c.dT * c.fr), one_fr = (R)(1.0 - c.fr);
-    auto ld_yv = [=](int i) { return Y[i]; };
-    C* Yn = Y == c.lds0 ? c.lds1 : c.lds0;
-    X = fft_fused<R, -1>(lg, Yn, Y, c.tw, ld_yv, [=](C* o, int i, C v) {
-      const C a = at[i];
-      const R it = a.x * a.x + a.y * a.y;
-      const C rs = cscale(v, dfr);
-      o[i] = cmul(a, mkc<R>(one_fr * it + rs.x, rs.y));
-    });
+    (void)fft_fused<R, -1>(lg - 1, Zi, Zp, c.tw, qt, 2, [=](int m) { return Zp[m]; },
+                           [=](C*, int m, C v) {
+                             const C a0 = at[2 * m], a1 = at[2 * m + 1];
+                             Y[2 * m] = cscale(a0, one_fr * (a0.x * a0.x + a0.y * a0.y) + dfr * v.x);
+                             Y[2 * m + 1] =
+                                 cscale(a1, one_fr * (a1.x * a1.x + a1.y * a1.y) + dfr * v.y);
+                           });
+    X = Y;
   } else {
     // AT |AT|^2
-    X = fft_fused<R, -1>(lg, c.lds0, c.lds1, c.tw, ld_y,
+    X = fft_fused<R, -1>(lg, c.lds0, c.lds1, c.tw, qt, 1, ld_y,
                          [=](C* o, int i, C v) { o[i] = cscale(v, v.x * v.x + v.y * v.y); });
   }
-  auto ld_m = [=](int i) { return X[i]; };
-  (void)fft_fused<R, 1>(lg, X == c.lds0 ? c.lds1 : c.lds0, X, c.tw, ld_m, st_f);
+  const C* Xm = X;
+  (void)fft_fused<R, 1>(lg, X == c.lds0 ? c.lds1 : c.lds0, X, c.tw, qt, 1,
+                        [=](int i) { return Xm[i]; }, st_f);
 }
 
 // dst = y + sum_j cf[j] f_j (in stage order, zero coefficients skipped)
@@ -403,6 +462,7 @@ OFX_DEV void gnlse_instance(const GnlseArgs& a, int inst, cpx<R>* lds0, cpx<R>* 
   c.RW = a.RW ? (const C*)a.RW + (size_t)inst * n : nullptr;
   for (int i = OFX_TID; i < fft_table_size(a.lg); i += OFX_NT) ldstw[i] = ((const C*)a.tw)[i];
   c.tw = ldstw;
+  c.qt = ldstw + fft_qt_offset(a.lg);
 
   c.gamma = a.gamma[inst];
   c.fr = a.fr[inst];

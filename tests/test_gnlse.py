@@ -179,6 +179,21 @@ def test_emulated_kernel_fp64_matches_the_oracle(emu_lib, raman, shock):
     assert np.array_equal(Z, Zo) and np.allclose(W, Wo) and np.allclose(B, Bo)
 
 
+@pytest.mark.parametrize("n", [4, 8, 16, 32, 64])
+def test_emulated_kernel_fp64_small_grids_match_the_oracle(emu_lib, n):
+    """Every FFT shape the kernel builds: one pass (n = 4, and the half-length
+    Raman transforms at n = 4 and 8), a last radix-2 pass (odd log2 n, and the
+    half-length transforms of even ones), the twiddle tables' smallest forms;
+    a strongly nonlinear pulse (about 220 steps) with the Raman term."""
+    c = dudley_case(n=n, twidth=0.02 * n, power=20000.0, flength=0.02)
+    _, ATo, _, _, _, st = _oracle(c)
+    _, AT, _, _, _ = _device(c, emu_lib, "double")
+    _, _, stats, _ = _device_stats(c, emu_lib, "double")
+    assert st["accepted"] > 150
+    assert stats[0, 0] == st["accepted"] and stats[0, 1] == st["rejected"]
+    assert rel(AT, ATo) < 1e-11
+
+
 def gain_case(n=512):
     """A weak pulse under strong gain (loss -400 dB/m): the nonlinearity turns
     on as the pulse grows, and ode45 rejects steps (3 of 75 in the oracle)."""
