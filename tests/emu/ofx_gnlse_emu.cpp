@@ -18,7 +18,7 @@ int run(int n_inst, int n, int lg, int nsaves, const double* zs, const double* y
         const double* dT, double rtol, double atol, long long max_steps, double* out,
         long long* stats) {
   typedef ofx::cpx<R> C;
-  const std::vector<C> tw = ofx::fft_stage_twiddles<R>(lg);
+  const std::vector<C> tw = ofx::fft_twiddles<R>(lg);
   std::vector<double> hB, hlre;
   std::vector<R> hW;
   std::vector<C> hRW;
