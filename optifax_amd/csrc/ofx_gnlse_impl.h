@@ -120,6 +120,32 @@ OFX_DEV double eps_of(double t) {
 #else
 #define OFX_HD __host__ __device__ inline
 #endif
+// Global-memory views of the work vectors (device pointers held in plain
+// pointer types): with them the compiler emits global instead of flat
+// accesses.
+#ifdef OFX_EMU
+#define OFX_GLOBAL
+#else
+#define OFX_GLOBAL __attribute__((address_space(1)))
+#endif
+template <typename T>
+OFX_DEV const OFX_GLOBAL T* gl(const T* p) {
+  return (const OFX_GLOBAL T*)p;
+}
+template <typename T>
+OFX_DEV OFX_GLOBAL T* gl(T* p) {
+  return (OFX_GLOBAL T*)p;
+}
+template <typename R>
+OFX_DEV cpx<R> ldg(const OFX_GLOBAL cpx<R>* p, int i) {
+  return mkc<R>(p[i].x, p[i].y);
+}
+template <typename R>
+OFX_DEV void stg(OFX_GLOBAL cpx<R>* p, int i, cpx<R> v) {
+  p[i].x = v.x;
+  p[i].y = v.y;
+}
+
 OFX_HD int fft_r4_top(int lg) { return 1 << (2 * ((lg - 2) >> 1)); }
 OFX_HD bool fft_qt_shared(int lg) { return !(lg & 1) && lg >= 4; }
 OFX_HD int fft_qt_offset(int lg) {
@@ -194,8 +220,9 @@ OFX_DEV cpx<R>* fft_fused(int lg, cpx<R>* b0, cpx<R>* b1, const cpx<R>* tw, cons
   const int L = 1 << lg, q4 = L >> 2, h = L >> 1;
   const int tid = OFX_TID, nt = OFX_NT;
   const int n4 = lg >> 1, np = n4 + (lg & 1);
-  cpx<R>* const bufs[2] = {b0, b1};
-  C* const outbuf = bufs[(np - 1) & 1];
+  // (buffers chosen by selects, not by indexing an array of pointers: through
+  // an array the compiler loses that they are LDS and emits flat accesses)
+  C* const outbuf = ((np - 1) & 1) ? b1 : b0;
   const C one = mkc<R>(R(1), R(0));
   if (np == 1) {  // L = 2 or 4: one pass from ld to st
     if (tid == 0) {
@@ -230,8 +257,8 @@ OFX_DEV cpx<R>* fft_fused(int lg, cpx<R>* b0, cpx<R>* b1, const cpx<R>* tw, cons
   const int mid_end = (lg & 1) ? n4 : n4 - 1;
   int Ns = 4;
   for (int p = 1; p < mid_end; p++, Ns <<= 2) {
-    const C* in = bufs[(p + 1) & 1];
-    C* o = bufs[p & 1];
+    const C* in = (p & 1) ? b0 : b1;
+    C* o = (p & 1) ? b1 : b0;
     const C* tws = tw + (Ns - 4) / 3;
 #pragma unroll 2
     for (int j = tid; j < q4; j += nt) {
@@ -247,7 +274,7 @@ OFX_DEV cpx<R>* fft_fused(int lg, cpx<R>* b0, cpx<R>* b1, const cpx<R>* tw, cons
     OFX_SYNC();
   }
   // last pass: element i of the result to st
-  const C* in = bufs[(np - 2) & 1];
+  const C* in = (np & 1) ? b1 : b0;   // written by pass np - 2
   if (lg & 1) {  // radix 2, Ns = L / 2
 #pragma unroll 1
     for (int j = tid; j < h; j += nt) {
@@ -291,7 +318,7 @@ struct GnlseInst {
 
 // exp(+-i B_i z): the phase formed and reduced in FP64
 template <typename R>
-OFX_DEV cpx<R> phase_factor(const double* B, int i, double z) {
+OFX_DEV cpx<R> phase_factor(const OFX_GLOBAL double* B, int i, double z) {
   return expi(B[i] * z, R(0));
 }
 
@@ -316,24 +343,26 @@ OFX_DEV void gnlse_rhs(const GnlseInst<R>& c, double z, const cpx<R>* y, cpx<R>*
   const R inv_n = (R)(1.0 / (double)n);
   // exp(L z) = exp(Re L z) exp(i B z); Re L is one number per fibre
   const R gf = (R)exp(c.lre * z), gb = (R)(exp(-c.lre * z) / (double)n);
-  const double* B = c.B;
-  C* const at = c.at;
+  const OFX_GLOBAL double* B = gl(c.B);
+  OFX_GLOBAL C* const at = gl(c.at);
   const C* qt = c.qt;
+  const OFX_GLOBAL C* yg = gl(y);
+  OFX_GLOBAL C* fo = gl(f);
   // AT = fft(AW exp(L z))
-  auto ld_y = [=](int i) { return cscale(cmul(y[i], phase_factor<R>(B, i, z)), gf); };
+  auto ld_y = [=](int i) { return cscale(cmul(ldg(yg, i), phase_factor<R>(B, i, z)), gf); };
   // rhs = i gamma W (M / n) exp(-L z), M = n ifft(.)
   const R gW = (R)c.gamma;
-  const R* W = c.W;
+  const OFX_GLOBAL R* W = gl(c.W);
   auto st_f = [=](C*, int i, C v) {
     const C m = cscale(cmulc(v, phase_factor<R>(B, i, z)), gb);
     const R s = gW * W[i];
-    f[i] = mkc<R>(-s * m.y, s * m.x);
+    stg(fo, i, mkc<R>(-s * m.y, s * m.x));
   };
   C* X;
   if (c.RW) {
     // AT kept for the last product; IT = |AT|^2 as n reals in the low half of X
     X = fft_fused<R, -1>(lg, c.lds0, c.lds1, c.tw, qt, 1, ld_y, [=](C* o, int i, C v) {
-      at[i] = v;
+      stg(at, i, v);
       reinterpret_cast<R*>(o)[i] = v.x * v.x + v.y * v.y;
     });
     C* const Y = X == c.lds0 ? c.lds1 : c.lds0;
@@ -342,7 +371,7 @@ OFX_DEV void gnlse_rhs(const GnlseInst<R>& c, double z, const cpx<R>* y, cpx<R>*
     C* Zi = fft_fused<R, 1>(lg - 1, X + h, X, c.tw, qt, 2, [=](int m) { return Xz[m]; },
                             [](C* o, int m, C v) { o[m] = v; });
     C* const Zp = Zi == X ? X + h : X;
-    const C* RW = c.RW;
+    const OFX_GLOBAL C* RW = gl(c.RW);
     const int tid = OFX_TID, nt = OFX_NT;
     for (int k = tid; k < h; k += nt) {
       const C a = Zi[k], b = Zi[k == 0 ? 0 : h - k];
@@ -352,8 +381,8 @@ OFX_DEV void gnlse_rhs(const GnlseInst<R>& c, double z, const cpx<R>* y, cpx<R>*
       const C O = mkc<R>(R(0.5) * d.y, R(-0.5) * d.x);   // d / 2i
       const C w = tw_half(qt, n, 1, k);                    // exp(-2 pi i k / n)
       const C wO = cmulc(O, w);
-      const C x0 = cscale(cmul(cadd(E, wO), RW[k]), inv_n);
-      const C x1 = cscale(cmul(csub(E, wO), RW[k + h]), inv_n);
+      const C x0 = cscale(cmul(cadd(E, wO), ldg(RW, k)), inv_n);
+      const C x1 = cscale(cmul(csub(E, wO), ldg(RW, k + h)), inv_n);
       const C P = cadd(x0, x1), Q = cmul(csub(x0, x1), w);
       Zp[k] = mkc<R>(P.x - Q.y, P.y + Q.x);   // P + i Q
     }
@@ -362,7 +391,7 @@ OFX_DEV void gnlse_rhs(const GnlseInst<R>& c, double z, const cpx<R>* y, cpx<R>*
     const R dfr = (R)(c.dT * c.fr), one_fr = (R)(1.0 - c.fr);
     (void)fft_fused<R, -1>(lg - 1, Zi, Zp, c.tw, qt, 2, [=](int m) { return Zp[m]; },
                            [=](C*, int m, C v) {
-                             const C a0 = at[2 * m], a1 = at[2 * m + 1];
+                             const C a0 = ldg(at, 2 * m), a1 = ldg(at, 2 * m + 1);
                              Y[2 * m] = cscale(a0, one_fr * (a0.x * a0.x + a0.y * a0.y) + dfr * v.x);
                              Y[2 * m + 1] =
                                  cscale(a1, one_fr * (a1.x * a1.x + a1.y * a1.y) + dfr * v.y);
@@ -378,47 +407,68 @@ OFX_DEV void gnlse_rhs(const GnlseInst<R>& c, double z, const cpx<R>* y, cpx<R>*
                         [=](int i) { return Xm[i]; }, st_f);
 }
 
-// dst = y + sum_j cf[j] f_j (in stage order, zero coefficients skipped)
+// dst = y + sum_j cf[j] f_j over the first nj stages, in stage order.  Every
+// stage vector is read (those past nj weigh nothing, by a select: they may
+// hold anything) so the loads of one point issue together; a zero
+// coefficient adds a zero, which leaves a finite sum unchanged.
 template <typename R>
 OFX_DEV void stage_comb(int n, const cpx<R>* y, cpx<R>* const* f, const double* cf, int nj,
                         cpx<R>* dst) {
+  R c[7];
+  const OFX_GLOBAL cpx<R>* fg[7];
+#pragma unroll
+  for (int j = 0; j < 7; j++) {
+    c[j] = j < nj ? (R)cf[j] : R(0);
+    fg[j] = gl(f[j]);
+  }
+  const OFX_GLOBAL cpx<R>* yg = gl(y);
+  OFX_GLOBAL cpx<R>* dg = gl(dst);
 #pragma unroll 2
   for (int i = OFX_TID; i < n; i += OFX_NT) {
     R ax = R(0), ay = R(0);
 #pragma unroll
     for (int j = 0; j < 7; j++) {
-      if (j >= nj || cf[j] == 0.0) continue;
-      const R c = (R)cf[j];
-      ax = ax + c * f[j][i].x;
-      ay = ay + c * f[j][i].y;
+      const R vx = fg[j][i].x, vy = fg[j][i].y;
+      ax = j < nj ? ax + c[j] * vx : ax;
+      ay = j < nj ? ay + c[j] * vy : ay;
     }
-    dst[i] = mkc<R>(y[i].x + ax, y[i].y + ay);
+    dg[i].x = yg[i].x + ax;
+    dg[i].y = yg[i].y + ay;
   }
 }
 
 template <typename R>
 OFX_DEV double norm2(int n, const cpx<R>* v, double* red) {
+  const OFX_GLOBAL cpx<R>* vg = gl(v);
   double s = 0.0;
 #pragma unroll 4
   for (int i = OFX_TID; i < n; i += OFX_NT) {
-    const double x = (double)v[i].x, y = (double)v[i].y;
+    const double x = (double)vg[i].x, y = (double)vg[i].y;
     s += x * x + y * y;
   }
   return sqrt(wg_sum(s, red));
 }
 
 // || sum_j E_j f_j ||, the error weights applied per point in stage order
+// (zero weights by a select, as in stage_comb)
 template <typename R>
 OFX_DEV double err_norm(int n, cpx<R>* const* f, const double* E, double* red) {
+  double e[7];
+  const OFX_GLOBAL cpx<R>* fg[7];
+#pragma unroll
+  for (int j = 0; j < 7; j++) {
+    e[j] = E[j];
+    fg[j] = gl(f[j]);
+  }
   double s = 0.0;
 #pragma unroll 2
   for (int i = OFX_TID; i < n; i += OFX_NT) {
     double ax = 0.0, ay = 0.0;
 #pragma unroll
     for (int j = 0; j < 7; j++) {
-      if (E[j] == 0.0) continue;
-      ax = ax + E[j] * (double)f[j][i].x;
-      ay = ay + E[j] * (double)f[j][i].y;
+      const double vx = (double)fg[j][i].x, vy = (double)fg[j][i].y;
+      ax = e[j] != 0.0 ? ax + e[j] * vx : ax;
+      ay = e[j] != 0.0 ? ay + e[j] * vy : ay;
     }
     s += ax * ax + ay * ay;
   }
