@@ -120,32 +120,6 @@ OFX_DEV double eps_of(double t) {
 #else
 #define OFX_HD __host__ __device__ inline
 #endif
-// Global-memory views of the work vectors (device pointers held in plain
-// pointer types): with them the compiler emits global instead of flat
-// accesses.
-#ifdef OFX_EMU
-#define OFX_GLOBAL
-#else
-#define OFX_GLOBAL __attribute__((address_space(1)))
-#endif
-template <typename T>
-OFX_DEV const OFX_GLOBAL T* gl(const T* p) {
-  return (const OFX_GLOBAL T*)p;
-}
-template <typename T>
-OFX_DEV OFX_GLOBAL T* gl(T* p) {
-  return (OFX_GLOBAL T*)p;
-}
-template <typename R>
-OFX_DEV cpx<R> ldg(const OFX_GLOBAL cpx<R>* p, int i) {
-  return mkc<R>(p[i].x, p[i].y);
-}
-template <typename R>
-OFX_DEV void stg(OFX_GLOBAL cpx<R>* p, int i, cpx<R> v) {
-  p[i].x = v.x;
-  p[i].y = v.y;
-}
-
 OFX_HD int fft_r4_top(int lg) { return 1 << (2 * ((lg - 2) >> 1)); }
 OFX_HD bool fft_qt_shared(int lg) { return !(lg & 1) && lg >= 4; }
 OFX_HD int fft_qt_offset(int lg) {

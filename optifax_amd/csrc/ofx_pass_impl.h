@@ -81,6 +81,34 @@ OFX_DEV cpx<R> cmulc(cpx<R> a, cpx<R> b) {
 template <typename R>
 OFX_DEV cpx<R> cscale(cpx<R> a, R s) { return mkc<R>(a.x * s, a.y * s); }
 
+// Global-memory views of device buffers held in plain pointer types: with
+// them the compiler emits global instead of flat accesses (a flat access
+// also counts as an LDS access, so every LDS wait after it waits for the
+// memory round trip too).
+#ifdef OFX_EMU
+#define OFX_GLOBAL
+#else
+#define OFX_GLOBAL __attribute__((address_space(1)))
+#endif
+template <typename T>
+OFX_DEV const OFX_GLOBAL T* gl(const T* p) {
+  return (const OFX_GLOBAL T*)p;
+}
+template <typename T>
+OFX_DEV OFX_GLOBAL T* gl(T* p) {
+  return (OFX_GLOBAL T*)p;
+}
+template <typename R, typename I>
+OFX_DEV cpx<R> ldg(const OFX_GLOBAL cpx<R>* p, I i) {
+  return mkc<R>(p[i].x, p[i].y);
+}
+template <typename R, typename I>
+OFX_DEV void stg(OFX_GLOBAL cpx<R>* p, I i, cpx<R> v) {
+  p[i].x = v.x;
+  p[i].y = v.y;
+}
+
+
 // x - 2*pi*round(x/(2*pi)) with a two-term (Cody-Waite) split of 2*pi; the
 // fma makes n*TWO_PI_HI exact, so the reduction is accurate for |x| < 2^40.
 OFX_DEV double reduce_2pi(double x) {
@@ -292,7 +320,8 @@ OFX_DEV void flag_set(int* p) {
 #ifdef OFX_EMU
   __atomic_store_n(p, 1, __ATOMIC_RELAXED);
 #else
-  __hip_atomic_store(p, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store((__attribute__((address_space(1))) int*)p, 1, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
 #endif
 }
 
@@ -906,13 +935,13 @@ OFX_DEV int rk23_pass(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc
     OFX_SYNC();
   };
   double k = 0.0, stepprev = 4.0, stepmod = 4.0;
-  kstage(K1, nullptr, R(0), 1, LC - 1, d.bdiffw0 * k * d.h, (R)(stepmod * d.h * d.G[0]));
+  kstage(K1, nullptr, R(0), 1, LC - 1, d.bdiffw0 * k * d.h, (R)(stepmod * d.h * gl(d.G)[0]));
   for (int chunk = 0; chunk < nchunks; chunk++) {
     stepmod = stepprev;
     const double cend = (double)(chunk + 1) * (double)sel;
     while (k < cend) {
       if ((int)k >= Ns) return ST_BAD_ARG;
-      const double Gk = d.G[(int)k];
+      const double Gk = gl(d.G)[(int)k];
       const R gk = (R)(stepmod * d.h * Gk);
       // K2 from A1 = A0 + K1/2
       kstage(K2, K1, a1, 2, LC - 2, d.bdiffw0 * (k + A1c * stepmod) * d.h, gk);
@@ -999,13 +1028,13 @@ OFX_DEV int rk23_pass(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc
         stepmod = stepmod - 1.0;
         stepprev = stepmod;
         kstage(K1, nullptr, R(0), 1, LC - 1, d.bdiffw0 * (k + stepmod) * d.h,
-               (R)(stepmod * d.h * d.G[(int)k]));
+               (R)(stepmod * d.h * gl(d.G)[(int)k]));
         k = k + stepmod;
         continue;
       }
       // ---------------- accept ----------------
       acc++;
-      if (rank == 0 && d.stepmods && (int)k < Ns) d.stepmods[(int)k] = (float)stepmod;
+      if (rank == 0 && d.stepmods && (int)k < Ns) gl(d.stepmods)[(int)k] = (float)stepmod;
       // (K1's halo already holds the neighbours' K4 boundary columns, read
       // with the rows)
       // frequency phase: rows, dispersion (step just taken), snapshot, inverse rows
@@ -1020,7 +1049,7 @@ OFX_DEV int rk23_pass(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc
         const R am = al == R(1) ? R(1) : (al == R(0) ? R(0) : (R)pow(al, (R)s_used));
         v = cscale(cmul(v, ph), am);
         F[k2l * g.sF + k1i] = v;
-        if (d.apft && chunk < d.n_cols) ((C*)d.apft)[(size_t)chunk * n + kk] = v;
+        if (d.apft && chunk < d.n_cols) stg(gl((C*)d.apft), (size_t)chunk * n + kk, v);
       };
 #pragma unroll
       for (int q = 0; q < MP; q++) {
@@ -1029,7 +1058,7 @@ OFX_DEV int rk23_pass(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc
       }
       for (int idx = OFX_TID + MP * OFX_NT; idx < g.Rr * n1; idx += OFX_NT) {
         const int kk = (g.k20 + (idx >> g.lg1)) + n2 * (idx & (n1 - 1));
-        disperse(idx, d.beta[kk], (R)d.alpha[kk]);
+        disperse(idx, gl(d.beta)[kk], (R)gl(d.alpha)[kk]);
       }
       OFX_SYNC();
       {
@@ -1183,8 +1212,8 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
     areg[q] = R(0);
     if (idx < g.Rr * n1) {
       const int kk = (g.k20 + (idx >> g.lg1)) + n2 * (idx & (n1 - 1));
-      breg[q] = d.beta[kk];
-      areg[q] = (R)d.alpha[kk];
+      breg[q] = gl(d.beta)[kk];
+      areg[q] = (R)gl(d.alpha)[kk];
     }
   }
 
@@ -1199,7 +1228,7 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
     // j clears (and at the trip's end copies) entries j, j + NT, ..., so each
     // thread's own program order already orders its clear, writes and copy.
     if (a.trips > 0 && rank == 0 && d.stepmods)
-      for (int j = OFX_TID; j < nsteps; j += OFX_NT) d.stepmods[j] = 0.0f;
+      for (int j = OFX_TID; j < nsteps; j += OFX_NT) gl(d.stepmods)[j] = 0.0f;
     if (d.method == METHOD_RK23) {
       status = rk23_pass<R, MP, XM>(a, cd, d, g, rank, red, sy, breg, areg, trip_att, trip_acc);
       attempts += trip_att;
@@ -1208,7 +1237,7 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
     }
     // ---------------- adaptive Heun pass (NEE_CPU.m:39-94) ----------------
     while ((double)chunk <= nChunks) {
-      const double Gk = (double)d.G[(int)k - 1];
+      const double Gk = (double)gl(d.G)[(int)k - 1];
       const double bdwz0 = d.bdiffw0 * (k - 1.0) * d.h;
       const double bdwz1 = d.bdiffw0 * (k - 1.0 + stepmod) * d.h;
       const R gk = (R)(d.h * stepmod * Gk);
@@ -1304,7 +1333,7 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
       // ---------------- accept ----------------
       accepted++;
       trip_acc++;
-      if (rank == 0 && d.stepmods) d.stepmods[(int)k - 1] = (float)stepmod;
+      if (rank == 0 && d.stepmods) gl(d.stepmods)[(int)k - 1] = (float)stepmod;
       const double s_used = stepmod;
       double ns = stepmod;
       if (pcterr < d.min_err) ns += 1.0;
@@ -1324,9 +1353,9 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
         v = cscale(cmul(v, ph), am);
         F[k2l * g.sF + k1] = v;
         if (d.apft) {
-          C* ap = (C*)d.apft;
+          OFX_GLOBAL C* ap = gl((C*)d.apft);
           for (int cc2 = chunk; cc2 < cend; cc2++)
-            if (cc2 - 1 < d.n_cols) ap[(size_t)(cc2 - 1) * n + kk] = v;
+            if (cc2 - 1 < d.n_cols) stg(ap, (size_t)(cc2 - 1) * n + kk, v);
         }
       };
 #pragma unroll
@@ -1336,7 +1365,7 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
       }
       for (int idx = OFX_TID + MP * OFX_NT; idx < g.Rr * n1; idx += OFX_NT) {
         const int kk = (g.k20 + (idx >> g.lg1)) + n2 * (idx & (n1 - 1));
-        disperse(idx, d.beta[kk], (R)d.alpha[kk]);
+        disperse(idx, gl(d.beta)[kk], (R)gl(d.alpha)[kk]);
       }
       OFX_SYNC();
       {
@@ -1358,7 +1387,7 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
     // count, the trip log and the history ring to the round trips.
     if (a.trips > 0 && rank == 0 && d.sm_hist && d.stepmods) {
       float* row = d.sm_hist + (size_t)((trip_base + trip) % d.hist_cap) * (size_t)nsteps;
-      for (int j = OFX_TID; j < nsteps; j += OFX_NT) row[j] = d.stepmods[j];
+      for (int j = OFX_TID; j < nsteps; j += OFX_NT) gl(row)[j] = gl(d.stepmods)[j];
     }
     if (a.trips > 0 && rank == 0 && d.trip_log && trip_base + trip < d.trip_cap) {
       double* tl = d.trip_log + 4 * (size_t)(trip_base + trip);
