@@ -370,16 +370,27 @@ __global__ void __launch_bounds__(256) rel_l2_loss(const float* __restrict__ y,
   }
 }
 
-// dst [cols][rows] = src [rows][cols]^T (BF16), 64 x 64 tiles through LDS
+// dst [cols][rows] = src [rows][cols]^T (BF16), 64 x 64 tiles through LDS;
+// rows and cols multiples of 64.  Thread t moves 16 consecutive values of
+// one row in and 16 of one column out, as 16-byte pieces.
 __global__ void __launch_bounds__(256) transpose_bf16(const bf16* __restrict__ src, int rows,
                                                       int cols, bf16* __restrict__ dst) {
-  __shared__ bf16 tile[64][66];
+  __shared__ bf16 tile[64][72];   // 144-byte rows: 16-byte aligned, staggered banks
   const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
-  for (int i = threadIdx.x; i < 4096; i += 256)
-    tile[i >> 6][i & 63] = src[(size_t)(r0 + (i >> 6)) * cols + c0 + (i & 63)];
+  const int a = threadIdx.x >> 2, b = (threadIdx.x & 3) * 16;
+  const bf16* sp = src + (size_t)(r0 + a) * cols + c0 + b;
+  *reinterpret_cast<bf16x8*>(&tile[a][b]) = *reinterpret_cast<const bf16x8*>(sp);
+  *reinterpret_cast<bf16x8*>(&tile[a][b + 8]) = *reinterpret_cast<const bf16x8*>(sp + 8);
   __syncthreads();
-  for (int i = threadIdx.x; i < 4096; i += 256)
-    dst[(size_t)(c0 + (i >> 6)) * rows + r0 + (i & 63)] = tile[i & 63][i >> 6];
+  bf16x8 o0, o1;
+#pragma unroll
+  for (int e = 0; e < 8; e++) {
+    o0[e] = tile[b + e][a];
+    o1[e] = tile[b + 8 + e][a];
+  }
+  bf16* dp = dst + (size_t)(c0 + a) * rows + r0 + b;
+  *reinterpret_cast<bf16x8*>(dp) = o0;
+  *reinterpret_cast<bf16x8*>(dp + 8) = o1;
 }
 
 // Bias gradients of all four layers, db[n] = sum_b dZT[n][b] (one wave per n,
@@ -457,32 +468,60 @@ __global__ void __launch_bounds__(256) weight_copies(const float* __restrict__ W
 
 // input rows (FP32, n_in wide) -> BF16 [batch][ld_in] and [ld_in][batch], zero
 // padded; rows come from a device dataset through idx (or in order when idx
-// is null).  64 x 64 tiles, the transposed copy through LDS.
+// is null).  A workgroup takes 64 rows x LI_COLS columns: thread t reads 16
+// consecutive values of row t / 4 in each 64-column half (8-byte loads when
+// n_in is even, so every row starts on an 8-byte boundary), writes them as
+// 16-byte BF16 pieces and, through LDS, 16-byte pieces of the transpose.
+constexpr int LI_COLS = 128;
 __global__ void __launch_bounds__(256) load_inputs(const float* __restrict__ X,
                                                    const int* __restrict__ idx, int batch, int n_in,
                                                    int ld_in, bf16* __restrict__ h0,
                                                    bf16* __restrict__ h0T) {
-  __shared__ bf16 tile[64][66];
-  const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
-  // thread t: column c0 + t % 64 of rows r0 + t / 64 + 4k (16 independent loads in flight)
-  const int cc = threadIdx.x & 63, c = c0 + cc, rq = threadIdx.x >> 6;
-  float v[16];
+  __shared__ bf16 tile[64][LI_COLS + 8];
+  const int r0 = blockIdx.y * 64, cb = blockIdx.x * LI_COLS;
+  const int rr = threadIdx.x >> 2, q = (threadIdx.x & 3) * 16;
+  const float* xr = X + (idx ? (size_t)idx[r0 + rr] : (size_t)(r0 + rr)) * n_in;
+  const bool pairs = (n_in & 1) == 0;
 #pragma unroll
-  for (int k = 0; k < 16; k++) {
-    const int rr = rq + 4 * k;
-    const size_t src = idx ? (size_t)idx[r0 + rr] : (size_t)(r0 + rr);
-    v[k] = c < n_in ? X[src * n_in + c] : 0.f;
-  }
+  for (int hh = 0; hh < LI_COLS; hh += 64) {
+    const int cl = hh + q, c = cb + cl;
+    if (c >= ld_in) break;
+    float v[16];
 #pragma unroll
-  for (int k = 0; k < 16; k++) {
-    const int rr = rq + 4 * k;
-    const bf16 b = (bf16)v[k];
-    h0[(size_t)(r0 + rr) * ld_in + c] = b;
-    tile[rr][cc] = b;
+    for (int e = 0; e < 16; e += 2) {
+      if (pairs && c + e + 1 < n_in) {
+        const float2 p = *reinterpret_cast<const float2*>(xr + c + e);
+        v[e] = p.x;
+        v[e + 1] = p.y;
+      } else {
+        v[e] = c + e < n_in ? xr[c + e] : 0.f;
+        v[e + 1] = c + e + 1 < n_in ? xr[c + e + 1] : 0.f;
+      }
+    }
+    bf16x8 o0, o1;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      o0[e] = (bf16)v[e];
+      o1[e] = (bf16)v[8 + e];
+    }
+    bf16* hp = h0 + (size_t)(r0 + rr) * ld_in + c;
+    *reinterpret_cast<bf16x8*>(hp) = o0;
+    *reinterpret_cast<bf16x8*>(hp + 8) = o1;
+    *reinterpret_cast<bf16x8*>(&tile[rr][cl]) = o0;
+    *reinterpret_cast<bf16x8*>(&tile[rr][cl + 8]) = o1;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < 4096; i += 256)
-    h0T[(size_t)(c0 + (i >> 6)) * batch + r0 + (i & 63)] = tile[i & 63][i >> 6];
+  // column cl of the block, rows rh .. rh + 31: four 16-byte pieces of h0T
+  const int cl = threadIdx.x >> 1, rh = (threadIdx.x & 1) * 32;
+  if (cb + cl >= ld_in) return;
+  bf16* tp = h0T + (size_t)(cb + cl) * batch + r0 + rh;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    bf16x8 o;
+#pragma unroll
+    for (int e = 0; e < 8; e++) o[e] = tile[rh + 8 * k + e][cl];
+    *reinterpret_cast<bf16x8*>(tp + 8 * k) = o;
+  }
 }
 
 }  // namespace ofxm
@@ -709,7 +748,8 @@ static AdamHP next_adam(ofx_mlp* m, float lr) {
 static int stage_batch(ofx_mlp* m, int B, const float* x, const float* t) {
   MCHK(m, hipMemcpyAsync(m->xin, x, sizeof(float) * (size_t)B * m->n_in, hipMemcpyHostToDevice,
                          m->stream));
-  hipLaunchKernelGGL(load_inputs, dim3(m->ld_in / 64, B / 64), dim3(256), 0, m->stream, m->xin,
+  hipLaunchKernelGGL(load_inputs, dim3((m->ld_in + LI_COLS - 1) / LI_COLS, B / 64), dim3(256), 0,
+                     m->stream, m->xin,
                      (const int*)nullptr, B,
                      m->n_in, m->ld_in, m->h[0], m->hT[0]);
   MLAUNCH(m);
@@ -969,7 +1009,8 @@ int ofx_mlp_train(ofx_mlp* m, int batch, int steps, const int* idx, float lr, fl
   MCHK(m, hipEventRecord(m->e0, m->stream));
   for (int s = 0; s < steps; s++) {
     const int* ix = m->idx + (size_t)s * batch;
-    hipLaunchKernelGGL(load_inputs, dim3(m->ld_in / 64, batch / 64), dim3(256), 0, m->stream,
+    hipLaunchKernelGGL(load_inputs, dim3((m->ld_in + LI_COLS - 1) / LI_COLS, batch / 64),
+                       dim3(256), 0, m->stream,
                        m->dsX, ix, batch, m->n_in, m->ld_in, m->h[0], m->hT[0]);
     MLAUNCH(m);
     st = forward(m, batch);
