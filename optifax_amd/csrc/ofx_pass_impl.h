@@ -192,12 +192,28 @@ OFX_DEV cpx<R> k_term(cpx<R> nl, cpx<R> up, cpx<R> dn, R g, R cc) {
 // of sequence q at buf[q*ss + e]: radix-4 passes (one radix-2 pass for odd
 // lg).  Ping-pongs between `in` and `out`; returns the buffer that holds the
 // result.  SIGN = -1 forward (unnormalised), +1 inverse (unnormalised).
-// tw[m] = exp(-2 pi i m / M), tws = M / L.  (Measured on the ring workload:
-// radix-8 passes with register DFTs were 6 % slower per step.)
+// tw: the passes' twiddle tables (pass_twiddle_index).  (Measured on the
+// ring workload: radix-8 passes with register DFTs were 6 % slower per step.)
 // ------------------------------------------------------------------------
+// Twiddle tables of the radix-4 passes of a length-2^lg transform: the pass
+// with stride Ns > 1 reads w_c[k] = exp(-2 pi i c k / (4 Ns)), c = 1, 2, 3,
+// k < Ns, stored as three arrays of Ns entries from offset Ns - Ns0 (Ns0 = 2
+// for odd lg, 4 for even lg), so that consecutive butterflies read
+// consecutive entries (L - Ns0 entries in all).  pass_twiddle_index gives,
+// for entry j, the index into exp(-2 pi i m / L) it holds.
+OFX_DEV int pass_ns0(int lg) { return (lg & 1) ? 2 : 4; }
+OFX_DEV int pass_twiddle_index(int lg, int j) {
+  int ns = pass_ns0(lg), off = 0;
+  while (j >= off + 3 * ns) {
+    off += 3 * ns;
+    ns <<= 2;
+  }
+  const int c = (j - off) / ns + 1, k = (j - off) - (c - 1) * ns;
+  return c * k * ((1 << lg) / (4 * ns));
+}
+
 template <typename R, int SIGN>
-OFX_DEV cpx<R>* lds_fft(cpx<R>* in, cpx<R>* out, int lg, int nseq, int ss, const cpx<R>* tw,
-                        int tws) {
+OFX_DEV cpx<R>* lds_fft(cpx<R>* in, cpx<R>* out, int lg, int nseq, int ss, const cpx<R>* tw) {
   typedef cpx<R> C;
   const int L = 1 << lg;
   const int tid = OFX_TID, nt = OFX_NT;
@@ -222,8 +238,9 @@ OFX_DEV cpx<R>* lds_fft(cpx<R>* in, cpx<R>* out, int lg, int nseq, int ss, const
   if (lg >= 2) {
     const int q4 = L >> 2;
     const int nb = nseq << (lg - 2);
+    const int ns0 = pass_ns0(lg);
     for (; Ns < L; Ns <<= 2) {
-      const int span = (L / (4 * Ns)) * tws;
+      const C* tp = tw + (Ns - ns0);   // this pass's w_1, w_2, w_3 (Ns > 1)
       for (int t = tid; t < nb; t += nt) {
         const int q = t >> (lg - 2), j = t & (q4 - 1);
         const int k = j & (Ns - 1);
@@ -231,8 +248,7 @@ OFX_DEV cpx<R>* lds_fft(cpx<R>* in, cpx<R>* out, int lg, int nseq, int ss, const
         C* ob = out + q * ss;
         C v0 = ib[j], v1 = ib[j + q4], v2 = ib[j + 2 * q4], v3 = ib[j + 3 * q4];
         if (Ns > 1) {
-          const int m = k * span;
-          C w1 = tw[m], w2 = tw[2 * m], w3 = tw[3 * m];
+          C w1 = tp[k], w2 = tp[Ns + k], w3 = tp[2 * Ns + k];
           if (SIGN > 0) {
             v1 = cmulc(v1, w1);
             v2 = cmulc(v2, w2);
@@ -689,8 +705,8 @@ struct Geo {
   int Cn, lgC, rank, W, lgW, Rr, lgR, HW, LC, sT, sF, c0, k20;
   int plain;  // exchange in plain form after a counter barrier (else tagged, polled)
   cpx<R>*B0, *B1, *B2, *B3, *B4;  // B3, B4: RK23 only
-  const cpx<R>* tw1;  // LDS: exp(-2 pi i j / n1), j < n1
-  const cpx<R>* tw2;  // LDS: exp(-2 pi i j / n2), j < n2
+  const cpx<R>* tw1;  // LDS: the length-n1 passes' twiddles (pass_twiddle_index), n1 entries
+  const cpx<R>* tw2;  // LDS: the same for length n2
   const cpx<double>* twq;  // LDS, FP64: exp(-2 pi i q / n2), q < n2
   const cpx<double>* twr;  // LDS, FP64: exp(-2 pi i r / n), r < n1
   const cpx<R>* wl;   // LDS: exp(i w0 t_l), l < n1
@@ -737,7 +753,7 @@ OFX_DEV cpx<R>* fwd_cols_to_t1(const Geo<R>& g, cpx<R>* src, cpx<R>* other, u64*
                                unsigned tag) {
   typedef cpx<R> C;
   typedef Tagged<C> TT;
-  C* res = lds_fft<R, -1>(src + g.HW * g.sT, other + g.HW * g.sT, g.lg2, g.W, g.sT, g.tw2, 1);
+  C* res = lds_fft<R, -1>(src + g.HW * g.sT, other + g.HW * g.sT, g.lg2, g.W, g.sT, g.tw2);
   const int tot = g.W * g.n2;
   for (int idx = OFX_TID; idx < tot; idx += OFX_NT) {
     const int cl = idx & (g.W - 1), k2 = idx >> g.lgW;
@@ -764,7 +780,7 @@ OFX_DEV cpx<R>* t1_to_rows_fwd(const Geo<R>& g, const u64* t1, unsigned tag, cpx
       },
       [&](int idx, C v) { dst[(idx >> g.lg1) * g.sF + (idx & (g.n1 - 1))] = v; }, sp, g.plain);
   ok = wg_all(ok, g.plain);
-  return lds_fft<R, -1>(dst, other, g.lg1, g.Rr, g.sF, g.tw1, 1);
+  return lds_fft<R, -1>(dst, other, g.lg1, g.Rr, g.sF, g.tw1);
 }
 
 // After the exchange of an attempt: read the cluster's error records, this
@@ -851,7 +867,7 @@ OFX_DEV cpx<R>* rows_inv_to_t2(const Geo<R>& g, cpx<R>* src, cpx<R>* other, u64*
                                unsigned tag) {
   typedef cpx<R> C;
   typedef Tagged<C> TT;
-  C* res = lds_fft<R, +1>(src, other, g.lg1, g.Rr, g.sF, g.tw1, 1);
+  C* res = lds_fft<R, +1>(src, other, g.lg1, g.Rr, g.sF, g.tw1);
   const int tot = g.Rr * g.n1;
   for (int idx = OFX_TID; idx < tot; idx += OFX_NT) {
     const int k2l = idx & (g.Rr - 1), c = idx >> g.lgR;
@@ -884,7 +900,7 @@ OFX_DEV bool t2_to_cols_inv(const Geo<R>& g, const u64* t2, unsigned tag, Spin& 
       },
       [&](int idx, C v) { dst[(idx >> g.lg2) * g.sT + (idx & (g.n2 - 1))] = v; }, sp, g.plain);
   ok = wg_all(ok, g.plain);
-  C* res = lds_fft<R, +1>(dst, other, g.lg2, g.LC, g.sT, g.tw2, 1);
+  C* res = lds_fft<R, +1>(dst, other, g.lg2, g.LC, g.sT, g.tw2);
   const R inv = R(1) / R(g.n);
   for (int idx = OFX_TID; idx < tot; idx += OFX_NT) {
     const int lc = idx >> g.lg2, rp = idx & (g.n2 - 1);
@@ -1055,7 +1071,7 @@ OFX_DEV int rk23_pass(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc
       // with the rows)
       // frequency phase: rows, dispersion (step just taken), snapshot, inverse rows
       const double s_used = stepmod;
-      C* F = lds_fft<R, -1>(g.B1, g.B2, g.lg1, g.Rr, g.sF, g.tw1, 1);
+      C* F = lds_fft<R, -1>(g.B1, g.B2, g.lg1, g.Rr, g.sF, g.tw1);
       C* Fo = (F == g.B1) ? g.B2 : g.B1;
       auto disperse = [&](int idx, double beta, R al) {
         const int k2l = idx >> g.lg1, k1i = idx & (n1 - 1);
@@ -1142,7 +1158,8 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
     g.B3 = lds + 3 * bs;  // RK23 only (the launch sizes LDS for 5 buffers then)
     g.B4 = lds + 4 * bs;
     // tables after the field buffers (the launch sizes LDS for them): the
-    // transforms' twiddles of both lengths and the two factors of
+    // transforms' pass twiddles of both lengths (n1 and n2 entries, the last
+    // Ns0 unused) and the two factors of
     // exp(i w0 t_j) (j = h n1 + l: wt[j] = wt[h n1] wt[l], since t_j is
     // linear in j within each half of the fftshifted axis and n/2 is a
     // multiple of n1), 2 n1 + 2 n2 complex; then, 16-byte aligned and in
@@ -1156,12 +1173,12 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
     const int s1 = a.nmax >> d.lg1, s2 = a.nmax >> d.lg2;
     const int n1 = d.n1, n2 = d.n2;
     for (int j = OFX_TID; j < 2 * n1 + 2 * n2; j += OFX_NT) {
-      C v;
-      if (j < n1)
-        v = twg[j * s1];
-      else if (j < n1 + n2)
-        v = twg[(j - n1) * s2];
-      else if (j < 2 * n1 + n2)
+      C v = mkc<R>(R(0), R(0));
+      if (j < n1) {
+        if (j < n1 - pass_ns0(d.lg1)) v = twg[pass_twiddle_index(d.lg1, j) * s1];
+      } else if (j < n1 + n2) {
+        if (j - n1 < n2 - pass_ns0(d.lg2)) v = twg[pass_twiddle_index(d.lg2, j - n1) * s2];
+      } else if (j < 2 * n1 + n2)
         v = wtg[j - n1 - n2];
       else
         v = wtg[(j - 2 * n1 - n2) * n1];
@@ -1358,7 +1375,7 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
       int cend = chunk;  // chunks [chunk, cend) end with this step
       while ((double)cend <= nChunks && !(knew < cend * sel && knew <= nsteps)) cend++;
       // frequency phase: rows, dispersion, snapshots, inverse rows
-      C* F = lds_fft<R, -1>(rows, cols, g.lg1, g.Rr, g.sF, g.tw1, 1);
+      C* F = lds_fft<R, -1>(rows, cols, g.lg1, g.Rr, g.sF, g.tw1);
       C* Fo = (F == rows) ? cols : rows;
       auto disperse = [&](int idx, double beta, R al) {
         const int k2l = idx >> g.lg1, k1 = idx & (n1 - 1);
