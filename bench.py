@@ -517,6 +517,67 @@ def main_fibre(args):
     print(json.dumps(out))
 
 
+def main_qpm(args):
+    """QPM gain map (qpmgain.m): an 8 mm MgO:PPLN at 30 um and a 300 fs, 1 nJ
+    sech^2 pump at 1040 nm on a 2^13-point window; signals 1100-1700 nm (at
+    most 2000) x pump wavelengths within +-3 FWHM (at most 100), the crystal
+    sampled as qpmgain.m samples it.  value = mismatch sums per second of the
+    device z-sum (runs of equal poling summed in closed form); parity and the
+    CPU baseline from the oracle's literal sample-by-sample sum on 200 of the
+    outputs (numpy, one core)."""
+    from optifax_amd import physics as ph, qpm
+    sw = ph.SimWindow(1040e-9, 2 ** 13, 10e-12)
+    area = np.pi * (30e-6) ** 2
+    E = ph.sech_pump(sw, 300e-15, 1e-9, area)
+    pulse = qpm.pulse_descriptors(sw, E, area)
+    xt = ph.NonlinearCrystal(30e-6, 8e-3)
+    a = qpm.qpm_arrays(xt, sw, pulse, (1100.0, 1700.0), step=1e-7)
+    dk = a["dk"].reshape(-1)
+    for _ in range(max(args.warmup, 1)):
+        qpm.zsum(dk, a["z"], a["w"])
+    ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        got = qpm.zsum(dk, a["z"], a["w"])
+        ms.append(qpm.zsum.last_ms)
+    wall = time.perf_counter() - t0
+    dev_ms = float(np.mean(ms))
+    out = {
+        "metric": METRIC,
+        "value": dk.size / (dev_ms * 1e-3),
+        "unit": "QPM mismatch sums/s (device time)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": max(args.warmup, 1),
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (sech^2 pump spectrum through an 8 mm PPLN grating)",
+        "config": {
+            "workload": "QPM gain map (qpmgain.m): 8 mm MgO:PPLN 30 um, 300 fs 1 nJ sech^2 pump "
+                        "at 1040 nm, 2^13 points over 10 ps, signals 1100-1700 nm",
+            "map": list(a["dk"].shape), "z_samples": int(a["z"].size),
+            "runs": int(qpm.zsum.last_runs), "device_ms": dev_ms,
+        },
+    }
+    if not args.no_cpu_baseline:
+        import oracle.qpm as oq
+        pick = np.random.default_rng(0).choice(dk.size, 200, replace=False)
+        t0 = time.perf_counter()
+        want = oq.literal_sum(dk[pick], a["z"], a["w"])
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": pick.size / dt, "unit": "QPM mismatch sums/s", "cores": 1, "kind": "port",
+            "sample": f"the oracle's literal sum (one exponential per z sample, numpy) for 200 "
+                      f"of the {dk.size} outputs: {dt:.2f} s"}
+        out["parity"] = {
+            "against": "FP64 oracle, literal sum over the z samples, 200 outputs",
+            "rel_l2": float(np.linalg.norm(got[pick] - want) / np.linalg.norm(want))}
+    print(json.dumps(out))
+
+
 def main_pass(args):
     """A single SHG/OPG crystal pass: 2^13 points, 10 ps window, 1040 nm;
     sech^2 200 fs pump in 5 um^2 with seeded one-photon noise; 30 um PPLN,
@@ -824,7 +885,7 @@ def main():
                     help="round trips of each run of the esd workload")
     ap.add_argument("--workload", default="ring",
                     choices=["ring", "batch", "warmstart", "surrogate", "steady", "esd", "pass",
-                             "mex", "config5", "fibre"],
+                             "mex", "config5", "fibre", "qpm"],
                     help="ring: one synchronously pumped ring OPO (the headline line); batch: 256 uneven "
                          "instances in one persistent launch")
     ap.add_argument("--fibre-batch", type=int, default=256)
@@ -871,6 +932,8 @@ def main():
         return main_config5(args)
     if args.workload == "fibre":
         return main_fibre(args)
+    if args.workload == "qpm":
+        return main_qpm(args)
     from optifax_amd.solver import Context, OPOBatch
     cfg, b = workload()
     si, ops = b["solver"], b["ops"]

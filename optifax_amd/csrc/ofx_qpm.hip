@@ -58,6 +58,9 @@ zsum_kernel(int n_out, const double* __restrict__ dk, int n_run, const double* _
   const double th = reduce_2pi(k * dz);
   const double s_half = sin(0.5 * th);
   double ar = 0.0, ai = 0.0;
+  // the series factor depends on the run length only: kept while consecutive
+  // runs share it (a uniform grating has one or two lengths)
+  double m_prev = -1.0, d_prev = 0.0;
   for (int base = 0; base < n_run; base += ZS_CHUNK) {
     const int cnt = min(ZS_CHUNK, n_run - base);
     __syncthreads();
@@ -68,14 +71,18 @@ zsum_kernel(int n_out, const double* __restrict__ dk, int n_run, const double* _
     }
     __syncthreads();
     for (int i = 0; i < cnt; i++) {
-      const double m = s_m[i];
+      const double m = s_m[i];   // the same for every thread
       double d;
       if (m == 1.0) {
         d = 1.0;
       } else if (s_half == 0.0) {
         d = m;  // every sample of the run has the same phase
+      } else if (m == m_prev) {
+        d = d_prev;
       } else {
         d = sin(reduce_2pi(0.5 * m * th)) / s_half;
+        m_prev = m;
+        d_prev = d;
       }
       const double ph = reduce_2pi(fma(k, s_z0[i], 0.5 * (m - 1.0) * th));
       double s, c;
