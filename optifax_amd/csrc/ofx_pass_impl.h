@@ -860,7 +860,16 @@ OFX_DEV bool gather_err_and_rows(const ClusterDesc& cd, int par, const Geo<R>& g
 }
 
 // Inverse length-n1 transforms of the own rows, conjugate twiddle, write the
-// tagged buffer t2 (column-major c*n2 + k2).
+// buffer t2.  Tagged exchanges (clusters of 64 and more, a row or two per
+// member) use row-major k2*n1 + c, as t1: a member writes its rows as
+// contiguous runs and the column owners read runs of LC samples per row;
+// plain exchanges (small clusters, many rows per member) use column-major
+// c*n2 + k2, where both sides move runs of Rr and n2 samples.
+template <typename R>
+OFX_DEV size_t t2_index(const Geo<R>& g, int c, int k2) {
+  return g.plain ? (size_t)c * g.n2 + k2 : (size_t)k2 * g.n1 + c;
+}
+
 // Returns the buffer (src or other) holding the transforms (see fwd_cols_to_t1).
 template <typename R>
 OFX_DEV cpx<R>* rows_inv_to_t2(const Geo<R>& g, cpx<R>* src, cpx<R>* other, u64* t2,
@@ -874,7 +883,7 @@ OFX_DEV cpx<R>* rows_inv_to_t2(const Geo<R>& g, cpx<R>* src, cpx<R>* other, u64*
     const int k2 = g.k20 + k2l;
     C v = res[k2l * g.sF + c];
     v = cmulc(v, fourstep_tw(g, c * k2));
-    const TagPtr t = tp_plane<C>(g.plain, t2, g.n, (size_t)(c * g.n2 + k2));
+    const TagPtr t = tp_plane<C>(g.plain, t2, g.n, t2_index(g, c, k2));
     TT::put(const_cast<u64*>(t.p), t.ws, t.plain, v, tag);
   }
   return res;
@@ -890,15 +899,33 @@ OFX_DEV bool t2_to_cols_inv(const Geo<R>& g, const u64* t2, unsigned tag, Spin& 
                             cpx<R>* other) {
   typedef cpx<R> C;
   const int tot = g.LC * g.n2;
+  // entry idx: consecutive threads read consecutive t2 entries, i.e.
+  // consecutive columns of one row (row-major t2: row k2 = idx / LC, local
+  // column lc = idx % LC) or consecutive rows of one column (column-major)
+  auto split = [&](int idx, int& lc, int& k2) {
+    if (g.plain) {
+      lc = idx >> g.lg2;
+      k2 = idx & (g.n2 - 1);
+    } else {
+      k2 = idx / g.LC;
+      lc = idx - k2 * g.LC;
+    }
+  };
   bool ok = read_tagged<C>(
       tot, tag,
       [&](int idx) {
-        const int lc = idx >> g.lg2, k2 = idx & (g.n2 - 1);
+        int lc, k2;
+        split(idx, lc, k2);
         int gc = g.c0 - g.HW + lc;
         gc = gc < 0 ? gc + g.n1 : (gc >= g.n1 ? gc - g.n1 : gc);
-        return tp_plane<C>(g.plain, t2, g.n, (size_t)(gc * g.n2 + k2));
+        return tp_plane<C>(g.plain, t2, g.n, t2_index(g, gc, k2));
       },
-      [&](int idx, C v) { dst[(idx >> g.lg2) * g.sT + (idx & (g.n2 - 1))] = v; }, sp, g.plain);
+      [&](int idx, C v) {
+        int lc, k2;
+        split(idx, lc, k2);
+        dst[lc * g.sT + k2] = v;
+      },
+      sp, g.plain);
   ok = wg_all(ok, g.plain);
   C* res = lds_fft<R, +1>(dst, other, g.lg2, g.LC, g.sT, g.tw2);
   const R inv = R(1) / R(g.n);
