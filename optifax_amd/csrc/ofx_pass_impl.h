@@ -748,6 +748,16 @@ OFX_DEV double dw_of(int k, int n, double dt) {
 // Returns the buffer (src or other) that holds the transforms: data read
 // from the exchange may go only to the other one, since a thread can
 // complete its reads while another is still reading these.
+// Entry of sample (column c, row k2) in the transpose buffer t1.  Plain
+// exchanges: row-major k2*n1 + c.  Tagged exchanges: member-blocked,
+// ((c / W) n2 + k2) W + c % W, so a member's columns are one contiguous run
+// for the writer and a row's reader takes runs of W samples.
+template <typename R>
+OFX_DEV size_t t1_index(const Geo<R>& g, int c, int k2) {
+  if (g.plain) return (size_t)k2 * g.n1 + c;
+  return (((size_t)(c >> g.lgW) * g.n2 + k2) << g.lgW) + (c & (g.W - 1));
+}
+
 template <typename R>
 OFX_DEV cpx<R>* fwd_cols_to_t1(const Geo<R>& g, cpx<R>* src, cpx<R>* other, u64* t1,
                                unsigned tag) {
@@ -760,7 +770,7 @@ OFX_DEV cpx<R>* fwd_cols_to_t1(const Geo<R>& g, cpx<R>* src, cpx<R>* other, u64*
     const int c = g.c0 + cl;
     C v = res[cl * g.sT + k2];
     v = cmul(v, fourstep_tw(g, c * k2));
-    const TagPtr t = tp_plane<C>(g.plain, t1, g.n, (size_t)(k2 * g.n1 + c));
+    const TagPtr t = tp_plane<C>(g.plain, t1, g.n, t1_index(g, c, k2));
     TT::put(const_cast<u64*>(t.p), t.ws, t.plain, v, tag);
   }
   return res - g.HW * g.sT;
@@ -776,7 +786,7 @@ OFX_DEV cpx<R>* t1_to_rows_fwd(const Geo<R>& g, const u64* t1, unsigned tag, cpx
   ok = read_tagged<C>(
       g.Rr * g.n1, tag,
       [&](int idx) {
-        return tp_plane<C>(g.plain, t1, g.n, (size_t)((g.k20 + (idx >> g.lg1)) * g.n1 + (idx & (g.n1 - 1))));
+        return tp_plane<C>(g.plain, t1, g.n, t1_index(g, idx & (g.n1 - 1), g.k20 + (idx >> g.lg1)));
       },
       [&](int idx, C v) { dst[(idx >> g.lg1) * g.sF + (idx & (g.n1 - 1))] = v; }, sp, g.plain);
   ok = wg_all(ok, g.plain);
@@ -822,7 +832,7 @@ OFX_DEV bool gather_err_and_rows(const ClusterDesc& cd, int par, const Geo<R>& g
         if (idx < ns) return TagPtr{slot_at<R>(cd, par, idx / nrec, idx % nrec, 0), 1, 0};
         idx -= ns;
         if (idx < nr)
-          return tp_plane<C>(g.plain, t1, g.n, (size_t)((g.k20 + (idx >> g.lg1)) * g.n1 + (idx & (g.n1 - 1))));
+          return tp_plane<C>(g.plain, t1, g.n, t1_index(g, idx & (g.n1 - 1), g.k20 + (idx >> g.lg1)));
         return kx_src(idx - nr);
       },
       [&](int idx, C x) {
