@@ -311,7 +311,7 @@ OFX_DEV cpx<R>* lds_fft(cpx<R>* in, cpx<R>* out, int lg, int nseq, int ss, const
 // readers reduce the C * waves records, one per thread.  op 0 = max (one
 // value), op 1 = sum (three values, combined in a fixed order, so all members
 // obtain bit-identical results: the step-control decisions depend on it).
-// Slot layout: [parity][rank][wave < 8][4 values], each value a tagged
+// Slot layout: [parity][wave < 8][4 values][rank], each value a tagged
 // element the size of one complex sample.
 // ------------------------------------------------------------------------
 #define OFX_SLOT_WAVES 8
@@ -571,8 +571,9 @@ OFX_DEV bool wg_all(bool ok, int complete) {
 // first word of slot value i of (parity, rank, wave)
 template <typename R>
 OFX_DEV u64* slot_at(const ClusterDesc& cd, int par, int rank, int w, int i) {
+  // [parity][wave][value][rank]: one value of every member is a contiguous run
   return (u64*)cd.slots +
-         (((size_t)(par * cd.C + rank) * OFX_SLOT_WAVES + w) * 4 + i) * Tagged<cpx<R>>::NW;
+         ((((size_t)par * OFX_SLOT_WAVES + w) * 4 + i) * cd.C + rank) * Tagged<cpx<R>>::NW;
 }
 
 template <typename R>
@@ -829,7 +830,7 @@ OFX_DEV bool gather_err_and_rows(const ClusterDesc& cd, int par, const Geo<R>& g
   bool ok = read_tagged<C>(
       ns + nr + nk, tag,
       [&](int idx) -> TagPtr {
-        if (idx < ns) return TagPtr{slot_at<R>(cd, par, idx / nrec, idx % nrec, 0), 1, 0};
+        if (idx < ns) return TagPtr{slot_at<R>(cd, par, idx & (g.Cn - 1), idx >> g.lgC, 0), 1, 0};
         idx -= ns;
         if (idx < nr)
           return tp_plane<C>(g.plain, t1, g.n, t1_index(g, idx & (g.n1 - 1), g.k20 + (idx >> g.lg1)));
