@@ -142,7 +142,7 @@ OFX_DEV cpx<float> expi(double x, float) {
                     1.0f / 24.0f),
            -0.5f),
       1.0f);
-  const int k = (int)q & 3;   // exp(i q pi/2) = i^q
+  const int k = (int)fma(-4.0, floor(0.25 * q), q);   // q mod 4 (exact): exp(i q pi/2) = i^k
   const float re = k == 0 ? c : (k == 1 ? -s : (k == 2 ? -c : s));
   const float im = k == 0 ? s : (k == 1 ? c : (k == 2 ? -s : -c));
   return mkc<float>(re, im);

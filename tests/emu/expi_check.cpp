@@ -1,7 +1,7 @@
 // Accuracy of the kernels' exp(i x) from an FP64 argument (ofx_pass_impl.h,
 // compiled as plain C++ with OFX_EMU): prints the largest |expi(x) - e^{ix}|
-// over a sample of x for the FP32 and the FP64 forms, e^{ix} from long double
-// cosl / sinl.  Test infrastructure; never part of the product.
+// over a sample of x (up to 1e10) for the FP32 and the FP64 forms, e^{ix}
+// from long double cosl / sinl.  Test infrastructure; never part of the product.
 #include <math.h>
 #include <stdio.h>
 
@@ -15,7 +15,8 @@ int main() {
     s ^= s >> 7;
     s ^= s << 17;
     const double u = (double)(s >> 11) * (1.0 / 9007199254740992.0) - 0.5;
-    const double x = (i % 4 == 0) ? u * 8.0 : u * 2.0e4;   // small and large arguments
+    // small, large and (one in 16) very large arguments, beyond an int's quadrant count
+    const double x = (i % 16 == 1) ? u * 2.0e10 : ((i % 4 == 0) ? u * 8.0 : u * 2.0e4);
     const long double c = cosl((long double)x), sn = sinl((long double)x);
     const ofx::cpx<float> f = ofx::expi(x, 0.0f);
     const ofx::cpx<double> d = ofx::expi(x, 0.0);

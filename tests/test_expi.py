@@ -2,7 +2,7 @@
 FP32 form reduces x to a quadrant and |r| <= pi/4 in FP64 (Cody-Waite with
 fma) and sums the Taylor polynomials of sin r and cos r in FP32; the FP64
 form reduces modulo 2 pi and calls sin / cos.  Pinned against long-double
-cos / sin over 4 million arguments up to 1e4 rad (tests/emu/expi_check.cpp,
+cos / sin over 4 million arguments up to 1e10 rad (tests/emu/expi_check.cpp,
 the kernels' header compiled for the CPU)."""
 import os
 import subprocess
