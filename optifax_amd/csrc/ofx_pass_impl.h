@@ -108,7 +108,6 @@ OFX_DEV void stg(OFX_GLOBAL cpx<R>* p, I i, cpx<R> v) {
   p[i].y = v.y;
 }
 
-
 // x - 2*pi*round(x/(2*pi)) with a two-term (Cody-Waite) split of 2*pi; the
 // fma makes n*TWO_PI_HI exact, so the reduction is accurate for |x| < 2^40.
 OFX_DEV double reduce_2pi(double x) {
@@ -125,7 +124,7 @@ OFX_DEV double reduce_2pi(double x) {
 // two-part pi/2 in FP64 (Cody & Waite's reduction; the fma keeps q * hi
 // exact), then the Taylor polynomials of sin r (to r^9) and cos r (to r^10)
 // in FP32, whose truncation stays below 2e-9 on |r| <= pi/4, rotated by q's
-// quadrant.  Largest error of exp(ix) over |x| <= 1e4: 9.6e-8 (FP32 ulp(1)
+// quadrant.  Largest error of exp(ix) over |x| <= 1e10: 9.4e-8 (FP32 ulp(1)
 // is 1.2e-7).
 OFX_DEV cpx<float> expi(double x, float) {
   const double PIO2_HI = 1.5707963267948966, PIO2_LO = 6.123233995736766e-17;
@@ -743,12 +742,6 @@ OFX_DEV double dw_of(int k, int n, double dt) {
   return TWO_PI * (double)(k < n / 2 ? k : k - n) / ((double)n * dt);
 }
 
-// Forward transform of the own time columns held in `src` (layout lc*sT + r,
-// own columns lc in [HW, HW+W)) into the tagged buffer t1 (row-major
-// k2*n1 + c, twiddled).
-// Returns the buffer (src or other) that holds the transforms: data read
-// from the exchange may go only to the other one, since a thread can
-// complete its reads while another is still reading these.
 // Entry of sample (column c, row k2) in the transpose buffer t1.  Plain
 // exchanges: row-major k2*n1 + c.  Tagged exchanges: member-blocked,
 // ((c / W) n2 + k2) W + c % W, so a member's columns are one contiguous run
@@ -759,6 +752,11 @@ OFX_DEV size_t t1_index(const Geo<R>& g, int c, int k2) {
   return (((size_t)(c >> g.lgW) * g.n2 + k2) << g.lgW) + (c & (g.W - 1));
 }
 
+// Forward transform of the own time columns held in `src` (layout lc*sT + r,
+// own columns lc in [HW, HW+W)) into the buffer t1 (t1_index, twiddled).
+// Returns the buffer (src or other) that holds the transforms: data read
+// from the exchange may go only to the other one, since a thread can
+// complete its reads while another is still reading these.
 template <typename R>
 OFX_DEV cpx<R>* fwd_cols_to_t1(const Geo<R>& g, cpx<R>* src, cpx<R>* other, u64* t1,
                                unsigned tag) {
@@ -870,18 +868,20 @@ OFX_DEV bool gather_err_and_rows(const ClusterDesc& cd, int par, const Geo<R>& g
 #endif
 }
 
-// Inverse length-n1 transforms of the own rows, conjugate twiddle, write the
-// buffer t2.  Tagged exchanges (clusters of 64 and more, a row or two per
-// member) use row-major k2*n1 + c, as t1: a member writes its rows as
-// contiguous runs and the column owners read runs of LC samples per row;
-// plain exchanges (small clusters, many rows per member) use column-major
-// c*n2 + k2, where both sides move runs of Rr and n2 samples.
+// Entry of sample (column c, row k2) in the transpose buffer t2.  Tagged
+// exchanges (clusters of 64 and more, a row or two per member): row-major
+// k2*n1 + c, so a member writes its rows as contiguous runs and the column
+// owners read runs of LC samples per row.  Plain exchanges (small clusters,
+// many rows per member): column-major c*n2 + k2, where both sides move runs
+// of Rr and n2 samples.
 template <typename R>
 OFX_DEV size_t t2_index(const Geo<R>& g, int c, int k2) {
   return g.plain ? (size_t)c * g.n2 + k2 : (size_t)k2 * g.n1 + c;
 }
 
-// Returns the buffer (src or other) holding the transforms (see fwd_cols_to_t1).
+// Inverse length-n1 transforms of the own rows, conjugate twiddle, write the
+// buffer t2.  Returns the buffer (src or other) holding the transforms (see
+// fwd_cols_to_t1).
 template <typename R>
 OFX_DEV cpx<R>* rows_inv_to_t2(const Geo<R>& g, cpx<R>* src, cpx<R>* other, u64* t2,
                                unsigned tag) {
