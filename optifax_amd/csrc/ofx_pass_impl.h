@@ -108,6 +108,25 @@ OFX_DEV void stg(OFX_GLOBAL cpx<R>* p, I i, cpx<R> v) {
   p[i].y = v.y;
 }
 
+// A 4- or 8-byte value that is the same in every lane, marked as such (the
+// compiler then keeps it in scalar registers instead of one per lane)
+template <typename T>
+OFX_DEV T uniform(T v) {
+#ifdef OFX_EMU
+  return v;
+#else
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "4- or 8-byte values only");
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+  } else {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned long long lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)u);
+    const unsigned long long hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(u >> 32));
+    return __builtin_bit_cast(T, (hi << 32) | lo);
+  }
+#endif
+}
+
 // x - 2*pi*round(x/(2*pi)) with a two-term (Cody-Waite) split of 2*pi; the
 // fma makes n*TWO_PI_HI exact, so the reduction is accurate for |x| < 2^40.
 OFX_DEV double reduce_2pi(double x) {
@@ -1263,6 +1282,16 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
   long long attempts = 0, accepted = 0;
   const int trip_base = d.trips_done[0];
   const bool fresh = d.fresh[0] != 0;
+  // Descriptor fields the step loop reads, copied once: the descriptor is in
+  // global memory and the loop's stores may alias it, so every use inside
+  // the loop would be a new load on the step's critical path (the coupling
+  // G(k) was two dependent loads at the start of each step).
+  const double* const Gd = uniform(d.G);
+  const double bdiffw0 = uniform(d.bdiffw0), hmin = uniform(d.h);
+  const double max_err = uniform(d.max_err), min_err = uniform(d.min_err);
+  float* const stepmods = uniform(d.stepmods);
+  C* const apft = uniform((C*)d.apft);
+  const int n_cols = uniform(d.n_cols);
 
   // load the crystal-input field with halo columns
   for (int idx = OFX_TID; idx < g.LC * n2; idx += OFX_NT) {
@@ -1307,11 +1336,13 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
       chunk = (int)nChunks + 1;  // skip the Heun loop
     }
     // ---------------- adaptive Heun pass (NEE_CPU.m:39-94) ----------------
+    // G(k) of the step: loaded when the previous step was accepted, so the
+    // load is in flight during that step's frequency phase and exchange
+    double Gk = (double)chunk <= nChunks ? uniform(gl(Gd)[(int)k - 1]) : 0.0;
     while ((double)chunk <= nChunks) {
-      const double Gk = (double)gl(d.G)[(int)k - 1];
-      const double bdwz0 = d.bdiffw0 * (k - 1.0) * d.h;
-      const double bdwz1 = d.bdiffw0 * (k - 1.0 + stepmod) * d.h;
-      const R gk = (R)(d.h * stepmod * Gk);
+      const double bdwz0 = bdiffw0 * (k - 1.0) * hmin;
+      const double bdwz1 = bdiffw0 * (k - 1.0 + stepmod) * hmin;
+      const R gk = (R)(hmin * stepmod * Gk);
       C* A0 = g.B0;
       C* X = g.B1;
       C* K = g.B2;
@@ -1397,19 +1428,21 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
         break;
       }
       const double pcterr = 100.0 * gmax;
-      if (pcterr > d.max_err && stepmod > 1.0) {
+      if (pcterr > max_err && stepmod > 1.0) {
         stepmod -= 1.0;
         continue;  // reject: A0 is untouched in B0
       }
       // ---------------- accept ----------------
       accepted++;
       trip_acc++;
-      if (rank == 0 && d.stepmods) gl(d.stepmods)[(int)k - 1] = (float)stepmod;
+      if (rank == 0 && stepmods) gl(stepmods)[(int)k - 1] = (float)stepmod;
       const double s_used = stepmod;
       double ns = stepmod;
-      if (pcterr < d.min_err) ns += 1.0;
+      if (pcterr < min_err) ns += 1.0;
       if (k < nsteps && (k + ns) > nsteps) ns = (double)nsteps - k;
       const double knew = k + ns;
+      // the next step's G(k) (index clamped: past the last step it is not used)
+      const double Gnext = gl(Gd)[((int)knew < nsteps ? (int)knew : nsteps) - 1];
       int cend = chunk;  // chunks [chunk, cend) end with this step
       while ((double)cend <= nChunks && !(knew < cend * sel && knew <= nsteps)) cend++;
       // frequency phase: rows, dispersion, snapshots, inverse rows
@@ -1423,10 +1456,10 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
         const R am = al == R(1) ? R(1) : (al == R(0) ? R(0) : (R)pow(al, (R)s_used));
         v = cscale(cmul(v, ph), am);
         F[k2l * g.sF + k1] = v;
-        if (d.apft) {
-          OFX_GLOBAL C* ap = gl((C*)d.apft);
+        if (apft) {
+          OFX_GLOBAL C* ap = gl(apft);
           for (int cc2 = chunk; cc2 < cend; cc2++)
-            if (cc2 - 1 < d.n_cols) stg(ap, (size_t)(cc2 - 1) * n + kk, v);
+            if (cc2 - 1 < n_cols) stg(ap, (size_t)(cc2 - 1) * n + kk, v);
         }
       };
 #pragma unroll
@@ -1452,6 +1485,7 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
       k = knew;
       stepmod = ns;
       chunk = cend;
+      Gk = uniform(Gnext);
     }
     if (status != ST_RUNNING) break;
     // A plain pass (trips == 0) is not a round trip: it leaves the trip
