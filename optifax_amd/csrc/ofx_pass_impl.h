@@ -1284,8 +1284,8 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
   const bool fresh = d.fresh[0] != 0;
   // Descriptor fields the step loop reads, copied once: the descriptor is in
   // global memory and the loop's stores may alias it, so every use inside
-  // the loop would be a new load on the step's critical path (the coupling
-  // G(k) was two dependent loads at the start of each step).
+  // the loop would be a new load on the step's critical path (for the
+  // coupling G(k), two dependent loads at the start of each step).
   const double* const Gd = uniform(d.G);
   const double bdiffw0 = uniform(d.bdiffw0), hmin = uniform(d.h);
   const double max_err = uniform(d.max_err), min_err = uniform(d.min_err);
