@@ -21,8 +21,8 @@ HIP_FLAGS = [
     "-Wno-unused-result", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-LIB_SOURCES = ["ofx_backend_hip.hip", "ofx_runtime.cpp", "ofx_mlp.hip", "ofx_qpm.hip",
-               "ofx_gnlse.hip"]
+LIB_SOURCES = ["ofx_backend_hip.hip", "ofx_backend_rkl.hip", "ofx_runtime.cpp", "ofx_mlp.hip",
+               "ofx_qpm.hip", "ofx_gnlse.hip"]
 
 
 def _run(cmd):

@@ -5,6 +5,11 @@
 #include <stddef.h>
 #include "ofx_internal.h"
 
+// Bit 1 of a launch's xmode (bit 0 is the exchange mode): the kernel variant
+// whose RK23 pass reads the cluster descriptor from a copy in LDS, used when
+// every instance of the launch is RK23 (ofx_backend_rkl.hip).
+enum { XMODE_RK23_LDS = 2 };
+
 namespace ofx {
 
 struct DeviceInfo {

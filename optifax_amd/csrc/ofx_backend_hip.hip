@@ -24,8 +24,14 @@ __global__ void __launch_bounds__(512) pass_kernel(LaunchArgs a) {
   cluster_main<R, XM>(a, (int)blockIdx.x, reinterpret_cast<cpx<R>*>(ofx_lds_raw), red, sflag);
 }
 
-// the kernel instantiation for a precision and exchange mode
+// the RK23 variant's kernels (ofx_backend_rkl.hip)
+const void* rkl_kernel(int precision, int xmode);
+
+// the kernel instantiation for a precision and xmode (exchange mode in bit 0,
+// XMODE_RK23_LDS in bit 1)
 static const void* pass_fn(int precision, int xmode) {
+  if (xmode & XMODE_RK23_LDS) return rkl_kernel(precision, xmode & 1);
+  xmode &= 1;
   if (precision == OFX_F64)
     return xmode == XM_BARRIER ? (const void*)pass_kernel<double, XM_BARRIER>
                                : (const void*)pass_kernel<double, XM_TAGS>;
@@ -137,7 +143,7 @@ int be_create(int device, Backend** out, DeviceInfo* info) {
     const int dyn = info->lds_max - 1024;
     bool ok = true;
     for (int prec : {OFX_F32, OFX_F64})
-      for (int xm : {XM_TAGS, XM_BARRIER})
+      for (int xm : {XM_TAGS, XM_BARRIER, XM_TAGS | XMODE_RK23_LDS, XM_BARRIER | XMODE_RK23_LDS})
         ok = ok && hipFuncSetAttribute(pass_fn(prec, xm), hipFuncAttributeMaxDynamicSharedMemorySize,
                                        dyn) == hipSuccess;
     if (!ok)

@@ -1174,12 +1174,31 @@ OFX_DEV int rk23_pass(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc
   return ST_RUNNING;
 }
 
+// OFX_RKL (defined by the translation unit that builds the kernel's RK23
+// variant, ofx_backend_rkl.hip): the RK23 pass reads the cluster's
+// descriptor from a copy in LDS instead of global memory, where each use is
+// a new load since the kernel's stores may alias it.  The copy is faster for
+// the RK23 pass and slower for the Heun pass (whose register allocation it
+// moves), so the variant lives in a code object of its own.
+#ifdef OFX_RKL
+#define OFX_RKL_PARAM , const ClusterDesc& cdr
+#define OFX_RKL_ARG , cdr
+#define OFX_RK23_CD cdr
+#define OFX_CDL_PARAM , ClusterDesc* cdl
+#else
+#define OFX_RKL_PARAM
+#define OFX_RKL_ARG
+#define OFX_RK23_CD cd
+#define OFX_CDL_PARAM
+#endif
+
 // ------------------------------------------------------------------------
 // One instance: single pass (trips == 0) or round trips.
 // ------------------------------------------------------------------------
 template <typename R, int XM>
 OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const InstDesc& d,
-                          int rank, cpx<R>* lds, double* red, int* sflag, SyncState& sy) {
+                          int rank, cpx<R>* lds, double* red, int* sflag,
+                          SyncState& sy OFX_RKL_PARAM) {
   typedef cpx<R> C;
   typedef Tagged<C> TT;
   // converged in an earlier launch (and not rewritten since): nothing to do
@@ -1330,7 +1349,8 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
     if (a.trips > 0 && rank == 0 && d.stepmods)
       for (int j = OFX_TID; j < nsteps; j += OFX_NT) gl(d.stepmods)[j] = 0.0f;
     if (d.method == METHOD_RK23) {
-      status = rk23_pass<R, MP, XM>(a, cd, d, g, rank, red, sy, breg, areg, trip_att, trip_acc);
+      status = rk23_pass<R, MP, XM>(a, OFX_RK23_CD, d, g, rank, red, sy, breg, areg, trip_att,
+                                    trip_acc);
       attempts += trip_att;
       accepted += trip_acc;
       chunk = (int)nChunks + 1;  // skip the Heun loop
@@ -1599,11 +1619,18 @@ OFX_DEV void run_instance(const LaunchArgs& a, const ClusterDesc& cd, const Inst
   }
 }
 
+// (with OFX_RKL, cdl: workgroup-shared storage for the descriptor's copy)
 template <typename R, int XM>
-OFX_DEV void cluster_main(const LaunchArgs& a, int wg, cpx<R>* lds, double* red, int* sflag) {
+OFX_DEV void cluster_main(const LaunchArgs& a, int wg, cpx<R>* lds, double* red,
+                          int* sflag OFX_CDL_PARAM) {
   const int ci = a.wg_cluster[wg];
   if (ci < 0) return;
   const ClusterDesc& cd = a.clusters[ci];
+#ifdef OFX_RKL
+  if (OFX_TID == 0) *cdl = cd;
+  OFX_SYNC();
+  const ClusterDesc& cdr = *cdl;
+#endif
   const int rank = a.wg_rank[wg];
   SyncState sy;
 #ifdef OFX_EMU
@@ -1624,7 +1651,7 @@ OFX_DEV void cluster_main(const LaunchArgs& a, int wg, cpx<R>* lds, double* red,
 #endif
   for (int ii = 0; ii < cd.n_inst; ii++) {
     const int id = a.inst_order[cd.first_inst + ii];
-    run_instance<R, XM>(a, cd, a.inst[id], rank, lds, red, sflag, sy);
+    run_instance<R, XM>(a, cd, a.inst[id], rank, lds, red, sflag, sy OFX_RKL_ARG);
     if (flag_load(cd.abort_flag)) break;
   }
   // the tags this member used end at base + nx: the next launch starts past

@@ -410,6 +410,9 @@ int ofx_opo_create(ofx_context* ctx, int precision, int n_inst, const ofx_opo_sp
   }
   r->lds = lds;
   r->xmode = exchange_mode(exchange, cmin);
+  bool all_rk23 = n_inst > 0;
+  for (int i = 0; i < n_inst; i++) all_rk23 = all_rk23 && specs[i].pass.method == OFX_METHOD_RK23;
+  if (all_rk23) r->xmode |= XMODE_RK23_LDS;
   const int bpc = be_blocks_per_cu(ctx->be, precision, r->xmode, r->block, lds);
   if (bpc < 1) {
     delete r;

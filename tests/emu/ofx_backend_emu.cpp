@@ -79,6 +79,9 @@ int be_d2h(Backend*, void* dst, const void* src, size_t bytes) { memcpy(dst, src
 int be_sync(Backend*) { return OFX_OK; }
 int be_blocks_per_cu(Backend*, int, int, int, size_t lds) { return lds <= 160 * 1024 ? 1 : 0; }
 
+// the RK23 variant's workgroup body (ofx_backend_emu_rkl.cpp)
+void (*emu_rkl_fn(int precision, int xmode))(const LaunchArgs*, int, size_t);
+
 template <typename R, int XM>
 static void run_wg(const LaunchArgs* a, int wg, size_t lds) {
   std::vector<unsigned char> mem(lds + 64);
@@ -100,13 +103,14 @@ int be_launch(Backend* b, const LaunchArgs& a, int precision, int xmode, int gri
       ep[2 * j + 1] = 0u;
     }
     std::vector<std::thread> th;
+    const bool bar = (xmode & 1) == XM_BARRIER;
     for (int wg = 0; wg < grid; wg++) {
-      if (precision == OFX_F64)
-        th.emplace_back(xmode == XM_BARRIER ? run_wg<double, XM_BARRIER> : run_wg<double, XM_TAGS>,
-                        &a, wg, lds);
+      if (xmode & XMODE_RK23_LDS)
+        th.emplace_back(emu_rkl_fn(precision, xmode & 1), &a, wg, lds);
+      else if (precision == OFX_F64)
+        th.emplace_back(bar ? run_wg<double, XM_BARRIER> : run_wg<double, XM_TAGS>, &a, wg, lds);
       else
-        th.emplace_back(xmode == XM_BARRIER ? run_wg<float, XM_BARRIER> : run_wg<float, XM_TAGS>,
-                        &a, wg, lds);
+        th.emplace_back(bar ? run_wg<float, XM_BARRIER> : run_wg<float, XM_TAGS>, &a, wg, lds);
     }
     for (auto& t : th) t.join();
   }
